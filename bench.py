@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the full space-charge step (deposit + solve + interpolate) on one MI355X.
+
+Workload (the single-bunch GPU configuration): 4,000,000 particles, a 50/50
+mix of a Gaussian (sigma = 1, 1, 3 mm) and a uniform ellipsoid (semi-axes 2,
+2, 6 mm), 1 nC total in equal macro-charges, 128^3 mesh with bounds from the
+particle extrema, gamma = 1, free space.  The mesh is built once, outside the
+timing, as in the reference benchmark (benchmark/full_pipeline_benchmark.jl:
+25-30); each timed step is deposit! + solve! + interpolate_field with inputs
+resident in HBM.
+
+Prints one JSON line.  `value` is particles/s of the whole job (one GPU).
+The `roofline` object is for the kernel with the largest share of the step
+time, measured with the library's HIP-event timers on the launching stream
+over a second, instrumented pass of the same steps.  The `cpu_baseline`
+object times the CPU oracle (a C restatement of the reference CPU path) on a
+bounded sample of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "spacecharge.jl_amd"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def make_particles(n, dtype, device, seed=42):
+    """50/50 Gaussian (1, 1, 3 mm) + uniform ellipsoid (2, 2, 6 mm), generated on the GPU."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    ng = n // 2
+    nu = n - ng
+    sig = torch.tensor([1e-3, 1e-3, 3e-3], dtype=torch.float64, device=device)
+    gauss = torch.randn((ng, 3), generator=g, dtype=torch.float64, device=device) * sig
+    semi = torch.tensor([2e-3, 2e-3, 6e-3], dtype=torch.float64, device=device)
+    parts = []
+    have = 0
+    while have < nu:  # rejection sampling in the bounding box
+        u = torch.rand((2 * (nu - have) + 1024, 3), generator=g, dtype=torch.float64,
+                       device=device) * 2 - 1
+        u = u[(u * u).sum(1) <= 1.0]
+        parts.append(u)
+        have += u.shape[0]
+    ell = torch.cat(parts)[:nu] * semi
+    p = torch.cat([gauss, ell]).to(dtype)
+    x, y, z = (p[:, i].contiguous() for i in range(3))
+    q = torch.full((n,), 1e-9 / n, dtype=dtype, device=device)
+    return x, y, z, q
+
+
+def kernel_bytes(name, n, grid, s, ps):
+    """Algorithmic HBM bytes of one launch of a step kernel (compulsory I/O)."""
+    nx, ny, nz = grid
+    M = nx * ny * nz
+    kx = nx + 1
+    spec = nz * 2 * ny * kx * 2 * s  # one complex spectrum [nz][2ny][nx+1]
+    if name == "deposit":
+        return 4 * n * ps + M * s
+    if name == "interpolate":
+        return 6 * n * ps + 3 * M * s
+    if name == "rho_fwd_x":
+        return M * s + ny * nz * kx * 2 * s
+    if name == "rho_fwd_y":
+        return ny * nz * kx * 2 * s + spec
+    if name == "zconv":
+        return spec + 3 * (nz + 1) * (ny + 1) * kx * s + 3 * spec
+    if name == "inv_y":
+        return 3 * spec + 3 * ny * nz * kx * 2 * s
+    if name == "inv_x":
+        return 3 * ny * nz * kx * 2 * s + 3 * M * s
+    return None
+
+
+def cpu_baseline(args, n, grid, dtype_name, max_seconds):
+    """Time the CPU oracle on a bounded sample: whole steps of the same workload."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as orc
+    npdt = np.float32 if dtype_name == "f32" else np.float64
+    x, y, z, q = make_particles(n, torch.float32 if dtype_name == "f32" else torch.float64,
+                                "cuda", seed=42)
+    x, y, z, q = (t.cpu().numpy() for t in (x, y, z, q))
+    threads = os.cpu_count() or 1
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid, npdt)
+    times = []
+    t_start = time.perf_counter()
+    while True:
+        t0 = time.perf_counter()
+        rho = orc.deposit(x, y, z, q, grid, lo, dl, npdt)
+        ef = orc.solve(rho, lo, hi, dl, 1.0, False, nthreads=threads)
+        orc.interpolate(ef, x, y, z, lo, dl, nthreads=threads)
+        times.append(time.perf_counter() - t0)
+        if time.perf_counter() - t_start > max_seconds or len(times) >= 10:
+            break
+    best = min(times)
+    return {
+        "value": n / best,
+        "unit": "particles/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{len(times)} full steps ({dtype_name}, {n} particles, {grid[0]}^3), "
+                   f"min {best * 1e3:.1f} ms, median {sorted(times)[len(times) // 2] * 1e3:.1f} ms; "
+                   "deposit single-threaded, FFT/Green/interpolate on all cores"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--particles", type=int, default=4_000_000)
+    ap.add_argument("--gamma", type=float, default=1.0)
+    ap.add_argument("--cathode", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="time budget of the CPU-oracle sample (0 disables it)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design note, not an implementation.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import spacecharge as sc
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    T = torch.float32 if args.dtype == "f32" else torch.float64
+    n = args.particles
+    grid = (args.grid, args.grid, args.grid)
+    x, y, z, q = make_particles(n, T, dev)
+    mesh = sc.Mesh3D(grid, x, y, z, T=T, gamma=args.gamma, total_charge=1e-9)
+
+    def step():
+        sc.deposit(mesh, x, y, z, q)
+        sc.solve(mesh, at_cathode=args.cathode)
+        return sc.interpolate_field(mesh, x, y, z)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: exactly `steps` steps, synchronised on both sides
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = elapsed * 1e3 / args.steps
+
+    # instrumented pass: per-kernel HIP events on the launching stream
+    with sc.KernelTimer(dev) as kt:
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        ms_instr = (time.perf_counter() - t1) * 1e3 / args.steps
+    per_kernel = kt.results()
+    s = 4 if T == torch.float32 else 8
+    tot = sum(v[0] for v in per_kernel.values())
+    dom = max(per_kernel.items(), key=lambda kv: kv[1][0])
+    dom_name, (dom_ms, dom_n) = dom
+    avg_s = dom_ms / dom_n * 1e-3
+    nbytes = kernel_bytes(dom_name, n, grid, s, s)
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom_name,
+        "achieved": (nbytes / avg_s / 1e9) if nbytes else None,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": (nbytes / avg_s / 1e9 / HBM_PEAK_GBS) if nbytes else None,
+        "traffic": None,
+        "algorithmic_bytes": nbytes,
+        "avg_launch_us": avg_s * 1e6,
+    }
+    breakdown = {k: round(v[0] / v[1] * 1e3, 2) for k, v in sorted(per_kernel.items(),
+                                                                      key=lambda kv: -kv[1][0])}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(args, n, grid, args.dtype, args.cpu_seconds)
+
+    out = {
+        "metric": "particles/s per full space-charge step (deposit+solve+interpolate)",
+        "value": n / (ms * 1e-3),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded Gaussian + uniform-ellipsoid bunch generated on the GPU)",
+        "config": {
+            "workload": (f"single bunch, {n} particles, {grid[0]}x{grid[1]}x{grid[2]} mesh, "
+                         f"gamma={args.gamma}, {'cathode' if args.cathode else 'free space'}"),
+            "particles": n,
+            "grid": list(grid),
+            "gamma": args.gamma,
+            "at_cathode": bool(args.cathode),
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernel_us_per_launch": breakdown,
+        "kernel_share": {k: round(v[0] / tot, 3) for k, v in per_kernel.items()},
+        "ms_per_step_instrumented": ms_instr,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,161 @@
+/*
+ * spacecharge_mi355.h -- C ABI of the MI355X (gfx950) space-charge step.
+ *
+ * This is the drop-in boundary for SpaceCharge.jl's particle-in-cell path:
+ * cloud-in-cell deposit, integrated-Green-function (IGF) solve by zero-padded
+ * 2N FFT convolution (free space and cathode image), trilinear interpolation.
+ * A host binding (Julia `ccall`, Python `ctypes`, C/C++) keeps the
+ * reference's API -- Mesh3D, deposit!, clear_mesh!, solve!, interpolate_field
+ * -- and forwards each call to one function below.
+ *
+ * Conventions
+ *   - Plain C types only.  Every device pointer is a HIP device address
+ *     (hipMalloc'd, or an allocation from a host framework such as torch).
+ *   - Grids use the reference's layout (src/mesh.jl:19-34): rho[i,j,k] at
+ *     i + nx*(j + ny*k) (x fastest); efield adds M*c for component c
+ *     (M = nx*ny*nz; components slowest); particles are one array per
+ *     coordinate.
+ *   - dtype codes are element sizes: SC_F32 = 4 (Float32), SC_F64 = 8
+ *     (Float64).  The mesh type T and the particle type may differ, as in
+ *     the reference; arithmetic uses the wider of the two.
+ *   - Mesh bounds, spacings and gamma cross as doubles that hold the mesh
+ *     type's values (a Float32 mesh passes Float32-representable doubles).
+ *   - All work is stream-ordered on the context's stream and asynchronous,
+ *     like the reference's GPU backend (no synchronisation inside deposit!,
+ *     solve! or interpolate_field).  Functions that return host values
+ *     (sc_bounds_from_particles) synchronise the stream.
+ *   - Calls on one context must come from one host thread at a time;
+ *     different contexts are independent.
+ *   - Status: 0 on success, otherwise an SC_ERR_* code; the message is
+ *     available from sc_last_error(ctx) (thread-local when ctx is NULL).
+ */
+#ifndef SPACECHARGE_MI355_H
+#define SPACECHARGE_MI355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SC_F32 4
+#define SC_F64 8
+
+#define SC_OK 0
+#define SC_ERR_INVALID 1     /* bad argument (message says which)           */
+#define SC_ERR_DEVICE 2      /* HIP runtime error                            */
+#define SC_ERR_NOMEM 3       /* device allocation failed                     */
+#define SC_ERR_UNSUPPORTED 4 /* valid request this build does not implement  */
+
+typedef struct sc_ctx sc_ctx;
+typedef struct sc_mesh sc_mesh;
+
+/* ---- context --------------------------------------------------------------
+ * No reference counterpart as a value: the reference selects its backend by
+ * dispatch on the array type (src/deposition.jl:234-246) and launches on the
+ * task-local stream.  A context pins one device and one stream. */
+int sc_ctx_create(int device, sc_ctx **out);
+int sc_ctx_destroy(sc_ctx *ctx);
+/* Use an existing HIP stream (a hipStream_t; NULL = the null stream). */
+int sc_ctx_set_stream(sc_ctx *ctx, void *stream);
+void *sc_ctx_get_stream(sc_ctx *ctx);
+/* Blocks until all work queued on the context's stream has finished
+ * (the reference's callers use CUDA.synchronize(),
+ * benchmark/full_pipeline_benchmark.jl:47). */
+int sc_synchronize(sc_ctx *ctx);
+const char *sc_last_error(sc_ctx *ctx);
+int sc_device_count(int *count);
+
+/* ---- device memory, for hosts without a device-array type ----------------
+ * Replaces KernelAbstractions.zeros / similar on the GPU backend
+ * (src/mesh.jl:159-160, src/interpolation.jl:110-112). */
+int sc_alloc(sc_ctx *ctx, size_t bytes, void **out);
+int sc_free(sc_ctx *ctx, void *ptr);
+int sc_memcpy_h2d(sc_ctx *ctx, void *dst, const void *src, size_t bytes);
+int sc_memcpy_d2h(sc_ctx *ctx, void *dst, const void *src, size_t bytes);
+int sc_memcpy_d2d(sc_ctx *ctx, void *dst, const void *src, size_t bytes);
+int sc_memset(sc_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ---- mesh -----------------------------------------------------------------
+ * Replaces Mesh3D{T,A,B} (src/mesh.jl:19-34) and its lazily created solver
+ * workspace (_get_workspace, src/mesh.jl:49-72).  sc_mesh_create allocates
+ * and zeroes rho (M elements) and efield (3M elements) like
+ * KernelAbstractions.zeros (src/mesh.jl:159-160, 222-223);
+ * sc_mesh_create_with_buffers wraps caller-owned device buffers of those
+ * sizes instead.  The workspace is allocated on the first sc_solve and freed
+ * by sc_mesh_destroy.  Errors: grid < 2 in any axis ("All elements of
+ * grid_size must be at least 2.", src/mesh.jl:106-108). */
+int sc_mesh_create(sc_ctx *ctx, int dtype, int64_t nx, int64_t ny, int64_t nz, sc_mesh **out);
+int sc_mesh_create_with_buffers(sc_ctx *ctx, int dtype, int64_t nx, int64_t ny, int64_t nz,
+                                void *rho, void *efield, sc_mesh **out);
+int sc_mesh_destroy(sc_mesh *mesh);
+void *sc_mesh_rho(sc_mesh *mesh);
+void *sc_mesh_efield(sc_mesh *mesh);
+/* Bytes of device workspace the mesh currently holds (0 before the first solve). */
+int64_t sc_mesh_workspace_bytes(sc_mesh *mesh);
+/* Opt-in reuse of the transformed Green function between solves whose
+ * (delta, gamma, at_cathode, zmin+zmax) are bit-identical.  Off by default,
+ * so every sc_solve rebuilds it as solve! does (src/solvers/free_space.jl:79-86). */
+int sc_mesh_set_green_reuse(sc_mesh *mesh, int enable);
+
+/* Replaces the particle-extrema constructor's bounds computation
+ * (src/mesh.jl:95-174): device min/max of x, y, z, then
+ *   delta = (max - min)/(n - 1) in the particle type,
+ *   min -= 1e-6*delta, max += 1e-6*delta in Float64,
+ *   delta = (max - min)/(n - 1), a zero delta becomes 1e-6,
+ *   all three rounded to the mesh type.
+ * Synchronises the stream.  Errors: grid < 2, n == 0 ("Particle arrays
+ * cannot be empty."). */
+int sc_bounds_from_particles(sc_ctx *ctx, int pdtype, const void *x, const void *y, const void *z,
+                             int64_t n, const int64_t grid[3], int dtype, double min_bounds[3],
+                             double max_bounds[3], double delta[3]);
+
+/* Replaces clear_mesh! (src/deposition.jl:10-12): rho = 0. */
+int sc_clear_mesh(sc_mesh *mesh);
+
+/* Replaces deposit! (src/deposition.jl:218-247) and its GPU kernel
+ * (deposit_kernel!, :106-129; CIC core :28-86): for every particle, q times
+ * the trilinear weights is added to the 8 surrounding nodes of rho.
+ * clear != 0 zeroes rho first (:230-232).  Particles whose lower cell index
+ * falls outside [0, n-2] are clamped to it (extrapolating weights, charge
+ * still conserved) instead of writing outside the mesh. */
+int sc_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+               const void *q, int64_t n, const double min_bounds[3], const double delta[3],
+               int clear);
+
+/* Replaces solve!(mesh; at_cathode) (src/solvers/free_space.jl:14-47) and
+ * solve_freespace! (:56-101): efield[c] = FPEI * (rho (*) G_c) with the
+ * integrated Green function of src/green_functions.jl:35-112 built for
+ * (delta, gamma), as a linear convolution on the 2N zero-padded grid.  With
+ * at_cathode != 0 the image charge -rho flipped in z, at z-offset
+ * min_bounds[2] + max_bounds[2] (:34-45), is added.  The Green function is
+ * evaluated and differenced in Float64 for both mesh types; the transforms
+ * and products run in the mesh type. */
+int sc_solve(sc_mesh *mesh, const double min_bounds[3], const double max_bounds[3],
+             const double delta[3], double gamma, int at_cathode);
+
+/* Replaces interpolate_field (src/interpolation.jl:100-128) and
+ * interpolate_kernel! (:17-86): Ex/Ey/Ez[p] = sum over the 8 surrounding
+ * nodes of the trilinear weight times efield, written in the particle type
+ * (caller-allocated, n elements each).  Same clamp policy as sc_deposit. */
+int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                   int64_t n, const double min_bounds[3], const double delta[3], void *Ex,
+                   void *Ey, void *Ez);
+
+/* ---- per-kernel timing (no reference counterpart; the reference profiles
+ * with CUDA.@profile, benchmark/full_pipeline_profile.jl:108-111) ----------
+ * When enabled, every kernel launch on the context is bracketed by HIP
+ * events on the context's stream.  sc_timing_read synchronises, folds the
+ * recorded events into per-kernel totals and returns them. */
+int sc_timing_enable(sc_ctx *ctx, int enable);
+int sc_timing_reset(sc_ctx *ctx);
+int sc_kernel_count(void);
+const char *sc_kernel_name(int kernel_id);
+int sc_timing_read(sc_ctx *ctx, int kernel_id, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPACECHARGE_MI355_H */

@@ -1,0 +1,181 @@
+"""ctypes front end of the CPU oracle (libsc_oracle.so, built from sc_oracle.c).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is a plain-C restatement of the
+SpaceCharge.jl v1.2.0 CPU path; tests compare the HIP library against it and
+bench.py times it as the CPU baseline.  The product (the HIP library and its
+host layer) never imports this module.
+
+Arrays follow the reference's Julia layout: ``rho`` is a Fortran-ordered
+(nx, ny, nz) array, ``efield`` a Fortran-ordered (nx, ny, nz, 3) array, so
+``rho[i, j, k]`` is Julia's ``rho[i+1, j+1, k+1]``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libsc_oracle.so")
+_lib = None
+
+F32, F64 = 4, 8
+FPEI = 299792458.0 ** 2 * 1.0e-7  # src/utils.jl:7-8
+
+
+def build() -> str:
+    """Compile libsc_oracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    vp, ll, i, d = ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_double
+    pd = ctypes.POINTER(ctypes.c_double)
+    pll = ctypes.POINTER(ctypes.c_longlong)
+    lib.sco_bounds_auto.argtypes = [i, vp, vp, vp, ll, pll, i, pd, pd, pd]
+    lib.sco_bounds_manual.argtypes = [pd, pd, pll, i, pd, pd, pd]
+    lib.sco_deposit.argtypes = [i, i, vp, vp, vp, vp, ll, pll, pd, pd, vp]
+    lib.sco_interpolate.argtypes = [i, i, vp, vp, vp, vp, ll, pll, pd, pd, vp, vp, vp, i]
+    lib.sco_solve.argtypes = [i, pll, pd, pd, pd, d, i, vp, vp, i]
+    lib.sco_field_green_function_f64.argtypes = [d, d, d]
+    lib.sco_field_green_function_f64.restype = d
+    lib.sco_field_green_function_f32.argtypes = [ctypes.c_float] * 3
+    lib.sco_field_green_function_f32.restype = ctypes.c_float
+    lib.sco_green_lattice.argtypes = [i, pll, pd, d, i, pd, vp, i]
+    lib.sco_fft3d.argtypes = [i, pll, vp, i, i]
+    lib.sco_fpei.restype = d
+    _lib = lib
+    return lib
+
+
+def _code(dtype) -> int:
+    dt = np.dtype(dtype)
+    if dt == np.float32:
+        return F32
+    if dt == np.float64:
+        return F64
+    raise TypeError(f"unsupported dtype {dt}")
+
+
+def _d3(v):
+    return (ctypes.c_double * 3)(*[float(a) for a in v])
+
+
+def _ll3(v):
+    return (ctypes.c_longlong * 3)(*[int(a) for a in v])
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleError(Exception):
+    pass
+
+
+def bounds_auto(x, y, z, grid, T=np.float64):
+    """src/mesh.jl:95-174: (min_bounds, max_bounds, delta) as mesh-type values."""
+    lib = _load()
+    x = np.ascontiguousarray(x)
+    y = np.ascontiguousarray(y, dtype=x.dtype)
+    z = np.ascontiguousarray(z, dtype=x.dtype)
+    lo, hi, dl = _d3([0] * 3), _d3([0] * 3), _d3([0] * 3)
+    rc = lib.sco_bounds_auto(_code(x.dtype), _ptr(x), _ptr(y), _ptr(z), x.size, _ll3(grid),
+                             _code(T), lo, hi, dl)
+    if rc != 0:
+        raise OracleError(f"bounds_auto failed ({rc})")
+    return tuple(lo), tuple(hi), tuple(dl)
+
+
+def bounds_manual(mn, mx, grid, T=np.float64):
+    """src/mesh.jl:196-238."""
+    lib = _load()
+    lo, hi, dl = _d3([0] * 3), _d3([0] * 3), _d3([0] * 3)
+    rc = lib.sco_bounds_manual(_d3(mn), _d3(mx), _ll3(grid), _code(T), lo, hi, dl)
+    if rc != 0:
+        raise OracleError(f"bounds_manual failed ({rc})")
+    return tuple(lo), tuple(hi), tuple(dl)
+
+
+def deposit(x, y, z, q, grid, lo, delta, T=np.float64, rho=None):
+    """src/deposition.jl:28-86,167-197: serial CIC deposit, accumulating into rho."""
+    lib = _load()
+    x = np.ascontiguousarray(x)
+    y, z, q = (np.ascontiguousarray(a, dtype=x.dtype) for a in (y, z, q))
+    if rho is None:
+        rho = np.zeros(tuple(grid), dtype=T, order="F")
+    assert rho.flags.f_contiguous and rho.dtype == np.dtype(T)
+    rc = lib.sco_deposit(_code(x.dtype), _code(T), _ptr(x), _ptr(y), _ptr(z), _ptr(q), x.size,
+                         _ll3(grid), _d3(lo), _d3(delta), _ptr(rho))
+    if rc != 0:
+        raise OracleError(f"deposit failed ({rc})")
+    return rho
+
+
+def interpolate(efield, x, y, z, lo, delta, nthreads=0):
+    """src/interpolation.jl:17-128: (Ex, Ey, Ez) in the particle dtype."""
+    lib = _load()
+    assert efield.flags.f_contiguous and efield.ndim == 4 and efield.shape[3] == 3
+    x = np.ascontiguousarray(x)
+    y, z = (np.ascontiguousarray(a, dtype=x.dtype) for a in (y, z))
+    out = [np.empty_like(x) for _ in range(3)]
+    rc = lib.sco_interpolate(_code(x.dtype), _code(efield.dtype), _ptr(efield), _ptr(x), _ptr(y),
+                             _ptr(z), x.size, _ll3(efield.shape[:3]), _d3(lo), _d3(delta),
+                             _ptr(out[0]), _ptr(out[1]), _ptr(out[2]), nthreads)
+    if rc != 0:
+        raise OracleError(f"interpolate failed ({rc})")
+    return tuple(out)
+
+
+def solve(rho, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0):
+    """src/solvers/free_space.jl:14-101: efield, Fortran-ordered (nx, ny, nz, 3)."""
+    lib = _load()
+    assert rho.flags.f_contiguous and rho.ndim == 3
+    ef = np.zeros(rho.shape + (3,), dtype=rho.dtype, order="F")
+    rc = lib.sco_solve(_code(rho.dtype), _ll3(rho.shape), _d3(lo), _d3(hi), _d3(delta),
+                       float(gamma), int(bool(at_cathode)), _ptr(rho), _ptr(ef), nthreads)
+    if rc != 0:
+        raise OracleError(f"solve failed ({rc})")
+    return ef
+
+
+def field_green_function(x, y, z, T=np.float64):
+    """src/green_functions.jl:35-38, evaluated in T."""
+    lib = _load()
+    if np.dtype(T) == np.float32:
+        return np.float32(lib.sco_field_green_function_f32(x, y, z))
+    return lib.sco_field_green_function_f64(x, y, z)
+
+
+def green_lattice(grid, delta, gamma, icomp, offset=(0.0, 0.0, 0.0), T=np.float64,
+                  nthreads=0):
+    """src/green_functions.jl:41-112: the (2nx,2ny,2nz) differenced lattice (real part)."""
+    lib = _load()
+    shape = tuple(2 * int(g) for g in grid)
+    cdt = np.complex64 if np.dtype(T) == np.float32 else np.complex128
+    out = np.zeros(shape, dtype=cdt, order="F")
+    rc = lib.sco_green_lattice(_code(T), _ll3(grid), _d3(delta), float(gamma), int(icomp),
+                               _d3(offset), _ptr(out), nthreads)
+    if rc != 0:
+        raise OracleError(f"green_lattice failed ({rc})")
+    return out
+
+
+def fft3d(a, sign=-1, nthreads=0):
+    """Unnormalised in-place 3-D transform of a Fortran-ordered complex array (returns it)."""
+    lib = _load()
+    assert a.flags.f_contiguous and a.ndim == 3
+    T = np.float32 if a.dtype == np.complex64 else np.float64
+    rc = lib.sco_fft3d(_code(T), _ll3(a.shape), _ptr(a), int(sign), nthreads)
+    if rc != 0:
+        raise OracleError(f"fft3d failed ({rc})")
+    return a

@@ -1,0 +1,470 @@
+// sc_api.cpp -- the extern "C" boundary of libspacecharge_mi355.so.
+//
+// Validation, error strings, contexts, meshes and the per-kernel timers.
+// Error messages that have a reference counterpart reuse its wording
+// (src/mesh.jl:106-116,206-211; src/deposition.jl:226-228) so a host
+// binding can re-raise them unchanged.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "sc_internal.h"
+
+namespace sc {
+
+const char *const kKernelNames[K_COUNT] = {
+    "clear",        "deposit",     "interpolate", "bounds_partial", "bounds_final",
+    "green_x",      "green_y",     "green_z",     "rho_fwd_x",      "rho_fwd_y",
+    "zconv",        "inv_y",       "inv_x",
+};
+
+namespace {
+thread_local std::string g_err;  // errors without a context
+}
+
+int set_err(Ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    else g_err = msg;
+    return code;
+}
+
+int hip_err(Ctx *ctx, hipError_t e, const char *what)
+{
+    std::string m = std::string("HIP error ") + hipGetErrorName(e) + " (" + hipGetErrorString(e) +
+                    ") in " + what;
+    return set_err(ctx, e == hipErrorOutOfMemory ? SC_ERR_NOMEM : SC_ERR_DEVICE, m);
+}
+
+static hipEvent_t take_event(Ctx *ctx)
+{
+    if (!ctx->pool.empty()) {
+        hipEvent_t e = ctx->pool.back();
+        ctx->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void launch_begin(Ctx *ctx, int kid)
+{
+    if (!ctx->timing) return;
+    Ctx::Rec r;
+    r.kid = kid;
+    r.a = take_event(ctx);
+    r.b = take_event(ctx);
+    if (!r.a || !r.b) return;
+    (void)hipEventRecord(r.a, ctx->stream);
+    ctx->pending.push_back(r);
+}
+
+void launch_end(Ctx *ctx, int kid)
+{
+    if (!ctx->timing || ctx->pending.empty()) return;
+    Ctx::Rec &r = ctx->pending.back();
+    if (r.kid != kid) return;
+    (void)hipEventRecord(r.b, ctx->stream);
+}
+
+static void fold_timings(Ctx *ctx)
+{
+    if (ctx->pending.empty()) return;
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &r : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            ctx->total_ms[r.kid] += ms;
+            ctx->launches[r.kid] += 1;
+        }
+        ctx->pool.push_back(r.a);
+        ctx->pool.push_back(r.b);
+    }
+    ctx->pending.clear();
+}
+
+static bool dtype_ok(int dt) { return dt == SC_F32 || dt == SC_F64; }
+
+static int check_dev(Ctx *ctx)
+{
+    SC_HIP(ctx, hipSetDevice(ctx->device));
+    return SC_OK;
+}
+
+}  // namespace sc
+
+using namespace sc;
+
+extern "C" {
+
+int sc_device_count(int *count)
+{
+    if (!count) return set_err(nullptr, SC_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_err(nullptr, e, "hipGetDeviceCount");
+    *count = n;
+    return SC_OK;
+}
+
+int sc_ctx_create(int device, sc_ctx **out)
+{
+    if (!out) return set_err(nullptr, SC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return hip_err(nullptr, e, "hipGetDeviceCount");
+    if (device < 0 || device >= n)
+        return set_err(nullptr, SC_ERR_INVALID,
+                       "device " + std::to_string(device) + " out of range (" + std::to_string(n) +
+                           " visible)");
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_err(nullptr, e, "hipSetDevice");
+    sc_ctx *c = new (std::nothrow) sc_ctx();
+    if (!c) return set_err(nullptr, SC_ERR_NOMEM, "host allocation failed");
+    c->c.device = device;
+    c->c.stream = nullptr;
+    *out = c;
+    return SC_OK;
+}
+
+int sc_ctx_destroy(sc_ctx *ctx)
+{
+    if (!ctx) return SC_OK;
+    Ctx *c = &ctx->c;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &r : c->pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c->pool) (void)hipEventDestroy(e);
+    if (c->red) (void)hipFree(c->red);
+    delete ctx;
+    return SC_OK;
+}
+
+int sc_ctx_set_stream(sc_ctx *ctx, void *stream)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // pending timer events belong to the old stream: fold them before switching
+    if (s != ctx->c.stream && !ctx->c.pending.empty()) fold_timings(&ctx->c);
+    ctx->c.stream = s;
+    return SC_OK;
+}
+
+void *sc_ctx_get_stream(sc_ctx *ctx) { return ctx ? (void *)ctx->c.stream : nullptr; }
+
+int sc_synchronize(sc_ctx *ctx)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    SC_HIP(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+    return SC_OK;
+}
+
+const char *sc_last_error(sc_ctx *ctx) { return ctx ? ctx->c.err.c_str() : g_err.c_str(); }
+
+// ---- memory ----
+int sc_alloc(sc_ctx *ctx, size_t bytes, void **out)
+{
+    if (!ctx || !out) return set_err(ctx ? &ctx->c : nullptr, SC_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    if (int rc = check_dev(&ctx->c)) return rc;
+    if (bytes == 0) return SC_OK;
+    SC_HIP(&ctx->c, hipMalloc(out, bytes));
+    return SC_OK;
+}
+
+int sc_free(sc_ctx *ctx, void *ptr)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (!ptr) return SC_OK;
+    SC_HIP(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+    SC_HIP(&ctx->c, hipFree(ptr));
+    return SC_OK;
+}
+
+int sc_memcpy_h2d(sc_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (bytes == 0) return SC_OK;
+    if (!dst || !src) return set_err(&ctx->c, SC_ERR_INVALID, "NULL pointer");
+    SC_HIP(&ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->c.stream));
+    SC_HIP(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+    return SC_OK;
+}
+
+int sc_memcpy_d2h(sc_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (bytes == 0) return SC_OK;
+    if (!dst || !src) return set_err(&ctx->c, SC_ERR_INVALID, "NULL pointer");
+    SC_HIP(&ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->c.stream));
+    SC_HIP(&ctx->c, hipStreamSynchronize(ctx->c.stream));
+    return SC_OK;
+}
+
+int sc_memcpy_d2d(sc_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (bytes == 0) return SC_OK;
+    if (!dst || !src) return set_err(&ctx->c, SC_ERR_INVALID, "NULL pointer");
+    SC_HIP(&ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->c.stream));
+    return SC_OK;
+}
+
+int sc_memset(sc_ctx *ctx, void *dst, int value, size_t bytes)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (bytes == 0) return SC_OK;
+    if (!dst) return set_err(&ctx->c, SC_ERR_INVALID, "NULL pointer");
+    SC_HIP(&ctx->c, hipMemsetAsync(dst, value, bytes, ctx->c.stream));
+    return SC_OK;
+}
+
+// ---- mesh ----
+static int mesh_new(sc_ctx *ctx, int dtype, int64_t nx, int64_t ny, int64_t nz, void *rho,
+                    void *efield, bool owns, sc_mesh **out)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    Ctx *c = &ctx->c;
+    if (!out) return set_err(c, SC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!dtype_ok(dtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (nx <= 1 || ny <= 1 || nz <= 1)
+        return set_err(c, SC_ERR_INVALID, "All elements of grid_size must be at least 2.");
+    if (nx > 1 << 20 || ny > 1 << 20 || nz > 1 << 20 || nx * ny * nz > (int64_t)1 << 34)
+        return set_err(c, SC_ERR_INVALID, "grid_size too large");
+    if (int rc = check_dev(c)) return rc;
+    sc_mesh *m = new (std::nothrow) sc_mesh();
+    if (!m) return set_err(c, SC_ERR_NOMEM, "host allocation failed");
+    Mesh &M = m->m;
+    M.ctx = c;
+    M.dtype = dtype;
+    M.n[0] = nx;
+    M.n[1] = ny;
+    M.n[2] = nz;
+    const size_t cells = (size_t)(nx * ny * nz);
+    const size_t s = (size_t)dtype;
+    if (owns) {
+        if (hipMalloc(&M.rho, cells * s) != hipSuccess ||
+            hipMalloc(&M.efield, 3 * cells * s) != hipSuccess) {
+            if (M.rho) (void)hipFree(M.rho);
+            delete m;
+            return set_err(c, SC_ERR_NOMEM, "device allocation of rho/efield failed");
+        }
+        M.owns = true;
+        int rc = launch_clear(c, M.rho, (int64_t)(cells * s));
+        if (rc == SC_OK) rc = launch_clear(c, M.efield, (int64_t)(3 * cells * s));
+        if (rc != SC_OK) {
+            (void)hipFree(M.rho);
+            (void)hipFree(M.efield);
+            delete m;
+            return rc;
+        }
+    } else {
+        if (!rho || !efield) {
+            delete m;
+            return set_err(c, SC_ERR_INVALID, "rho and efield buffers must be non-NULL");
+        }
+        M.rho = rho;
+        M.efield = efield;
+        M.owns = false;
+    }
+    *out = m;
+    return SC_OK;
+}
+
+int sc_mesh_create(sc_ctx *ctx, int dtype, int64_t nx, int64_t ny, int64_t nz, sc_mesh **out)
+{
+    return mesh_new(ctx, dtype, nx, ny, nz, nullptr, nullptr, true, out);
+}
+
+int sc_mesh_create_with_buffers(sc_ctx *ctx, int dtype, int64_t nx, int64_t ny, int64_t nz,
+                                void *rho, void *efield, sc_mesh **out)
+{
+    return mesh_new(ctx, dtype, nx, ny, nz, rho, efield, false, out);
+}
+
+int sc_mesh_destroy(sc_mesh *mesh)
+{
+    if (!mesh) return SC_OK;
+    Mesh &M = mesh->m;
+    (void)hipSetDevice(M.ctx->device);
+    (void)hipStreamSynchronize(M.ctx->stream);
+    free_workspace(&M);
+    if (M.owns) {
+        (void)hipFree(M.rho);
+        (void)hipFree(M.efield);
+    }
+    delete mesh;
+    return SC_OK;
+}
+
+void *sc_mesh_rho(sc_mesh *mesh) { return mesh ? mesh->m.rho : nullptr; }
+void *sc_mesh_efield(sc_mesh *mesh) { return mesh ? mesh->m.efield : nullptr; }
+int64_t sc_mesh_workspace_bytes(sc_mesh *mesh) { return mesh ? mesh->m.ws.bytes : 0; }
+
+int sc_mesh_set_green_reuse(sc_mesh *mesh, int enable)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    mesh->m.green_reuse = enable != 0;
+    if (!enable) mesh->m.ws.g_valid = false;
+    return SC_OK;
+}
+
+// ---- bounds (src/mesh.jl:95-174) ----
+int sc_bounds_from_particles(sc_ctx *ctx, int pdtype, const void *x, const void *y, const void *z,
+                             int64_t n, const int64_t grid[3], int dtype, double lo[3],
+                             double hi[3], double delta[3])
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    Ctx *c = &ctx->c;
+    if (!grid || !lo || !hi || !delta) return set_err(c, SC_ERR_INVALID, "NULL argument");
+    if (!dtype_ok(pdtype) || !dtype_ok(dtype))
+        return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    for (int a = 0; a < 3; ++a)
+        if (grid[a] <= 1)
+            return set_err(c, SC_ERR_INVALID, "All elements of grid_size must be at least 2.");
+    if (n <= 0) return set_err(c, SC_ERR_INVALID, "Particle arrays cannot be empty.");
+    if (!x || !y || !z) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
+    if (int rc = check_dev(c)) return rc;
+    double mn[3], mx[3];
+    if (int rc = launch_bounds(c, pdtype, x, y, z, n, mn, mx)) return rc;
+    for (int a = 0; a < 3; ++a) {
+        const double nm1 = (double)(grid[a] - 1);
+        // first spacing in the particle type (src/mesh.jl:127-129)
+        double d0;
+        if (pdtype == SC_F32) d0 = (double)(((float)mx[a] - (float)mn[a]) / (float)nm1);
+        else d0 = (mx[a] - mn[a]) / nm1;
+        // Float64 padding (:132-137), spacing again (:140-142), zero patch (:145-149)
+        const double l = mn[a] - 1e-6 * d0;
+        const double h = mx[a] + 1e-6 * d0;
+        double d = (h - l) / nm1;
+        if (d == 0.0) d = 1e-6;
+        // conversion to the mesh type (:150-156)
+        if (dtype == SC_F32) {
+            lo[a] = (double)(float)l;
+            hi[a] = (double)(float)h;
+            delta[a] = (double)(float)d;
+        } else {
+            lo[a] = l;
+            hi[a] = h;
+            delta[a] = d;
+        }
+    }
+    return SC_OK;
+}
+
+// ---- the step ----
+static int check_geom(Ctx *c, const double lo[3], const double dl[3])
+{
+    if (!lo || !dl) return set_err(c, SC_ERR_INVALID, "NULL bounds or delta");
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(lo[a]) || !std::isfinite(dl[a]) || !(dl[a] > 0.0))
+            return set_err(c, SC_ERR_INVALID, "bounds must be finite and delta positive");
+    }
+    return SC_OK;
+}
+
+int sc_clear_mesh(sc_mesh *mesh)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    if (int rc = check_dev(M.ctx)) return rc;
+    return launch_clear(M.ctx, M.rho, M.n[0] * M.n[1] * M.n[2] * M.dtype);
+}
+
+int sc_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+               const void *q, int64_t n, const double lo[3], const double dl[3], int clear)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+    if (n > 0 && (!x || !y || !z || !q)) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
+    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_dev(c)) return rc;
+    if (clear) {
+        if (int rc = launch_clear(c, M.rho, M.n[0] * M.n[1] * M.n[2] * M.dtype)) return rc;
+    }
+    return launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl);
+}
+
+int sc_solve(sc_mesh *mesh, const double lo[3], const double hi[3], const double dl[3],
+             double gamma, int at_cathode)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (!hi) return set_err(c, SC_ERR_INVALID, "NULL max_bounds");
+    if (!std::isfinite(gamma) || !(gamma > 0.0))
+        return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
+    if (at_cathode && !std::isfinite(hi[2]))
+        return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
+    if (int rc = check_dev(c)) return rc;
+    return solve_mesh(&M, lo, hi, dl, gamma, at_cathode);
+}
+
+int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                   int64_t n, const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+    if (n > 0 && (!x || !y || !z || !Ex || !Ey || !Ez))
+        return set_err(c, SC_ERR_INVALID, "NULL particle or output pointer");
+    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_dev(c)) return rc;
+    return launch_interp(&M, pdtype, x, y, z, n, lo, dl, Ex, Ey, Ez);
+}
+
+// ---- timers ----
+int sc_timing_enable(sc_ctx *ctx, int enable)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (!enable) fold_timings(&ctx->c);
+    ctx->c.timing = enable != 0;
+    return SC_OK;
+}
+
+int sc_timing_reset(sc_ctx *ctx)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    fold_timings(&ctx->c);
+    for (int k = 0; k < K_COUNT; ++k) {
+        ctx->c.total_ms[k] = 0.0;
+        ctx->c.launches[k] = 0;
+    }
+    return SC_OK;
+}
+
+int sc_kernel_count(void) { return K_COUNT; }
+
+const char *sc_kernel_name(int kernel_id)
+{
+    if (kernel_id < 0 || kernel_id >= K_COUNT) return "";
+    return kKernelNames[kernel_id];
+}
+
+int sc_timing_read(sc_ctx *ctx, int kernel_id, double *total_ms, int64_t *launches)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (kernel_id < 0 || kernel_id >= K_COUNT)
+        return set_err(&ctx->c, SC_ERR_INVALID, "kernel id out of range");
+    fold_timings(&ctx->c);
+    if (total_ms) *total_ms = ctx->c.total_ms[kernel_id];
+    if (launches) *launches = ctx->c.launches[kernel_id];
+    return SC_OK;
+}
+
+}  // extern "C"

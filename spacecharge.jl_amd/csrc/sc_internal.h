@@ -1,0 +1,157 @@
+// sc_internal.h -- shared declarations of the gfx950 space-charge library.
+//
+// Host-side objects (context, mesh, workspace), error plumbing, the kernel
+// registry used by the per-kernel timers, and the launch helpers that every
+// .hip translation unit uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "spacecharge_mi355.h"
+
+namespace sc {
+
+// ---------------------------------------------------------------------------
+// Kernel registry (indices are stable; names are what rocprofv3 shows).
+enum KernelId : int {
+    K_CLEAR = 0,
+    K_DEPOSIT,
+    K_INTERP,
+    K_BOUNDS_PARTIAL,
+    K_BOUNDS_FINAL,
+    K_GREEN_X,
+    K_GREEN_Y,
+    K_GREEN_Z,
+    K_RHO_FWD_X,
+    K_RHO_FWD_Y,
+    K_ZCONV,
+    K_INV_Y,
+    K_INV_X,
+    K_COUNT
+};
+extern const char *const kKernelNames[K_COUNT];
+
+// ---------------------------------------------------------------------------
+// Mixed-radix plan for one transform length.  Radices are 2, 3, 4, 5, 7, 8
+// (3, 5, 7 use the butterfly from the DFT definition).  Twiddles live in device memory,
+// tw[m] = exp(-2 pi i m / n) in the transform's precision.
+constexpr int kMaxStages = 24;
+struct FftPlan {
+    int n = 0;
+    int nst = 0;
+    int rad[kMaxStages] = {};
+};
+
+// Complex elements each lane holds in registers during one transform stage:
+// a batch of nb pencils of length n in precision T needs
+// nb*n <= fft_elems<T>() * kFftThreads.
+constexpr int kFftThreads = 256;
+template <typename T>
+constexpr int fft_elems()
+{
+    return sizeof(T) == 4 ? 8 : 4;
+}
+template <typename T>
+constexpr int fft_cap()
+{
+    return fft_elems<T>() * kFftThreads;
+}
+constexpr int kFftMaxRadix = 7;
+
+bool make_plan(int n, FftPlan *p);  // false if n has a prime factor > kFftMaxRadix
+
+// ---------------------------------------------------------------------------
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool timing = false;
+    struct Rec {
+        int kid;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[K_COUNT] = {};
+    int64_t launches[K_COUNT] = {};
+    // bounds-reduction partials, allocated on first use
+    double *red = nullptr;
+};
+
+// Device workspace of one mesh (lazy, like src/mesh.jl:49-72).
+struct Workspace {
+    bool ready = false;
+    int dtype = 0;
+    int64_t bytes = 0;
+    // spectra in the mesh type T: A = rho after the x and y transforms,
+    // B = three field spectra, each [nz][2ny][nx+1] complex, kx fastest.
+    void *A = nullptr;
+    void *B = nullptr;
+    // transformed Green functions in T: free space [nz+1][ny+1][nx+1] real x3,
+    // image (cathode) [2nz][ny+1][nx+1] complex x3.
+    void *Gf = nullptr;
+    void *Gi = nullptr;
+    // Float64 scratch for the Green build: [2nz][ny+1][nx+1].
+    double *H = nullptr;
+    // twiddle tables: T precision for the field transforms, double for the
+    // Green transforms, for the lengths 2nx, 2ny, 2nz.
+    void *twT[3] = {nullptr, nullptr, nullptr};
+    double *twD[3] = {nullptr, nullptr, nullptr};
+    FftPlan plan[3];
+    // cached Green key for the opt-in reuse
+    bool g_valid = false;
+    double g_key[6] = {};
+    int g_cath = 0;
+};
+
+struct Mesh {
+    Ctx *ctx = nullptr;
+    int dtype = 0;
+    int64_t n[3] = {0, 0, 0};
+    void *rho = nullptr;
+    void *efield = nullptr;
+    bool owns = false;
+    bool green_reuse = false;
+    Workspace ws;
+};
+
+// ---------------------------------------------------------------------------
+// Errors
+int set_err(Ctx *ctx, int code, const std::string &msg);
+int hip_err(Ctx *ctx, hipError_t e, const char *what);
+
+#define SC_HIP(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return ::sc::hip_err((ctx), e_, #call); \
+    } while (0)
+
+// Launch bracketing for the per-kernel timers.
+void launch_begin(Ctx *ctx, int kid);
+void launch_end(Ctx *ctx, int kid);
+
+// Kernel-launch entry points (defined in the .hip files).
+int launch_clear(Ctx *ctx, void *p, int64_t bytes);
+int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
+                   int64_t n, const double lo[3], const double dl[3]);
+int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                  const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez);
+int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                  double mn[3], double mx[3]);
+int solve_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
+               int at_cathode);
+int ensure_workspace(Mesh *m);
+void free_workspace(Mesh *m);
+
+}  // namespace sc
+
+struct sc_ctx {
+    sc::Ctx c;
+};
+struct sc_mesh {
+    sc::Mesh m;
+};

@@ -1,0 +1,293 @@
+// sc_particles.hip -- particle <-> mesh kernels on gfx950.
+//
+//  * deposit: cloud-in-cell (src/deposition.jl:28-86, GPU kernel :106-129).
+//  * interpolate: trilinear gather (src/interpolation.jl:17-86).
+//  * bounds: min/max reduction for the particle-extrema mesh constructor
+//    (src/mesh.jl:120-122).
+//
+// Per-particle arithmetic keeps the reference's evaluation order without
+// FMA contraction, so a particle's eight deposit contributions and its
+// interpolated fields are bit-identical to the CPU path; only the order in
+// which the atomics add up differs.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "sc_internal.h"
+
+namespace sc {
+
+namespace {
+
+template <typename A, typename B>
+using wider_t = typename std::conditional<(sizeof(A) >= sizeof(B)), A, B>::type;
+
+// Lower cell index of a normalised coordinate, clamped to [0, n-2] (NaN -> 0).
+template <typename C>
+__device__ __forceinline__ int cell_index(C norm, int n)
+{
+    const C f = floor(norm);
+    if (!(f >= (C)0)) return 0;
+    if (f > (C)(n - 2)) return n - 2;
+    return (int)f;
+}
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(256)
+k_deposit(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
+          const PT *__restrict__ pq, int64_t n, MT *__restrict__ rho, int nx, int ny, int nz,
+          double lx_, double ly_, double lz_, double dx_, double dy_, double dz_)
+{
+#pragma clang fp contract(off)
+    using C = wider_t<PT, MT>;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const C lx = (C)(MT)lx_, ly = (C)(MT)ly_, lz = (C)(MT)lz_;
+    const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
+    const C nxn = ((C)px[p] - lx) / ddx;
+    const C nyn = ((C)py[p] - ly) / ddy;
+    const C nzn = ((C)pz[p] - lz) / ddz;
+    const int ix = cell_index(nxn, nx), iy = cell_index(nyn, ny), iz = cell_index(nzn, nz);
+    const C dx = nxn - (C)ix, dy = nyn - (C)iy, dz = nzn - (C)iz;
+    const C wx0 = (C)1 - dx, wx1 = dx, wy0 = (C)1 - dy, wy1 = dy, wz0 = (C)1 - dz, wz1 = dz;
+    const C c = (C)pq[p];
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    MT *b = rho + ix + sy * iy + sz * iz;
+    atomicAdd(b, (MT)(((c * wx0) * wy0) * wz0));
+    atomicAdd(b + 1, (MT)(((c * wx1) * wy0) * wz0));
+    atomicAdd(b + sy, (MT)(((c * wx0) * wy1) * wz0));
+    atomicAdd(b + sy + 1, (MT)(((c * wx1) * wy1) * wz0));
+    atomicAdd(b + sz, (MT)(((c * wx0) * wy0) * wz1));
+    atomicAdd(b + sz + 1, (MT)(((c * wx1) * wy0) * wz1));
+    atomicAdd(b + sz + sy, (MT)(((c * wx0) * wy1) * wz1));
+    atomicAdd(b + sz + sy + 1, (MT)(((c * wx1) * wy1) * wz1));
+}
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(256)
+k_interp(const MT *__restrict__ ef, const PT *__restrict__ px, const PT *__restrict__ py,
+         const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz, double lx_, double ly_,
+         double lz_, double dx_, double dy_, double dz_, PT *__restrict__ Ex, PT *__restrict__ Ey,
+         PT *__restrict__ Ez)
+{
+#pragma clang fp contract(off)
+    using C = wider_t<PT, MT>;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const C lx = (C)(MT)lx_, ly = (C)(MT)ly_, lz = (C)(MT)lz_;
+    const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
+    const C nxn = ((C)px[p] - lx) / ddx;
+    const C nyn = ((C)py[p] - ly) / ddy;
+    const C nzn = ((C)pz[p] - lz) / ddz;
+    const int ix = cell_index(nxn, nx), iy = cell_index(nyn, ny), iz = cell_index(nzn, nz);
+    const C dx = nxn - (C)ix, dy = nyn - (C)iy, dz = nzn - (C)iz;
+    const C one = (C)1;
+    const C w000 = ((one - dx) * (one - dy)) * (one - dz);
+    const C w100 = (dx * (one - dy)) * (one - dz);
+    const C w010 = ((one - dx) * dy) * (one - dz);
+    const C w110 = (dx * dy) * (one - dz);
+    const C w001 = ((one - dx) * (one - dy)) * dz;
+    const C w101 = (dx * (one - dy)) * dz;
+    const C w011 = ((one - dx) * dy) * dz;
+    const C w111 = (dx * dy) * dz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny, M = sz * nz;
+    const int64_t b = ix + sy * iy + sz * iz;
+    PT *out[3] = {Ex, Ey, Ez};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const MT *e = ef + c * M + b;
+        C s = (C)e[0] * w000 + (C)e[1] * w100;
+        s = s + (C)e[sy] * w010;
+        s = s + (C)e[sy + 1] * w110;
+        s = s + (C)e[sz] * w001;
+        s = s + (C)e[sz + 1] * w101;
+        s = s + (C)e[sz + sy] * w011;
+        s = s + (C)e[sz + sy + 1] * w111;
+        out[c][p] = (PT)s;
+    }
+}
+
+// ---- bounds: grid-stride min/max per block, then one block folds them ----
+constexpr int kRedBlocks = 1024;
+
+template <typename PT>
+__global__ void __launch_bounds__(256)
+k_bounds_partial(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
+                 int64_t n, double *__restrict__ part)
+{
+    __shared__ double sm[6][256];
+    PT mn[3], mx[3];
+    const PT *P[3] = {px, py, pz};
+    const int64_t first = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    const int64_t s0 = first < n ? first : 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) mn[a] = mx[a] = P[a][s0];
+    for (int64_t i = first; i < n; i += step) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const PT v = P[a][i];
+            mn[a] = v < mn[a] ? v : mn[a];
+            mx[a] = v > mx[a] ? v : mx[a];
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sm[a][threadIdx.x] = (double)mn[a];
+        sm[3 + a][threadIdx.x] = (double)mx[a];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double o = sm[a][threadIdx.x + w];
+                sm[a][threadIdx.x] = o < sm[a][threadIdx.x] ? o : sm[a][threadIdx.x];
+                const double O = sm[3 + a][threadIdx.x + w];
+                sm[3 + a][threadIdx.x] = O > sm[3 + a][threadIdx.x] ? O : sm[3 + a][threadIdx.x];
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[(size_t)blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_bounds_final(double *__restrict__ part, int nblk)
+{
+    __shared__ double sm[6][256];
+    for (int a = 0; a < 6; ++a) {
+        double v = part[a];
+        for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+            const double o = part[(size_t)i * 6 + a];
+            v = (a < 3) ? (o < v ? o : v) : (o > v ? o : v);
+        }
+        sm[a][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int a = 0; a < 6; ++a) {
+                const double o = sm[a][threadIdx.x + w];
+                double &t = sm[a][threadIdx.x];
+                t = (a < 3) ? (o < t ? o : t) : (o > t ? o : t);
+            }
+        }
+        __syncthreads();
+    }
+    // all reads of part[] happened before the first barrier; the final
+    // values go to a separate slot past the partials
+    if (threadIdx.x < 6) part[(size_t)nblk * 6 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_clear(uint32_t *__restrict__ p, int64_t nwords)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t step = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = i; k < nwords; k += step) p[k] = 0u;
+}
+
+int64_t blocks_for(int64_t n, int bs) { return (n + bs - 1) / bs; }
+
+template <typename PT, typename MT>
+void dep(Mesh *m, const void *x, const void *y, const void *z, const void *q, int64_t n,
+         const double lo[3], const double dl[3])
+{
+    hipLaunchKernelGGL((k_deposit<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+                       m->ctx->stream, (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q, n,
+                       (MT *)m->rho, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1], lo[2],
+                       dl[0], dl[1], dl[2]);
+}
+
+template <typename PT, typename MT>
+void itp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, const double lo[3],
+         const double dl[3], void *Ex, void *Ey, void *Ez)
+{
+    hipLaunchKernelGGL((k_interp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+                       m->ctx->stream, (const MT *)m->efield, (const PT *)x, (const PT *)y,
+                       (const PT *)z, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1],
+                       lo[2], dl[0], dl[1], dl[2], (PT *)Ex, (PT *)Ey, (PT *)Ez);
+}
+
+}  // namespace
+
+int launch_clear(Ctx *ctx, void *p, int64_t bytes)
+{
+    if (bytes <= 0) return SC_OK;
+    // all buffers cleared here are whole 4-byte words (float/double arrays)
+    const int64_t words = bytes / 4;
+    int64_t nb = blocks_for(words, 256);
+    if (nb > 4096) nb = 4096;
+    launch_begin(ctx, K_CLEAR);
+    hipLaunchKernelGGL(k_clear, dim3((unsigned)nb), dim3(256), 0, ctx->stream, (uint32_t *)p, words);
+    launch_end(ctx, K_CLEAR);
+    SC_HIP(ctx, hipGetLastError());
+    return SC_OK;
+}
+
+int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
+                   int64_t n, const double lo[3], const double dl[3])
+{
+    if (n <= 0) return SC_OK;
+    if ((n + 255) / 256 > 0x7fffffffLL)
+        return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
+    launch_begin(m->ctx, K_DEPOSIT);
+    if (pdt == SC_F32 && m->dtype == SC_F32) dep<float, float>(m, x, y, z, q, n, lo, dl);
+    else if (pdt == SC_F64 && m->dtype == SC_F64) dep<double, double>(m, x, y, z, q, n, lo, dl);
+    else if (pdt == SC_F64 && m->dtype == SC_F32) dep<double, float>(m, x, y, z, q, n, lo, dl);
+    else dep<float, double>(m, x, y, z, q, n, lo, dl);
+    launch_end(m->ctx, K_DEPOSIT);
+    SC_HIP(m->ctx, hipGetLastError());
+    return SC_OK;
+}
+
+int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                  const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez)
+{
+    if (n <= 0) return SC_OK;
+    if ((n + 255) / 256 > 0x7fffffffLL)
+        return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
+    launch_begin(m->ctx, K_INTERP);
+    if (pdt == SC_F32 && m->dtype == SC_F32) itp<float, float>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    else if (pdt == SC_F64 && m->dtype == SC_F64) itp<double, double>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    else if (pdt == SC_F64 && m->dtype == SC_F32) itp<double, float>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    else itp<float, double>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    launch_end(m->ctx, K_INTERP);
+    SC_HIP(m->ctx, hipGetLastError());
+    return SC_OK;
+}
+
+int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                  double mn[3], double mx[3])
+{
+    int64_t nblk = blocks_for(n, 256);
+    if (nblk > kRedBlocks) nblk = kRedBlocks;
+    if (!ctx->red) {
+        if (hipMalloc((void **)&ctx->red, sizeof(double) * 6 * (kRedBlocks + 1)) != hipSuccess)
+            return set_err(ctx, SC_ERR_NOMEM, "device allocation for the bounds reduction failed");
+    }
+    double *part = ctx->red;
+    launch_begin(ctx, K_BOUNDS_PARTIAL);
+    if (pdt == SC_F32)
+        hipLaunchKernelGGL(k_bounds_partial<float>, dim3((unsigned)nblk), dim3(256), 0, ctx->stream,
+                           (const float *)x, (const float *)y, (const float *)z, n, part);
+    else
+        hipLaunchKernelGGL(k_bounds_partial<double>, dim3((unsigned)nblk), dim3(256), 0, ctx->stream,
+                           (const double *)x, (const double *)y, (const double *)z, n, part);
+    launch_end(ctx, K_BOUNDS_PARTIAL);
+    launch_begin(ctx, K_BOUNDS_FINAL);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, ctx->stream, part, (int)nblk);
+    launch_end(ctx, K_BOUNDS_FINAL);
+    double h[6];
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h, part + 6 * nblk, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_err(ctx, e, "bounds reduction");
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = h[a];
+        mx[a] = h[3 + a];
+    }
+    return SC_OK;
+}
+
+}  // namespace sc

@@ -1,0 +1,309 @@
+"""spacecharge -- Python host layer of the MI355X space-charge step.
+
+Mirrors the SpaceCharge.jl API (src/SpaceCharge.jl:17):
+
+    Julia                                     Python
+    Mesh3D(grid, x, y, z; T, gamma, ...)      Mesh3D(grid, x, y, z, T=..., gamma=...)
+    Mesh3D(grid, min_b, max_b; T, ...)        Mesh3D(grid, min_b, max_b, T=...)
+    deposit!(mesh, x, y, z, q; clear=true)    deposit(mesh, x, y, z, q, clear=True)
+    clear_mesh!(mesh)                         clear_mesh(mesh)
+    solve!(mesh; at_cathode=false)            solve(mesh, at_cathode=False)
+    interpolate_field(mesh, x, y, z)          interpolate_field(mesh, x, y, z)
+
+Every call forwards to one function of the C ABI (include/spacecharge_mi355.h)
+through ctypes; the arithmetic runs in the hand-written gfx950 kernels of
+libspacecharge_mi355.so.  Device memory and streams come from torch: mesh
+arrays are torch tensors on the GPU, and each call runs on torch's current
+stream, asynchronously, like the reference's GPU backend.
+
+``mesh.rho`` is a (nx, ny, nz) tensor and ``mesh.efield`` a (nx, ny, nz, 3)
+tensor with the reference's column-major strides, so ``mesh.rho[i, j, k]``
+is Julia's ``mesh.rho[i+1, j+1, k+1]``.  Errors raise SpaceChargeError with
+the reference's messages where it has one.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Sequence, Tuple
+
+import torch
+
+from . import _lib
+from ._lib import SC_F32, SC_F64
+
+__all__ = [
+    "Mesh3D",
+    "deposit",
+    "clear_mesh",
+    "solve",
+    "interpolate_field",
+    "SpaceChargeError",
+    "CLIGHT",
+    "FPEI",
+    "synchronize",
+    "KernelTimer",
+]
+
+# src/utils.jl:7-8
+CLIGHT = 299792458.0
+FPEI = CLIGHT ** 2 * 1.0e-7
+
+
+class SpaceChargeError(Exception):
+    """Raised where the reference raises ErrorException (error(msg))."""
+
+
+_DT = {torch.float32: SC_F32, torch.float64: SC_F64}
+_ctxs = {}
+
+
+def _ctx(device: torch.device) -> ctypes.c_void_p:
+    idx = device.index if device.index is not None else torch.cuda.current_device()
+    c = _ctxs.get(idx)
+    if c is None:
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = lib.sc_ctx_create(idx, ctypes.byref(h))
+        if rc != 0:
+            raise SpaceChargeError(lib.sc_last_error(None).decode())
+        c = h
+        _ctxs[idx] = c
+    lib = _lib.load()
+    lib.sc_ctx_set_stream(c, ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream))
+    return c
+
+
+def _check(rc: int, ctx) -> None:
+    if rc != 0:
+        raise SpaceChargeError(_lib.load().sc_last_error(ctx).decode())
+
+
+def _default_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise SpaceChargeError("no GPU visible: the space-charge step runs on MI355X only")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_device(a, device: torch.device, dtype=None) -> torch.Tensor:
+    t = torch.as_tensor(a, device=device)
+    if dtype is not None and t.dtype != dtype:
+        t = t.to(dtype)
+    if t.dtype not in _DT:
+        t = t.to(torch.float64)
+    return t.contiguous().reshape(-1)
+
+
+def _to_T(v: float, T: torch.dtype) -> float:
+    if T == torch.float32:
+        return float(torch.tensor(v, dtype=torch.float32).item())
+    return float(v)
+
+
+def _check_grid(grid_size) -> Tuple[int, int, int]:
+    g = tuple(int(a) for a in grid_size)
+    if len(g) != 3:
+        raise SpaceChargeError("grid_size must have three elements.")
+    if any(a <= 1 for a in g):
+        raise SpaceChargeError("All elements of grid_size must be at least 2.")  # src/mesh.jl:106-108
+    return g
+
+
+class Mesh3D:
+    """A 3-D Cartesian mesh on one GPU (src/mesh.jl:19-34).
+
+    Mesh3D(grid_size, x, y, z, *, T=torch.float64, gamma=1.0, total_charge=0.0)
+        bounds from the particle extrema with the reference's 1e-6*delta
+        padding (src/mesh.jl:95-174);
+    Mesh3D(grid_size, min_bounds, max_bounds, *, T=..., gamma=..., total_charge=...)
+        manual bounds (src/mesh.jl:196-238).
+    """
+
+    def __init__(self, grid_size: Sequence[int], *args, T: torch.dtype = torch.float64,
+                 gamma: float = 1.0, total_charge: float = 0.0, device=None):
+        if T not in _DT:
+            raise SpaceChargeError("T must be torch.float32 or torch.float64")
+        g = _check_grid(grid_size)
+        if len(args) == 3:
+            px, py, pz = args
+            dev = torch.device(device) if device is not None else (
+                px.device if isinstance(px, torch.Tensor) and px.is_cuda else _default_device())
+            lens = [len(a) for a in (px, py, pz)]
+            if min(lens) == 0:
+                raise SpaceChargeError("Particle arrays cannot be empty.")  # :110-112
+            if not (lens[0] == lens[1] == lens[2]):
+                raise SpaceChargeError("Particle coordinate arrays must have the same length.")
+            x = _as_device(px, dev)
+            y = _as_device(py, dev, x.dtype)
+            z = _as_device(pz, dev, x.dtype)
+            lo, hi, dl = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+            ctx = _ctx(dev)
+            with torch.cuda.device(dev):
+                rc = _lib.load().sc_bounds_from_particles(
+                    ctx, _DT[x.dtype], x.data_ptr(), y.data_ptr(), z.data_ptr(), x.numel(),
+                    _lib.i3(g), _DT[T], lo, hi, dl)
+            _check(rc, ctx)
+            self.min_bounds = tuple(lo)
+            self.max_bounds = tuple(hi)
+            self.delta = tuple(dl)
+        elif len(args) == 2:
+            mn, mx = (tuple(float(v) for v in a) for a in args)
+            if len(mn) != 3 or len(mx) != 3:
+                raise SpaceChargeError("min_bounds and max_bounds must have three elements.")
+            if any(b <= a for a, b in zip(mn, mx)):
+                raise SpaceChargeError(
+                    "max_bounds must be strictly greater than min_bounds for all dimensions.")
+            dev = torch.device(device) if device is not None else _default_device()
+            lo = tuple(_to_T(v, T) for v in mn)
+            hi = tuple(_to_T(v, T) for v in mx)
+            # delta = (T(max) - T(min)) / (n - 1) in T (:219)
+            lo_t = torch.tensor(lo, dtype=T)
+            hi_t = torch.tensor(hi, dtype=T)
+            n1 = torch.tensor([a - 1 for a in g], dtype=T)
+            self.min_bounds, self.max_bounds = lo, hi
+            self.delta = tuple(float(v) for v in ((hi_t - lo_t) / n1).tolist())
+        else:
+            raise SpaceChargeError(
+                "Mesh3D takes (grid_size, x, y, z) or (grid_size, min_bounds, max_bounds)")
+        self.grid_size = g
+        self.T = T
+        self.device = dev
+        self.gamma = _to_T(gamma, T)
+        self.total_charge = _to_T(total_charge, T)
+        nx, ny, nz = g
+        M = nx * ny * nz
+        self._rho = torch.zeros(M, dtype=T, device=dev)
+        self._ef = torch.zeros(3 * M, dtype=T, device=dev)
+        self.rho = self._rho.as_strided((nx, ny, nz), (1, nx, nx * ny))
+        self.efield = self._ef.as_strided((nx, ny, nz, 3), (1, nx, nx * ny, M))
+        ctx = _ctx(dev)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(dev):
+            rc = _lib.load().sc_mesh_create_with_buffers(
+                ctx, _DT[T], nx, ny, nz, self._rho.data_ptr(), self._ef.data_ptr(), ctypes.byref(h))
+        _check(rc, ctx)
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib._lib is not None:
+            _lib._lib.sc_mesh_destroy(h)
+            self._h = None
+
+    def __repr__(self) -> str:  # mirrors Base.show (src/mesh.jl:240-246)
+        nx, ny, nz = self.grid_size
+        lo, hi = self.min_bounds, self.max_bounds
+        return (f"Mesh3D{{{str(self.T).replace('torch.', '')}, MI355X}} ({nx}x{ny}x{nz}) "
+                f"bounds=[({lo[0]},{lo[1]},{lo[2]}), ({hi[0]},{hi[1]},{hi[2]})] "
+                f"gamma={self.gamma}")
+
+    @property
+    def workspace_bytes(self) -> int:
+        """Device bytes held by the solver workspace (0 before the first solve)."""
+        return int(_lib.load().sc_mesh_workspace_bytes(self._h))
+
+    def set_green_reuse(self, enable: bool = True) -> None:
+        """Reuse the transformed Green function while delta/gamma/cathode are unchanged."""
+        _check(_lib.load().sc_mesh_set_green_reuse(self._h, int(bool(enable))), None)
+
+
+def _particles(mesh: Mesh3D, arrays, names) -> list:
+    out = []
+    first = None
+    for a in arrays:
+        t = _as_device(a, mesh.device, first)
+        if first is None:
+            first = t.dtype
+        out.append(t)
+    return out
+
+
+def clear_mesh(mesh: Mesh3D) -> None:
+    """clear_mesh! (src/deposition.jl:10-12): rho = 0."""
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        _check(_lib.load().sc_clear_mesh(mesh._h), ctx)
+
+
+def deposit(mesh: Mesh3D, particles_x, particles_y, particles_z, particles_q, *,
+            clear: bool = True) -> None:
+    """deposit! (src/deposition.jl:218-247): CIC charge deposit into mesh.rho."""
+    lens = [len(a) for a in (particles_x, particles_y, particles_z, particles_q)]
+    if not all(n == lens[0] for n in lens):
+        raise SpaceChargeError("Particle coordinate and charge arrays must have the same length.")
+    x, y, z, q = _particles(mesh, (particles_x, particles_y, particles_z, particles_q), "xyzq")
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_deposit(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
+                                    z.data_ptr(), q.data_ptr(), x.numel(),
+                                    _lib.d3(mesh.min_bounds), _lib.d3(mesh.delta), int(bool(clear)))
+    _check(rc, ctx)
+
+
+def solve(mesh: Mesh3D, *, at_cathode: bool = False) -> None:
+    """solve! (src/solvers/free_space.jl:14-47): mesh.efield from mesh.rho."""
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_solve(mesh._h, _lib.d3(mesh.min_bounds), _lib.d3(mesh.max_bounds),
+                                  _lib.d3(mesh.delta), float(mesh.gamma), int(bool(at_cathode)))
+    _check(rc, ctx)
+
+
+def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
+    """interpolate_field (src/interpolation.jl:100-128): (Ex, Ey, Ez) in the particle dtype."""
+    lens = [len(a) for a in (particles_x, particles_y, particles_z)]
+    if not all(n == lens[0] for n in lens):
+        raise SpaceChargeError("Particle coordinate arrays must have the same length.")
+    x, y, z = _particles(mesh, (particles_x, particles_y, particles_z), "xyz")
+    Ex, Ey, Ez = (torch.empty_like(x) for _ in range(3))
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_interpolate(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
+                                        z.data_ptr(), x.numel(), _lib.d3(mesh.min_bounds),
+                                        _lib.d3(mesh.delta), Ex.data_ptr(), Ey.data_ptr(),
+                                        Ez.data_ptr())
+    _check(rc, ctx)
+    return Ex, Ey, Ez
+
+
+def synchronize(device=None) -> None:
+    """Wait for all queued space-charge work (CUDA.synchronize() in the reference benches)."""
+    dev = torch.device(device) if device is not None else _default_device()
+    ctx = _ctx(dev)
+    _check(_lib.load().sc_synchronize(ctx), ctx)
+
+
+class KernelTimer:
+    """Per-kernel HIP-event timers of the library, on the launching stream.
+
+    with KernelTimer() as t:
+        ... space-charge calls ...
+    t.results() -> {kernel_name: (total_ms, launches)}
+    """
+
+    def __init__(self, device=None):
+        self.device = torch.device(device) if device is not None else _default_device()
+        self._res = {}
+
+    def __enter__(self):
+        ctx = _ctx(self.device)
+        lib = _lib.load()
+        _check(lib.sc_timing_reset(ctx), ctx)
+        _check(lib.sc_timing_enable(ctx, 1), ctx)
+        return self
+
+    def __exit__(self, *exc):
+        ctx = _ctx(self.device)
+        lib = _lib.load()
+        _check(lib.sc_timing_enable(ctx, 0), ctx)
+        res = {}
+        for k in range(lib.sc_kernel_count()):
+            ms, n = ctypes.c_double(), ctypes.c_int64()
+            _check(lib.sc_timing_read(ctx, k, ctypes.byref(ms), ctypes.byref(n)), ctx)
+            if n.value:
+                res[lib.sc_kernel_name(k).decode()] = (ms.value, int(n.value))
+        self._res = res
+        return False
+
+    def results(self) -> dict:
+        return dict(self._res)

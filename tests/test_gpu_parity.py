@@ -1,0 +1,129 @@
+"""GPU parity: the HIP library (through the C ABI) against the CPU oracle.
+
+Tolerances, and why:
+  * interpolate: bit-identical to the oracle for the same efield (both keep
+    the reference's evaluation order and neither contracts into FMAs).
+  * deposit: each particle's 8 contributions are bit-identical; only the
+    order of the atomic sums differs, so rho agrees to a few ulp of the
+    largest node value (1e-13 relative for Float64, 1e-5 for Float32).
+  * solve, Float64: relative L2 per component against the oracle's
+    reference-order computation.  The bound (1e-11 here, measured values
+    are printed) is set by the reference algorithm's own round-off: the
+    8-corner difference of the antiderivative cancels ~log2(N^3) bits at
+    far cells, and the two implementations round differently there.
+  * solve, Float32 mesh: relative L2 <= 1e-5 against the Float64 oracle
+    (the Green function is built in Float64, transforms run in Float32).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as orc
+from _common import config_a, gaussian, mesh_to_np, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _sc():
+    import spacecharge
+    return spacecharge
+
+
+def _tensors(arrs, dtype):
+    return [torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device=DEV) for a in arrs]
+
+
+@pytest.mark.parametrize("pt,mt", [(torch.float64, torch.float64), (torch.float32, torch.float32),
+                                   (torch.float64, torch.float32), (torch.float32, torch.float64)])
+def test_deposit_matches_oracle(pt, mt):
+    sc = _sc()
+    npdt = np.float64 if pt == torch.float64 else np.float32
+    x, y, z, q = config_a(20_000, seed=1, dtype=npdt)
+    grid = (32, 32, 32)
+    xt, yt, zt, qt = _tensors((x, y, z, q), pt)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=mt)
+    sc.deposit(mesh, xt, yt, zt, qt)
+    torch.cuda.synchronize()
+    got = mesh_to_np(mesh._rho, grid)
+    mnp = np.float64 if mt == torch.float64 else np.float32
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid, mnp)
+    assert mesh.min_bounds == lo and mesh.max_bounds == hi and mesh.delta == dl
+    ref = orc.deposit(x, y, z, q, grid, lo, dl, mnp)
+    err = np.abs(got.astype(np.float64) - ref).max() / np.abs(ref).max()
+    tol = 1e-13 if mt == torch.float64 else 1e-5
+    print(f"deposit {pt}->{mt}: max rel err {err:.3e}")
+    assert err <= tol
+    assert abs(got.sum(dtype=np.float64) - q.sum(dtype=np.float64)) <= 1e-6 * abs(q.sum())
+
+
+@pytest.mark.parametrize("pt,mt", [(torch.float64, torch.float64), (torch.float32, torch.float32),
+                                   (torch.float64, torch.float32), (torch.float32, torch.float64)])
+def test_interpolate_bit_identical(pt, mt):
+    sc = _sc()
+    npdt = np.float64 if pt == torch.float64 else np.float32
+    mnp = np.float64 if mt == torch.float64 else np.float32
+    x, y, z, _ = config_a(30_000, seed=2, dtype=npdt)
+    grid = (17, 12, 20)
+    xt, yt, zt = _tensors((x, y, z), pt)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=mt)
+    rng = np.random.default_rng(3)
+    ef = rng.standard_normal(grid + (3,)).astype(mnp)
+    ef = np.asfortranarray(ef)
+    mesh._ef.copy_(torch.as_tensor(ef.ravel(order="F"), device=DEV))
+    Ex, Ey, Ez = sc.interpolate_field(mesh, xt, yt, zt)
+    assert Ex.dtype == pt
+    rEx, rEy, rEz = orc.interpolate(ef, x, y, z, mesh.min_bounds, mesh.delta)
+    for g, r in ((Ex, rEx), (Ey, rEy), (Ez, rEz)):
+        assert np.array_equal(g.cpu().numpy(), r)
+
+
+def _solve_pair(grid, T, gamma=1.0, at_cathode=False, n=20_000, seed=4, sigma=(1e-3, 1e-3, 2e-3),
+                zshift=0.0):
+    """Deposit + solve on the GPU, and the Float64 oracle solve of the same rho."""
+    sc = _sc()
+    x, y, z = gaussian(n, sigma, seed)
+    z = z + zshift
+    q = np.full(n, 1e-9 / n)
+    xt, yt, zt, qt = _tensors((x, y, z, q), torch.float64)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=T, gamma=gamma)
+    sc.deposit(mesh, xt, yt, zt, qt)
+    sc.solve(mesh, at_cathode=at_cathode)
+    torch.cuda.synchronize()
+    rho = mesh_to_np(mesh._rho, grid).astype(np.float64)
+    got = mesh_to_np(mesh._ef, grid)
+    ref = orc.solve(np.asfortranarray(rho), mesh.min_bounds, mesh.max_bounds, mesh.delta,
+                    mesh.gamma, at_cathode)
+    return got, ref
+
+
+SOLVE_CASES = [((4, 4, 4), 1.0), ((6, 6, 6), 1.0), ((8, 8, 8), 2.0), ((16, 8, 12), 1.0),
+               ((10, 14, 20), 3.0), ((32, 32, 32), 1.0), ((24, 20, 40), 10.0)]
+
+
+@pytest.mark.parametrize("grid,gamma", SOLVE_CASES)
+def test_solve_fp64_matches_oracle(grid, gamma):
+    got, ref = _solve_pair(grid, torch.float64, gamma)
+    errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
+    print(f"solve f64 {grid} gamma={gamma}: rel L2 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= 1e-11
+
+
+@pytest.mark.parametrize("grid,gamma", SOLVE_CASES)
+def test_solve_fp32_matches_fp64_oracle(grid, gamma):
+    got, ref = _solve_pair(grid, torch.float32, gamma)
+    errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
+    print(f"solve f32 {grid} gamma={gamma}: rel L2 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= 1e-5
+
+
+@pytest.mark.parametrize("grid,gamma", [((8, 8, 8), 1.0), ((16, 16, 32), 2.0), ((12, 10, 6), 1.0)])
+@pytest.mark.parametrize("T", [torch.float64, torch.float32])
+def test_cathode_matches_oracle(grid, gamma, T):
+    got, ref = _solve_pair(grid, T, gamma, at_cathode=True, zshift=4e-3)
+    errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
+    print(f"cathode {T} {grid} gamma={gamma}: rel L2 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= (1e-11 if T == torch.float64 else 1e-5)
