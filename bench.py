@@ -63,8 +63,18 @@ def kernel_bytes(name, n, grid, s, ps):
     spec = nz * 2 * ny * kx * 2 * s  # one complex spectrum [nz][2ny][nx+1]
     if name == "deposit":
         return 4 * n * ps + M * s
+    if name == "bin_count":
+        return 3 * n * ps
+    if name == "bin_scatter":
+        return 8 * n * ps
+    if name == "deposit_tiles":
+        return 4 * n * ps
+    if name == "gather_tiles":
+        return M * s
+    if name == "interleave":
+        return 7 * M * s
     if name == "interpolate":
-        return 6 * n * ps + 3 * M * s
+        return 6 * n * ps + 4 * M * s
     if name == "rho_fwd_x":
         return M * s + ny * nz * kx * 2 * s
     if name == "rho_fwd_y":

@@ -18,7 +18,8 @@ namespace sc {
 const char *const kKernelNames[K_COUNT] = {
     "clear",        "deposit",     "interpolate", "bounds_partial", "bounds_final",
     "green_x",      "green_y",     "green_z",     "rho_fwd_x",      "rho_fwd_y",
-    "zconv",        "inv_y",       "inv_x",
+    "zconv",        "inv_y",       "inv_x",       "bin_count",      "bin_scan",
+    "bin_scatter",  "deposit_tiles", "interleave",  "gather_tiles",   "green_f",
 };
 
 namespace {
@@ -298,6 +299,8 @@ int sc_mesh_destroy(sc_mesh *mesh)
     (void)hipSetDevice(M.ctx->device);
     (void)hipStreamSynchronize(M.ctx->stream);
     free_workspace(&M);
+    free_bins(&M);
+    if (M.efi) (void)hipFree(M.efi);
     if (M.owns) {
         (void)hipFree(M.rho);
         (void)hipFree(M.efield);
@@ -391,10 +394,7 @@ int sc_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y, const vo
     if (n > 0 && (!x || !y || !z || !q)) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
     if (int rc = check_geom(c, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
-    if (clear) {
-        if (int rc = launch_clear(c, M.rho, M.n[0] * M.n[1] * M.n[2] * M.dtype)) return rc;
-    }
-    return launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl);
+    return launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl, clear);
 }
 
 int sc_solve(sc_mesh *mesh, const double lo[3], const double hi[3], const double dl[3],

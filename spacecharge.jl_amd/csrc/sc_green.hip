@@ -1,0 +1,362 @@
+// sc_green.hip -- transformed integrated Green functions on gfx950.
+//
+// Reference: get_green_function! / get_green_kernel! /
+// apply_8point_differencing! (src/green_functions.jl:41-112) build, per
+// field component, the pointwise antiderivative F (:35-38) on the whole
+// (2nx, 2ny, 2nz) node lattice, difference it in 8 points and FFT the
+// result (src/solvers/free_space.jl:79-89).
+//
+// Here:
+//   green_f  F_c in Float64 at the nodes of the non-negative octant (free
+//            space; the cathode image needs the full z range) for all
+//            components in one launch.
+//   green_1  8-point difference in Float64 (the only place where the large
+//            F values cancel), rounded to the mesh type, then the x
+//            transform of the cell rows' symmetric extension (odd for G_x,
+//            even otherwise), two real rows per complex transform.
+//   green_2  y transform, two kx columns per complex transform.
+//   green_3  z transform; free space keeps kz in [0, nz] of the symmetric
+//            extension, the image takes the full transform of its 2nz-1
+//            cells in circular order.  FPEI, the cell-volume factor of
+//            :76-80 and the 1/(8 nx ny nz) of the backward transform are
+//            applied here.
+// G_x is odd in x and even in y, z (x/r^3), and cyclically for G_y, G_z, so
+// every axis transform of the free-space function is a sine or cosine
+// transform: the spectrum is i*R_c with R_c real on (nx+1)(ny+1)(nz+1)
+// points.  The image function (offset along z) keeps the x, y symmetry and
+// has a complex z spectrum.
+#include <hip/hip_runtime.h>
+
+#include "sc_solve_common.h"
+
+namespace sc {
+
+namespace {
+
+// src/green_functions.jl:35-38, in Float64.
+__device__ __forceinline__ double field_green(double x, double y, double z)
+{
+    const double r = sqrt((x * x + y * y) + z * z);
+    const double t1 = x * atan((y * z) / (r * x));
+    const double t2 = z * log(r + y);
+    const double t3 = (y * log((r - z) / (r + z))) * 0.5;
+    return (t1 - t2) + t3;
+}
+
+// Component permutation of get_green_kernel! (src/green_functions.jl:90-98).
+__device__ __forceinline__ double field_green_comp(int c, double u, double v, double w)
+{
+    if (c == 0) return field_green(u, v, w);
+    if (c == 1) return field_green(v, w, u);
+    return field_green(w, u, v);
+}
+
+struct GreenArgs {
+    Dims d;
+    double dx, dy, dz;  // dz = gamma * delta_z (green_functions.jl:73)
+    double offzg;       // image z offset times gamma (:84)
+    double scale[3];    // FPEI * factor_c / (8 nx ny nz)
+    int64_t f_free, f_img;  // F elements per component
+    int64_t r_free, r_img;  // R elements per component
+};
+
+// z node planes / z cell rows of component set `image`
+__device__ __forceinline__ int z_nodes(const GreenArgs &a, int image) { return image ? a.d.Lz : a.d.nz + 1; }
+__device__ __forceinline__ int z_cells(const GreenArgs &a, int image) { return image ? a.d.Lz - 1 : a.d.nz; }
+__device__ __forceinline__ int64_t f_off(const GreenArgs &a, int q)
+{
+    return q < 3 ? q * a.f_free : 3 * a.f_free + (q - 3) * a.f_img;
+}
+__device__ __forceinline__ int64_t r_off(const GreenArgs &a, int q)
+{
+    return q < 3 ? q * a.r_free : 3 * a.r_free + (q - 3) * a.r_img;
+}
+
+// Unpack two real sequences transformed together as A + iB whose extensions
+// share one parity: even -> both transforms real; odd -> both i*real, and
+// the stored value is the coefficient of i.
+template <typename T>
+__device__ __forceinline__ void unpack_pair(cpx<T> X, bool odd, T &ra, T &rb)
+{
+    if (odd) { ra = X.y; rb = -X.x; }
+    else { ra = X.x; rb = X.y; }
+}
+
+__global__ void __launch_bounds__(256) k_green_f(double *__restrict__ F, GreenArgs a)
+{
+    const int q = blockIdx.y, c = q % 3, image = q / 3;
+    const int nx1 = a.d.nx + 1, ny1 = a.d.ny + 1;
+    const int64_t nodes = (int64_t)nx1 * ny1 * z_nodes(a, image);
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nodes) return;
+    const int in = (int)(idx % nx1);
+    const int jn = (int)((idx / nx1) % ny1);
+    const int kn = (int)(idx / ((int64_t)nx1 * ny1));
+    const double u = ((double)in - 0.5) * a.dx;
+    const double v = ((double)jn - 0.5) * a.dy;
+    const double w = image ? ((double)(kn - a.d.nz) + 0.5) * a.dz + a.offzg
+                           : ((double)kn - 0.5) * a.dz;
+    F[f_off(a, q) + idx] = field_green_comp(c, u, v, w);
+}
+
+// 8-point difference of the F lattice for cell (i, j, k), in the order of
+// apply_8point_differencing! (src/green_functions.jl:108-111).
+__device__ __forceinline__ double diff8(const double *__restrict__ F, int i, int j, int k, int nx1,
+                                        int ny1)
+{
+    const int64_t sy = nx1, sz = (int64_t)nx1 * ny1;
+    const double *p = F + i + sy * j + sz * k;
+    double s = p[1 + sy + sz] - p[sy + sz];
+    s = s - p[1 + sz];
+    s = s - p[1 + sy];
+    s = s - p[0];
+    s = s + p[sz];
+    s = s + p[sy];
+    s = s + p[1];
+    return s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads)
+k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restrict__ twg,
+          FftPlan pl, GreenArgs a, int nb)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
+    cpx<T> *buf = tw + a.d.Lx;
+    const int q = blockIdx.y, c = q % 3, image = q / 3;
+    const int nx = a.d.nx, ny = a.d.ny, Lx = a.d.Lx, Kx = a.d.Kx;
+    const int bpairs = (ny + 1) / 2;
+    const int npencils = z_cells(a, image) * bpairs;
+    const double *Fq = F + f_off(a, q);
+    T *Rq = R + r_off(a, q);
+    load_twiddles(tw, twg, Lx);
+    const int p0 = blockIdx.x * nb;
+    for (int idx = threadIdx.x; idx < nb * nx; idx += blockDim.x) {
+        const int p = idx / nx, i = idx - p * nx;
+        const int P = p0 + p;
+        double ga = 0.0, gb = 0.0;
+        if (P < npencils) {
+            const int k = P / bpairs, j = 2 * (P - k * bpairs);
+            ga = diff8(Fq, i, j, k, nx + 1, ny + 1);
+            if (j + 1 < ny) gb = diff8(Fq, i, j + 1, k, nx + 1, ny + 1);
+        }
+        buf[p * Lx + i] = cpx<T>{(T)ga, (T)gb};
+    }
+    __syncthreads();
+    const T sgn = (c == 0) ? (T)-1 : (T)1;
+    for (int idx = threadIdx.x; idx < nb * nx; idx += blockDim.x) {
+        const int p = idx / nx, e = nx + (idx - p * nx);
+        cpx<T> v{(T)0, (T)0};
+        if (e > nx) {
+            const cpx<T> s = buf[p * Lx + Lx - e];
+            v = cpx<T>{sgn * s.x, sgn * s.y};
+        }
+        buf[p * Lx + e] = v;
+    }
+    __syncthreads();
+    lds_fft(buf, tw, pl, nb, 1, Lx, false, -1);
+    for (int idx = threadIdx.x; idx < nb * Kx; idx += blockDim.x) {
+        const int p = idx / Kx, kx = idx - p * Kx;
+        const int P = p0 + p;
+        if (P < npencils) {
+            const int k = P / bpairs, j = 2 * (P - k * bpairs);
+            T ra, rb;
+            unpack_pair(buf[p * Lx + kx], c == 0, ra, rb);
+            Rq[((size_t)k * (ny + 1) + j) * Kx + kx] = ra;
+            if (j + 1 < ny) Rq[((size_t)k * (ny + 1) + j + 1) * Kx + kx] = rb;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads)
+k_green_2(T *__restrict__ R, const cpx<T> *__restrict__ twg, FftPlan pl, GreenArgs a, int nb)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
+    cpx<T> *buf = tw + a.d.Ly;
+    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int k = blockIdx.y;
+    if (k >= z_cells(a, image)) return;  // uniform over the workgroup
+    const int ny = a.d.ny, Ly = a.d.Ly, Kx = a.d.Kx;
+    T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kx;
+    load_twiddles(tw, twg, Ly);
+    const int kp0 = blockIdx.x * nb;
+    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) {
+        const int e = idx / nb, p = idx - e * nb;
+        const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
+        const T va = kxa < Kx ? Rrow[(size_t)e * Kx + kxa] : (T)0;
+        const T vb = kxb < Kx ? Rrow[(size_t)e * Kx + kxb] : (T)0;
+        buf[e * nb + p] = cpx<T>{va, vb};
+    }
+    __syncthreads();
+    const T sgn = (c == 1) ? (T)-1 : (T)1;
+    for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) {
+        const int e = ny + idx / nb, p = idx % nb;
+        cpx<T> v{(T)0, (T)0};
+        if (e > ny) {
+            const cpx<T> s = buf[(Ly - e) * nb + p];
+            v = cpx<T>{sgn * s.x, sgn * s.y};
+        }
+        buf[e * nb + p] = v;
+    }
+    __syncthreads();
+    lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
+    for (int idx = threadIdx.x; idx < nb * (ny + 1); idx += blockDim.x) {
+        const int ky = idx / nb, p = idx - ky * nb;
+        const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
+        T ra, rb;
+        unpack_pair(buf[ky * nb + p], c == 1, ra, rb);
+        if (kxa < Kx) Rrow[(size_t)ky * Kx + kxa] = ra;
+        if (kxb < Kx) Rrow[(size_t)ky * Kx + kxb] = rb;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads)
+k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
+          const cpx<T> *__restrict__ twg, FftPlan pl, GreenArgs a, int nb)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
+    cpx<T> *buf = tw + a.d.Lz;
+    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int ky = blockIdx.y;
+    const int nz = a.d.nz, Lz = a.d.Lz, Kx = a.d.Kx, ny1 = a.d.ny + 1;
+    const T *Rq = R + r_off(a, q);
+    load_twiddles(tw, twg, Lz);
+    const int kp0 = blockIdx.x * nb;
+    for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
+        const int e = idx / nb, p = idx - e * nb;
+        const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
+        int cell = -1;  // which z cell row feeds slot e
+        if (!image) {
+            if (e < nz) cell = e;
+        } else if (e != nz) {
+            // slot e holds displacement e (e < nz) or e - Lz (e > nz);
+            // cell row = displacement + nz - 1
+            cell = (e < nz) ? e + nz - 1 : e - nz - 1;
+        }
+        cpx<T> v{(T)0, (T)0};
+        if (cell >= 0) {
+            const T *row = Rq + ((size_t)cell * ny1 + ky) * Kx;
+            v.x = kxa < Kx ? row[kxa] : (T)0;
+            v.y = kxb < Kx ? row[kxb] : (T)0;
+        }
+        buf[idx] = v;
+    }
+    __syncthreads();
+    if (!image) {
+        // symmetric extension along z (odd for G_z)
+        const T sgn = (c == 2) ? (T)-1 : (T)1;
+        for (int idx = threadIdx.x; idx < nb * nz; idx += blockDim.x) {
+            const int e = nz + idx / nb, p = idx % nb;
+            if (e > nz) {
+                const cpx<T> s = buf[(Lz - e) * nb + p];
+                buf[e * nb + p] = cpx<T>{sgn * s.x, sgn * s.y};
+            }
+        }
+        __syncthreads();
+    }
+    lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
+    const T scale = (T)a.scale[c];
+    if (!image) {
+        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kx;
+        for (int idx = threadIdx.x; idx < nb * (nz + 1); idx += blockDim.x) {
+            const int kz = idx / nb, p = idx - kz * nb;
+            const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
+            T ra, rb;
+            unpack_pair(buf[kz * nb + p], c == 2, ra, rb);
+            T *row = G + ((size_t)kz * ny1 + ky) * Kx;
+            if (kxa < Kx) row[kxa] = ra * scale;
+            if (kxb < Kx) row[kxb] = rb * scale;
+        }
+    } else {
+        // separate the two real sequences: A_k = (X_k + conj X_{-k})/2,
+        // B_k = (X_k - conj X_{-k})/(2i)
+        cpx<T> *G = Gi + (size_t)c * Lz * ny1 * Kx;
+        for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
+            const int kz = idx / nb, p = idx - kz * nb;
+            const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
+            const cpx<T> X = buf[kz * nb + p];
+            const cpx<T> Y = buf[((Lz - kz) % Lz) * nb + p];
+            const T h = (T)0.5 * scale;
+            cpx<T> *row = G + ((size_t)kz * ny1 + ky) * Kx;
+            if (kxa < Kx) row[kxa] = cpx<T>{(X.x + Y.x) * h, (X.y - Y.y) * h};
+            if (kxb < Kx) row[kxb] = cpx<T>{(X.y + Y.y) * h, (Y.x - X.x) * h};
+        }
+    }
+}
+
+}  // namespace
+
+template <typename T>
+int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cath)
+{
+    Ctx *ctx = m->ctx;
+    Workspace &ws = m->ws;
+    hipStream_t st = ctx->stream;
+    GreenArgs a;
+    a.d = make_dims(m);
+    const Dims &d = a.d;
+    a.dx = dl[0];
+    a.dy = dl[1];
+    a.dz = dl[2] * gamma;
+    a.offzg = offz * gamma;
+    const double vol = (a.dx * a.dy) * a.dz;
+    const double P = (double)d.Lx * (double)d.Ly * (double)d.Lz;
+    for (int c = 0; c < 3; ++c) a.scale[c] = kFPEI * ((c < 2 ? gamma : 1.0) / vol) / P;
+    a.f_free = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+    a.f_img = (int64_t)(d.nx + 1) * (d.ny + 1) * d.Lz;
+    a.r_free = (int64_t)d.nz * (d.ny + 1) * d.Kx;
+    a.r_img = (int64_t)(d.Lz - 1) * (d.ny + 1) * d.Kx;
+    const int ncomp = cath ? 6 : 3;
+    const cpx<T> *twx = static_cast<const cpx<T> *>(ws.twT[0]);
+    const cpx<T> *twy = static_cast<const cpx<T> *>(ws.twT[1]);
+    const cpx<T> *twz = static_cast<const cpx<T> *>(ws.twT[2]);
+    T *R = static_cast<T *>(ws.R);
+    {
+        const int64_t nodes = cath ? a.f_img : a.f_free;
+        launch_begin(ctx, K_GREEN_F);
+        hipLaunchKernelGGL(k_green_f, dim3((unsigned)((nodes + 255) / 256), ncomp), dim3(256), 0, st,
+                           ws.F, a);
+        launch_end(ctx, K_GREEN_F);
+    }
+    {
+        const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
+        const int zc = cath ? d.Lz - 1 : d.nz;
+        const int np = zc * ((d.ny + 1) / 2);
+        const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
+        launch_begin(ctx, K_GREEN_X);
+        hipLaunchKernelGGL(k_green_1<T>, dim3(div_up(np, nb), ncomp), dim3(kFftThreads), sh, st,
+                           (const double *)ws.F, R, twx, ws.plan[0], a, nb);
+        launch_end(ctx, K_GREEN_X);
+    }
+    const int kp = (d.Kx + 1) / 2;
+    {
+        const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
+        const int zc = cath ? d.Lz - 1 : d.nz;
+        const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
+        launch_begin(ctx, K_GREEN_Y);
+        hipLaunchKernelGGL(k_green_2<T>, dim3(div_up(kp, nb), zc, ncomp), dim3(kFftThreads), sh, st,
+                           R, twy, ws.plan[1], a, nb);
+        launch_end(ctx, K_GREEN_Y);
+    }
+    {
+        const int nb = pick_nb(d.Lz, fft_cap<T>(), 16);
+        const size_t sh = sizeof(cpx<T>) * (d.Lz + (size_t)nb * d.Lz);
+        launch_begin(ctx, K_GREEN_Z);
+        hipLaunchKernelGGL(k_green_3<T>, dim3(div_up(kp, nb), d.ny + 1, ncomp), dim3(kFftThreads),
+                           sh, st, (const T *)R, static_cast<T *>(ws.Gf),
+                           static_cast<cpx<T> *>(ws.Gi), twz, ws.plan[2], a, nb);
+        launch_end(ctx, K_GREEN_Z);
+    }
+    SC_HIP(ctx, hipGetLastError());
+    return SC_OK;
+}
+
+template int build_green<float>(Mesh *, const double[3], double, double, int);
+template int build_green<double>(Mesh *, const double[3], double, double, int);
+
+}  // namespace sc

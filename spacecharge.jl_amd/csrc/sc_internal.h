@@ -31,6 +31,13 @@ enum KernelId : int {
     K_ZCONV,
     K_INV_Y,
     K_INV_X,
+    K_BIN_COUNT,
+    K_BIN_SCAN,
+    K_BIN_SCATTER,
+    K_DEPOSIT_TILES,
+    K_INTERLEAVE,
+    K_GATHER_TILES,
+    K_GREEN_F,
     K_COUNT
 };
 extern const char *const kKernelNames[K_COUNT];
@@ -95,17 +102,28 @@ struct Workspace {
     // image (cathode) [2nz][ny+1][nx+1] complex x3.
     void *Gf = nullptr;
     void *Gi = nullptr;
-    // Float64 scratch for the Green build: [2nz][ny+1][nx+1].
-    double *H = nullptr;
-    // twiddle tables: T precision for the field transforms, double for the
-    // Green transforms, for the lengths 2nx, 2ny, 2nz.
+    // Green build scratch: the antiderivative at the octant nodes (Float64,
+    // free-space and image components) and the partially transformed cell
+    // rows (T).
+    double *F = nullptr;
+    void *R = nullptr;
+    // twiddle tables in T for the lengths 2nx, 2ny, 2nz
     void *twT[3] = {nullptr, nullptr, nullptr};
-    double *twD[3] = {nullptr, nullptr, nullptr};
     FftPlan plan[3];
     // cached Green key for the opt-in reuse
     bool g_valid = false;
     double g_key[6] = {};
     int g_cath = 0;
+};
+
+// Scratch of the tile-binned deposit, grown on demand.
+struct DepositBins {
+    void *sorted = nullptr;       // (x, y, z, q) records in tile order
+    size_t sorted_bytes = 0;
+    void *chunks = nullptr;       // int4 (tile, start, length, 0)
+    void *partials = nullptr;     // one 17^3 sub-grid per chunk
+    int64_t chunk_cap = 0;
+    unsigned *counts = nullptr;   // counts | cursor | chunk_base | nchunks
 };
 
 struct Mesh {
@@ -117,6 +135,8 @@ struct Mesh {
     bool owns = false;
     bool green_reuse = false;
     Workspace ws;
+    DepositBins bins;
+    void *efi = nullptr;  // node-interleaved field for the interpolation gather
 };
 
 // ---------------------------------------------------------------------------
@@ -137,7 +157,11 @@ void launch_end(Ctx *ctx, int kid);
 // Kernel-launch entry points (defined in the .hip files).
 int launch_clear(Ctx *ctx, void *p, int64_t bytes);
 int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
-                   int64_t n, const double lo[3], const double dl[3]);
+                   int64_t n, const double lo[3], const double dl[3], int clear);
+int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const void *z,
+                          const void *q, int64_t n, const double lo[3], const double dl[3],
+                          int clear);
+void free_bins(Mesh *m);
 int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
                   const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez);
 int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,

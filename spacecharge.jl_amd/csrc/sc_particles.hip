@@ -63,12 +63,39 @@ k_deposit(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__rest
     atomicAdd(b + sz + sy + 1, (MT)(((c * wx1) * wy1) * wz1));
 }
 
+// Node-interleaved copy of the field, (Ex, Ey, Ez, 0) per node, so that the
+// gather below needs one vector load per node instead of three scalar ones.
+template <typename T>
+struct V4;
+template <>
+struct V4<float> {
+    using type = float4;
+};
+template <>
+struct V4<double> {
+    using type = double4;
+};
+
+template <typename MT>
+__global__ void __launch_bounds__(256)
+k_interleave(const MT *__restrict__ ef, typename V4<MT>::type *__restrict__ efi, int64_t M)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    typename V4<MT>::type v;
+    v.x = ef[i];
+    v.y = ef[M + i];
+    v.z = ef[2 * M + i];
+    v.w = (MT)0;
+    efi[i] = v;
+}
+
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(256)
-k_interp(const MT *__restrict__ ef, const PT *__restrict__ px, const PT *__restrict__ py,
-         const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz, double lx_, double ly_,
-         double lz_, double dx_, double dy_, double dz_, PT *__restrict__ Ex, PT *__restrict__ Ey,
-         PT *__restrict__ Ez)
+k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
+         const PT *__restrict__ py, const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz,
+         double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
+         PT *__restrict__ Ex, PT *__restrict__ Ey, PT *__restrict__ Ez)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
@@ -90,21 +117,18 @@ k_interp(const MT *__restrict__ ef, const PT *__restrict__ px, const PT *__restr
     const C w101 = (dx * (one - dy)) * dz;
     const C w011 = ((one - dx) * dy) * dz;
     const C w111 = (dx * dy) * dz;
-    const int64_t sy = nx, sz = (int64_t)nx * ny, M = sz * nz;
-    const int64_t b = ix + sy * iy + sz * iz;
-    PT *out[3] = {Ex, Ey, Ez};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const MT *e = ef + c * M + b;
-        C s = (C)e[0] * w000 + (C)e[1] * w100;
-        s = s + (C)e[sy] * w010;
-        s = s + (C)e[sy + 1] * w110;
-        s = s + (C)e[sz] * w001;
-        s = s + (C)e[sz + 1] * w101;
-        s = s + (C)e[sz + sy] * w011;
-        s = s + (C)e[sz + sy + 1] * w111;
-        out[c][p] = (PT)s;
-    }
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const typename V4<MT>::type *e = efi + (ix + sy * iy + sz * iz);
+    const auto n000 = e[0], n100 = e[1], n010 = e[sy], n110 = e[sy + 1];
+    const auto n001 = e[sz], n101 = e[sz + 1], n011 = e[sz + sy], n111 = e[sz + sy + 1];
+    // the reference's sum order, per component (src/interpolation.jl:56-85)
+#define SC_ITP(f)                                              \
+    ((((((((C)n000.f * w000 + (C)n100.f * w100) + (C)n010.f * w010) + (C)n110.f * w110) + \
+         (C)n001.f * w001) + (C)n101.f * w101) + (C)n011.f * w011) + (C)n111.f * w111)
+    Ex[p] = (PT)SC_ITP(x);
+    Ey[p] = (PT)SC_ITP(y);
+    Ez[p] = (PT)SC_ITP(z);
+#undef SC_ITP
 }
 
 // ---- bounds: grid-stride min/max per block, then one block folds them ----
@@ -198,14 +222,24 @@ void dep(Mesh *m, const void *x, const void *y, const void *z, const void *q, in
                        dl[0], dl[1], dl[2]);
 }
 
+template <typename MT>
+void interleave(Mesh *m)
+{
+    const int64_t M = m->n[0] * m->n[1] * m->n[2];
+    hipLaunchKernelGGL((k_interleave<MT>), dim3((unsigned)blocks_for(M, 256)), dim3(256), 0,
+                       m->ctx->stream, (const MT *)m->efield,
+                       static_cast<typename V4<MT>::type *>(m->efi), M);
+}
+
 template <typename PT, typename MT>
 void itp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, const double lo[3],
          const double dl[3], void *Ex, void *Ey, void *Ez)
 {
     hipLaunchKernelGGL((k_interp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
-                       m->ctx->stream, (const MT *)m->efield, (const PT *)x, (const PT *)y,
-                       (const PT *)z, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1],
-                       lo[2], dl[0], dl[1], dl[2], (PT *)Ex, (PT *)Ey, (PT *)Ez);
+                       m->ctx->stream, static_cast<const typename V4<MT>::type *>(m->efi),
+                       (const PT *)x, (const PT *)y, (const PT *)z, n, (int)m->n[0], (int)m->n[1],
+                       (int)m->n[2], lo[0], lo[1], lo[2], dl[0], dl[1], dl[2], (PT *)Ex, (PT *)Ey,
+                       (PT *)Ez);
 }
 
 }  // namespace
@@ -225,9 +259,16 @@ int launch_clear(Ctx *ctx, void *p, int64_t bytes)
 }
 
 int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
-                   int64_t n, const double lo[3], const double dl[3])
+                   int64_t n, const double lo[3], const double dl[3], int clear)
 {
-    if (n <= 0) return SC_OK;
+    const int64_t rho_bytes = m->n[0] * m->n[1] * m->n[2] * m->dtype;
+    if (n <= 0) return clear ? launch_clear(m->ctx, m->rho, rho_bytes) : SC_OK;
+    const int rc = launch_deposit_binned(m, pdt, x, y, z, q, n, lo, dl, clear);
+    if (rc != SC_ERR_UNSUPPORTED) return rc;
+    // more tiles than the binned path holds: one atomic per node contribution
+    if (clear) {
+        if (int rc2 = launch_clear(m->ctx, m->rho, rho_bytes)) return rc2;
+    }
     if ((n + 255) / 256 > 0x7fffffffLL)
         return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
     launch_begin(m->ctx, K_DEPOSIT);
@@ -246,6 +287,17 @@ int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z,
     if (n <= 0) return SC_OK;
     if ((n + 255) / 256 > 0x7fffffffLL)
         return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
+    const int64_t M = m->n[0] * m->n[1] * m->n[2];
+    if (!m->efi) {
+        if (hipMalloc(&m->efi, (size_t)M * 4 * m->dtype) != hipSuccess)
+            return set_err(m->ctx, SC_ERR_NOMEM, "device allocation of the interleaved field failed");
+    }
+    // efield may have been written by the caller since the last solve, so the
+    // interleaved copy is refreshed on every call
+    launch_begin(m->ctx, K_INTERLEAVE);
+    if (m->dtype == SC_F32) interleave<float>(m);
+    else interleave<double>(m);
+    launch_end(m->ctx, K_INTERLEAVE);
     launch_begin(m->ctx, K_INTERP);
     if (pdt == SC_F32 && m->dtype == SC_F32) itp<float, float>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
     else if (pdt == SC_F64 && m->dtype == SC_F64) itp<double, double>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);

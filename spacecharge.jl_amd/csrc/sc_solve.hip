@@ -1,257 +1,39 @@
 // sc_solve.hip -- the free-space / cathode field solve on gfx950.
 //
-// Reference: solve! and solve_freespace! (src/solvers/free_space.jl:14-101)
-// with the integrated Green function of src/green_functions.jl:35-112.
+// Reference: solve! and solve_freespace! (src/solvers/free_space.jl:14-101).
 // The reference zero-pads rho to (2nx,2ny,2nz), builds each G_c on the full
 // padded lattice, runs 7 dense 3-D complex FFTs (14 with the cathode) and
 // extracts the physical octant.  The same linear convolution is computed
 // here with far less HBM traffic:
 //
-//  * Green function (Float64).  G_c = Dx Dy Dz F_c with D the forward
-//    difference along one axis; the differences and the axis transforms
-//    are separable, so each axis pass evaluates/differences along its own
-//    axis and transforms right away.  Each pass's input has a definite
-//    parity along its axis (G_x odd in x and even in y, z; cyclically for
-//    G_y, G_z), so only the non-negative half of the lattice is built and
-//    the transform is that of its symmetric extension: one eighth of the
-//    antiderivative evaluations, and the spectrum is i*R_c with R_c real on
-//    (nx+1)(ny+1)(nz+1) points.  The image Green function of the cathode
-//    solve has no z symmetry and keeps a full complex z spectrum.
+//  * Green spectra: sc_green.hip (symmetric transforms of the non-negative
+//    octant, Float64 antiderivative and differencing).
 //  * rho.  x transform of the real rows (kx in [0, nx] kept), then the y
 //    transform; both skip the zero half of the padded input.
 //  * z convolution, fused: per (kx, ky) pencil, forward z transform of rho
 //    (and of the flipped, negated image charge), product with all three
-//    Green spectra, backward z transform, only z < nz written.
+//    Green spectra, backward z transform, only z < nz written.  The cathode
+//    image adds its product before the same backward transform, so the
+//    cathode solve costs one extra forward pencil transform, not a second
+//    solve.
 //  * backward y transform keeping y < ny, then the Hermitian (complex to
 //    real) backward x transform keeping x < nx; FPEI, the cell-volume factor
 //    and 1/(8 nx ny nz) are folded into the Green spectra.
 // Only rows that can be non-zero are read, only outputs that are used are
 // written.  Workspace: 4 spectra of nz*2ny*(nx+1) complex T, plus the Green
-// spectra and a Float64 scratch.
+// spectra and their build scratch.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <vector>
 
-#include "sc_fft_dev.h"
-#include "sc_internal.h"
+#include "sc_solve_common.h"
 
 namespace sc {
 
 namespace {
-
-constexpr double kFPEI = 299792458.0 * 299792458.0 * 1.0e-7;  // src/utils.jl:7-8
-
-// src/green_functions.jl:35-38, in Float64.
-__device__ __forceinline__ double field_green(double x, double y, double z)
-{
-    const double r = sqrt((x * x + y * y) + z * z);
-    const double t1 = x * atan((y * z) / (r * x));
-    const double t2 = z * log(r + y);
-    const double t3 = (y * log((r - z) / (r + z))) * 0.5;
-    return (t1 - t2) + t3;
-}
-
-// Component permutation of get_green_kernel! (src/green_functions.jl:90-98).
-__device__ __forceinline__ double field_green_comp(int c, double u, double v, double w)
-{
-    if (c == 0) return field_green(u, v, w);
-    if (c == 1) return field_green(v, w, u);
-    return field_green(w, u, v);
-}
-
-struct Dims {
-    int nx, ny, nz;
-    int Kx;          // nx + 1 kept x frequencies
-    int Lx, Ly, Lz;  // padded lengths
-};
-
-// ---------------------------------------------------------------------------
-// Green x pass: one pencil per node row (jn, kn).  F_c at the x nodes
-// u_i = (i - 1/2) dx, i in [0, nx]; D_x; symmetric extension (odd when
-// c == 0); forward transform; Im (odd) or Re (even) of kx in [0, nx] to H.
-__global__ void __launch_bounds__(kFftThreads)
-k_green_x(double *__restrict__ H, const cpx<double> *__restrict__ twg, FftPlan pl, Dims d,
-          int nb, int nrows, int c, double dx, double dy, double dz, double w0)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    cpx<double> *tw = reinterpret_cast<cpx<double> *>(smem);
-    cpx<double> *buf = tw + d.Lx;
-    const int Lx = d.Lx, nx = d.nx;
-    load_twiddles(tw, twg, Lx);
-    const int r0 = blockIdx.x * nb;
-    // F at the nx+1 nodes of each row
-    for (int idx = threadIdx.x; idx < nb * (nx + 1); idx += blockDim.x) {
-        const int b = idx / (nx + 1), i = idx - b * (nx + 1);
-        const int r = r0 + b;
-        double f = 0.0;
-        if (r < nrows) {
-            const int jn = r % (d.ny + 1), kn = r / (d.ny + 1);
-            const double u = ((double)i - 0.5) * dx;
-            const double v = ((double)jn - 0.5) * dy;
-            const double w = ((double)kn - 0.5) * dz + w0;
-            f = field_green_comp(c, u, v, w);
-        }
-        buf[b * Lx + i].x = f;
-    }
-    __syncthreads();
-    // difference + symmetric extension, held in registers across the barrier
-    const double sgn = (c == 0) ? -1.0 : 1.0;
-    double ext[fft_elems<double>()];
-#pragma unroll
-    for (int s = 0; s < fft_elems<double>(); ++s) {
-        const int idx = threadIdx.x + s * kFftThreads;
-        double e = 0.0;
-        if (idx < nb * Lx) {
-            const int b = idx / Lx, p = idx - b * Lx;
-            const double *row = &buf[b * Lx].x;
-            if (p < nx) e = row[2 * (p + 1)] - row[2 * p];
-            else if (p > nx) { const int a = Lx - p; e = sgn * (row[2 * (a + 1)] - row[2 * a]); }
-        }
-        ext[s] = e;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < fft_elems<double>(); ++s) {
-        const int idx = threadIdx.x + s * kFftThreads;
-        if (idx < nb * Lx) buf[idx] = cpx<double>{ext[s], 0.0};
-    }
-    __syncthreads();
-    lds_fft(buf, tw, pl, nb, 1, Lx, false, -1);
-    const bool odd = (c == 0);
-    for (int idx = threadIdx.x; idx < nb * d.Kx; idx += blockDim.x) {
-        const int b = idx / d.Kx, kx = idx - b * d.Kx;
-        const int r = r0 + b;
-        if (r < nrows) {
-            const cpx<double> X = buf[b * Lx + kx];
-            H[(size_t)r * d.Kx + kx] = odd ? X.y : X.x;
-        }
-    }
-}
-
-// Green y pass: pencils along jn in [0, ny], batched over kx; in place in H.
-__global__ void __launch_bounds__(kFftThreads)
-k_green_y(double *__restrict__ H, const cpx<double> *__restrict__ twg, FftPlan pl, Dims d, int nb,
-          int c)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    cpx<double> *tw = reinterpret_cast<cpx<double> *>(smem);
-    cpx<double> *buf = tw + d.Ly;
-    const int Ly = d.Ly, ny = d.ny, Kx = d.Kx;
-    load_twiddles(tw, twg, Ly);
-    const int kx0 = blockIdx.x * nb, kn = blockIdx.y;
-    double *Hp = H + (size_t)kn * (ny + 1) * Kx;
-    // load node values D[jn] into buf[jn*nb + b].x
-    for (int idx = threadIdx.x; idx < nb * (ny + 1); idx += blockDim.x) {
-        const int jn = idx / nb, b = idx - jn * nb;
-        const int kx = kx0 + b;
-        buf[jn * nb + b].x = (kx < Kx) ? Hp[(size_t)jn * Kx + kx] : 0.0;
-    }
-    __syncthreads();
-    const double sgn = (c == 1) ? -1.0 : 1.0;
-    double ext[fft_elems<double>()];
-#pragma unroll
-    for (int s = 0; s < fft_elems<double>(); ++s) {
-        const int idx = threadIdx.x + s * kFftThreads;
-        double e = 0.0;
-        if (idx < nb * Ly) {
-            const int p = idx / nb, b = idx - p * nb;
-            if (p < ny) e = buf[(p + 1) * nb + b].x - buf[p * nb + b].x;
-            else if (p > ny) { const int a = Ly - p; e = sgn * (buf[(a + 1) * nb + b].x - buf[a * nb + b].x); }
-        }
-        ext[s] = e;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < fft_elems<double>(); ++s) {
-        const int idx = threadIdx.x + s * kFftThreads;
-        if (idx < nb * Ly) buf[idx] = cpx<double>{ext[s], 0.0};
-    }
-    __syncthreads();
-    lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
-    const bool odd = (c == 1);
-    for (int idx = threadIdx.x; idx < nb * (ny + 1); idx += blockDim.x) {
-        const int ky = idx / nb, b = idx - ky * nb;
-        const int kx = kx0 + b;
-        if (kx < Kx) {
-            const cpx<double> X = buf[ky * nb + b];
-            Hp[(size_t)ky * Kx + kx] = odd ? X.y : X.x;
-        }
-    }
-}
-
-// Green z pass: pencils along kn, batched over kx, one ky' per blockIdx.y.
-// Free space: symmetric extension (odd when c == 2), real spectrum for
-// kz in [0, nz] to Gf.  Cathode image: the 2nz-1 cells at displacements
-// -(nz-1)..(nz-1) in circular order, full complex spectrum to Gi.
-template <typename T>
-__global__ void __launch_bounds__(kFftThreads)
-k_green_z(const double *__restrict__ H, T *__restrict__ Gout, const cpx<double> *__restrict__ twg,
-          FftPlan pl, Dims d, int nb, int c, int image, double scale)
-{
-    extern __shared__ __align__(16) unsigned char smem[];
-    cpx<double> *tw = reinterpret_cast<cpx<double> *>(smem);
-    cpx<double> *buf = tw + d.Lz;
-    const int Lz = d.Lz, nz = d.nz, Kx = d.Kx, ny1 = d.ny + 1;
-    load_twiddles(tw, twg, Lz);
-    const int kx0 = blockIdx.x * nb, ky = blockIdx.y;
-    const int nnodes = image ? Lz : nz + 1;
-    for (int idx = threadIdx.x; idx < nb * nnodes; idx += blockDim.x) {
-        const int kn = idx / nb, b = idx - kn * nb;
-        const int kx = kx0 + b;
-        buf[kn * nb + b].x = (kx < Kx) ? H[((size_t)kn * ny1 + ky) * Kx + kx] : 0.0;
-    }
-    __syncthreads();
-    const double sgn = (c == 2) ? -1.0 : 1.0;
-    double ext[fft_elems<double>()];
-#pragma unroll
-    for (int s = 0; s < fft_elems<double>(); ++s) {
-        const int idx = threadIdx.x + s * kFftThreads;
-        double e = 0.0;
-        if (idx < nb * Lz) {
-            const int p = idx / nb, b = idx - p * nb;
-            if (!image) {
-                if (p < nz) e = buf[(p + 1) * nb + b].x - buf[p * nb + b].x;
-                else if (p > nz) { const int a = Lz - p; e = sgn * (buf[(a + 1) * nb + b].x - buf[a * nb + b].x); }
-            } else if (p != nz) {
-                // circular slot p holds displacement p (p < nz) or p - Lz (p > nz);
-                // cell t = displacement + nz - 1 spans nodes t, t+1
-                const int t = (p < nz) ? p + nz - 1 : p - nz - 1;
-                e = buf[(t + 1) * nb + b].x - buf[t * nb + b].x;
-            }
-        }
-        ext[s] = e;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < fft_elems<double>(); ++s) {
-        const int idx = threadIdx.x + s * kFftThreads;
-        if (idx < nb * Lz) buf[idx] = cpx<double>{ext[s], 0.0};
-    }
-    __syncthreads();
-    lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
-    if (!image) {
-        const bool odd = (c == 2);
-        for (int idx = threadIdx.x; idx < nb * (nz + 1); idx += blockDim.x) {
-            const int kz = idx / nb, b = idx - kz * nb;
-            const int kx = kx0 + b;
-            if (kx < Kx) {
-                const cpx<double> X = buf[kz * nb + b];
-                Gout[((size_t)kz * ny1 + ky) * Kx + kx] = (T)((odd ? X.y : X.x) * scale);
-            }
-        }
-    } else {
-        cpx<T> *Gc = reinterpret_cast<cpx<T> *>(Gout);
-        for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
-            const int kz = idx / nb, b = idx - kz * nb;
-            const int kx = kx0 + b;
-            if (kx < Kx) {
-                const cpx<double> X = buf[kz * nb + b];
-                Gc[((size_t)kz * ny1 + ky) * Kx + kx] = cpx<T>{(T)(X.x * scale), (T)(X.y * scale)};
-            }
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // rho x pass: real rows (y, z), zero-padded to 2nx, kx in [0, nx] to A.
@@ -457,22 +239,6 @@ k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ E, const cpx<T> *__restric
     }
 }
 
-// ---------------------------------------------------------------------------
-// host helpers
-
-int div_up(int a, int b) { return (a + b - 1) / b; }
-
-// Largest batch with nb*n <= cap elements, at most nbmax pencils.
-int pick_nb(int n, int cap, int nbmax)
-{
-    int nb = cap / n;
-    if (nb > nbmax) nb = nbmax;
-    if (nb < 1) nb = 1;
-    return nb;
-}
-
-
-
 template <typename T>
 int upload_twiddles(Ctx *ctx, int n, void **dst)
 {
@@ -495,73 +261,18 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     Ctx *ctx = m->ctx;
     Workspace &ws = m->ws;
     hipStream_t st = ctx->stream;
-    Dims d;
-    d.nx = (int)m->n[0];
-    d.ny = (int)m->n[1];
-    d.nz = (int)m->n[2];
-    d.Kx = d.nx + 1;
-    d.Lx = 2 * d.nx;
-    d.Ly = 2 * d.ny;
-    d.Lz = 2 * d.nz;
+    const Dims d = make_dims(m);
     const cpx<T> *twx = static_cast<const cpx<T> *>(ws.twT[0]);
     const cpx<T> *twy = static_cast<const cpx<T> *>(ws.twT[1]);
     const cpx<T> *twz = static_cast<const cpx<T> *>(ws.twT[2]);
 
-    // ---- Green spectra (Float64 build) ----
-    const double dx = dl[0], dy = dl[1], dz = dl[2] * gamma;  // green_functions.jl:73
-    const double offz = lo[2] + hi[2];                        // free_space.jl:39
+    // ---- Green spectra ----
+    const double offz = lo[2] + hi[2];  // 2*zmin + (zmax - zmin), free_space.jl:39
     const double key[6] = {dl[0], dl[1], dl[2], gamma, cath ? offz : 0.0, 0.0};
-    bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == cath &&
-                 std::memcmp(key, ws.g_key, sizeof(key)) == 0;
+    const bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == cath &&
+                       std::memcmp(key, ws.g_key, sizeof(key)) == 0;
     if (!reuse) {
-        const double vol = (dx * dy) * dz;
-        const double P = (double)d.Lx * (double)d.Ly * (double)d.Lz;
-        for (int image = 0; image <= cath; ++image) {
-            const int nzn = image ? d.Lz : d.nz + 1;       // z node planes
-            const double w0 = image ? ((double)(-d.nz + 1) * dz + offz * gamma) : 0.0;
-            for (int c = 0; c < 3; ++c) {
-                const double factor = (c < 2 ? gamma : 1.0) / vol;  // :76-80
-                const double scale = kFPEI * factor / P;
-                // x pass
-                {
-                    const int nb = pick_nb(d.Lx, fft_cap<double>(), 16);
-                    const int nrows = (d.ny + 1) * nzn;
-                    const size_t sh = sizeof(cpx<double>) * (d.Lx + (size_t)nb * d.Lx);
-                    launch_begin(ctx, K_GREEN_X);
-                    hipLaunchKernelGGL(k_green_x, dim3(div_up(nrows, nb)), dim3(kFftThreads), sh, st,
-                                       ws.H, (const cpx<double> *)ws.twD[0], ws.plan[0], d, nb,
-                                       nrows, c, dx, dy, dz, w0);
-                    launch_end(ctx, K_GREEN_X);
-                }
-                // y pass
-                {
-                    const int nb = pick_nb(d.Ly, fft_cap<double>(), 16);
-                    const size_t sh = sizeof(cpx<double>) * (d.Ly + (size_t)nb * d.Ly);
-                    launch_begin(ctx, K_GREEN_Y);
-                    hipLaunchKernelGGL(k_green_y, dim3(div_up(d.Kx, nb), nzn), dim3(kFftThreads), sh,
-                                       st, ws.H, (const cpx<double> *)ws.twD[1], ws.plan[1], d, nb,
-                                       c);
-                    launch_end(ctx, K_GREEN_Y);
-                }
-                // z pass
-                {
-                    const int nb = pick_nb(d.Lz, fft_cap<double>(), 16);
-                    const size_t sh = sizeof(cpx<double>) * (d.Lz + (size_t)nb * d.Lz);
-                    T *gout;
-                    if (!image)
-                        gout = static_cast<T *>(ws.Gf) + (size_t)c * (d.nz + 1) * (d.ny + 1) * d.Kx;
-                    else
-                        gout = reinterpret_cast<T *>(static_cast<cpx<T> *>(ws.Gi) +
-                                                     (size_t)c * d.Lz * (d.ny + 1) * d.Kx);
-                    launch_begin(ctx, K_GREEN_Z);
-                    hipLaunchKernelGGL(k_green_z<T>, dim3(div_up(d.Kx, nb), d.ny + 1),
-                                       dim3(kFftThreads), sh, st, ws.H, gout,
-                                       (const cpx<double> *)ws.twD[2], ws.plan[2], d, nb, c, image,
-                                       scale);
-                    launch_end(ctx, K_GREEN_Z);
-                }
-            }
-        }
+        if (int rc = build_green<T>(m, dl, gamma, offz, cath)) return rc;
         ws.g_valid = true;
         ws.g_cath = cath;
         std::memcpy(ws.g_key, key, sizeof(key));
@@ -642,8 +353,7 @@ bool make_plan(int n, FftPlan *p)
 void free_workspace(Mesh *m)
 {
     Workspace &ws = m->ws;
-    void *ptrs[] = {ws.A, ws.B, ws.Gf, ws.Gi, ws.H, ws.twT[0], ws.twT[1], ws.twT[2],
-                    ws.twD[0], ws.twD[1], ws.twD[2]};
+    void *ptrs[] = {ws.A, ws.B, ws.Gf, ws.Gi, ws.F, ws.R, ws.twT[0], ws.twT[1], ws.twT[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     ws = Workspace();
@@ -656,23 +366,27 @@ int ensure_workspace(Mesh *m)
     Ctx *ctx = m->ctx;
     const int64_t nx = m->n[0], ny = m->n[1], nz = m->n[2];
     const int L[3] = {(int)(2 * nx), (int)(2 * ny), (int)(2 * nz)};
+    const int cap = (m->dtype == SC_F32) ? fft_cap<float>() : fft_cap<double>();
     for (int a = 0; a < 3; ++a) {
         if (!make_plan(L[a], &ws.plan[a]))
             return set_err(ctx, SC_ERR_UNSUPPORTED,
                            "grid size " + std::to_string(L[a] / 2) +
                                ": the padded length has a prime factor above 7, which the "
                                "MI355X transform does not support");
-        // the Float64 Green transforms bound the length (one pencil per workgroup)
-        if (L[a] > fft_cap<double>())
+        // one pencil must fit a workgroup's register budget
+        if (L[a] > cap)
             return set_err(ctx, SC_ERR_UNSUPPORTED,
                            "grid size " + std::to_string(L[a] / 2) + " exceeds the maximum of " +
-                               std::to_string(fft_cap<double>() / 2) + " per axis");
+                               std::to_string(cap / 2) + " per axis");
     }
     const size_t s = (m->dtype == SC_F32) ? 4 : 8;
     const size_t Kx = nx + 1;
     const size_t spec = (size_t)nz * 2 * ny * Kx;  // complex elements of one spectrum
     const size_t gf = (size_t)(nz + 1) * (ny + 1) * Kx;
     const size_t gi = (size_t)2 * nz * (ny + 1) * Kx;
+    // Green build scratch: F at the nodes (Float64), R rows (T), free + image
+    const size_t fn = (size_t)(nx + 1) * (ny + 1) * ((nz + 1) + 2 * nz);
+    const size_t rn = (size_t)(ny + 1) * Kx * (nz + (2 * nz - 1));
     auto fail = [&](void) {
         free_workspace(m);
         return set_err(ctx, SC_ERR_NOMEM, "device allocation of the solver workspace failed");
@@ -681,13 +395,12 @@ int ensure_workspace(Mesh *m)
     if (hipMalloc(&ws.B, 3 * 2 * s * spec) != hipSuccess) return fail();
     if (hipMalloc(&ws.Gf, 3 * s * gf) != hipSuccess) return fail();
     if (hipMalloc(&ws.Gi, 3 * 2 * s * gi) != hipSuccess) return fail();
-    if (hipMalloc((void **)&ws.H, sizeof(double) * gi) != hipSuccess) return fail();
-    ws.bytes = (int64_t)(8 * s * spec + 3 * s * gf + 6 * s * gi + 8 * gi);
+    if (hipMalloc((void **)&ws.F, sizeof(double) * 3 * fn) != hipSuccess) return fail();
+    if (hipMalloc(&ws.R, s * 3 * rn) != hipSuccess) return fail();
+    ws.bytes = (int64_t)(8 * s * spec + 3 * s * gf + 6 * s * gi + 24 * fn + 3 * s * rn);
     for (int a = 0; a < 3; ++a) {
         int rc = (m->dtype == SC_F32) ? upload_twiddles<float>(ctx, L[a], &ws.twT[a])
                                       : upload_twiddles<double>(ctx, L[a], &ws.twT[a]);
-        if (rc != SC_OK) { free_workspace(m); return rc; }
-        rc = upload_twiddles<double>(ctx, L[a], (void **)&ws.twD[a]);
         if (rc != SC_OK) { free_workspace(m); return rc; }
     }
     ws.dtype = m->dtype;

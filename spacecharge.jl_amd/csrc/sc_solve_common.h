@@ -1,0 +1,49 @@
+// sc_solve_common.h -- pieces shared by the Green-function build and the
+// field transforms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sc_fft_dev.h"
+#include "sc_internal.h"
+
+namespace sc {
+
+constexpr double kFPEI = 299792458.0 * 299792458.0 * 1.0e-7;  // src/utils.jl:7-8
+
+struct Dims {
+    int nx, ny, nz;
+    int Kx;          // nx + 1 kept x frequencies
+    int Lx, Ly, Lz;  // padded lengths
+};
+
+inline Dims make_dims(const Mesh *m)
+{
+    Dims d;
+    d.nx = (int)m->n[0];
+    d.ny = (int)m->n[1];
+    d.nz = (int)m->n[2];
+    d.Kx = d.nx + 1;
+    d.Lx = 2 * d.nx;
+    d.Ly = 2 * d.ny;
+    d.Lz = 2 * d.nz;
+    return d;
+}
+
+inline int div_up(int a, int b) { return (a + b - 1) / b; }
+
+// Largest batch with nb*n <= cap elements, at most nbmax pencils.
+inline int pick_nb(int n, int cap, int nbmax)
+{
+    int nb = cap / n;
+    if (nb > nbmax) nb = nbmax;
+    if (nb < 1) nb = 1;
+    return nb;
+}
+
+// Builds the transformed Green functions of one solve into ws.Gf (and
+// ws.Gi when cathode != 0).
+template <typename T>
+int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cathode);
+
+}  // namespace sc
