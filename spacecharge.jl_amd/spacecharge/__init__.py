@@ -123,16 +123,17 @@ class Mesh3D:
                  gamma: float = 1.0, total_charge: float = 0.0, device=None):
         if T not in _DT:
             raise SpaceChargeError("T must be torch.float32 or torch.float64")
+        # argument checks first, in the reference's order (src/mesh.jl:106-116, 206-211)
         g = _check_grid(grid_size)
         if len(args) == 3:
             px, py, pz = args
-            dev = torch.device(device) if device is not None else (
-                px.device if isinstance(px, torch.Tensor) and px.is_cuda else _default_device())
             lens = [len(a) for a in (px, py, pz)]
             if min(lens) == 0:
-                raise SpaceChargeError("Particle arrays cannot be empty.")  # :110-112
+                raise SpaceChargeError("Particle arrays cannot be empty.")
             if not (lens[0] == lens[1] == lens[2]):
                 raise SpaceChargeError("Particle coordinate arrays must have the same length.")
+            dev = torch.device(device) if device is not None else (
+                px.device if isinstance(px, torch.Tensor) and px.is_cuda else _default_device())
             x = _as_device(px, dev)
             y = _as_device(py, dev, x.dtype)
             z = _as_device(pz, dev, x.dtype)
