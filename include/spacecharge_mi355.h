@@ -98,6 +98,11 @@ int64_t sc_mesh_workspace_bytes(sc_mesh *mesh);
  * (delta, gamma, at_cathode, zmin+zmax) are bit-identical.  Off by default,
  * so every sc_solve rebuilds it as solve! does (src/solvers/free_space.jl:79-86). */
 int sc_mesh_set_green_reuse(sc_mesh *mesh, int enable);
+/* Diagnostic (no reference counterpart): enable != 0 runs every transform on
+ * the generic mixed-radix LDS engine even for power-of-two lengths that the
+ * register-resident engine handles, so both engines can be checked against
+ * the same results. */
+int sc_mesh_force_generic_transforms(sc_mesh *mesh, int enable);
 
 /* Replaces the particle-extrema constructor's bounds computation
  * (src/mesh.jl:95-174): device min/max of x, y, z, then

@@ -24,8 +24,9 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libspacecharge_mi355.so")
 ARCH = "gfx950"
 
-SOURCES = ["sc_api.cpp", "sc_particles.hip", "sc_deposit.hip", "sc_green.hip", "sc_solve.hip"]
-HEADERS = ["sc_internal.h", "sc_fft_dev.h", "sc_solve_common.h"]
+SOURCES = ["sc_api.cpp", "sc_particles.hip", "sc_deposit.hip", "sc_green.hip", "sc_solve.hip",
+           "sc_solve_fast.hip"]
+HEADERS = ["sc_internal.h", "sc_fft_dev.h", "sc_fft_reg.h", "sc_solve_common.h"]
 
 
 def hipcc() -> str:

@@ -321,6 +321,13 @@ int sc_mesh_set_green_reuse(sc_mesh *mesh, int enable)
     return SC_OK;
 }
 
+int sc_mesh_force_generic_transforms(sc_mesh *mesh, int enable)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    mesh->m.generic_transforms = enable != 0;
+    return SC_OK;
+}
+
 // ---- bounds (src/mesh.jl:95-174) ----
 int sc_bounds_from_particles(sc_ctx *ctx, int pdtype, const void *x, const void *y, const void *z,
                              int64_t n, const int64_t grid[3], int dtype, double lo[3],

@@ -134,6 +134,7 @@ struct Mesh {
     void *efield = nullptr;
     bool owns = false;
     bool green_reuse = false;
+    bool generic_transforms = false;  // diagnostic: skip the register-resident engine
     Workspace ws;
     DepositBins bins;
     void *efi = nullptr;  // node-interleaved field for the interpolation gather

@@ -278,10 +278,15 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         std::memcpy(ws.g_key, key, sizeof(key));
     }
 
+    // Each transform kernel runs on the register-resident engine when its
+    // length is a supported power of two, else on the generic LDS engine.
+    const bool fast_ok = !m->generic_transforms;
+    auto fast = [&](int which) { return fast_ok && launch_fast<T>(m, which, cath); };
+
     // ---- rho forward x, y ----
     cpx<T> *A = static_cast<cpx<T> *>(ws.A);
     cpx<T> *B = static_cast<cpx<T> *>(ws.B);
-    {
+    if (!fast(0)) {
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
         launch_begin(ctx, K_RHO_FWD_X);
@@ -289,7 +294,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
                            static_cast<const T *>(m->rho), A, twx, ws.plan[0], d, nb);
         launch_end(ctx, K_RHO_FWD_X);
     }
-    {
+    if (!fast(1)) {
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
         launch_begin(ctx, K_RHO_FWD_Y);
@@ -298,7 +303,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         launch_end(ctx, K_RHO_FWD_Y);
     }
     // ---- fused z convolution ----
-    {
+    if (!fast(2)) {
         const int nbuf = cath ? 3 : 2;
         int nb = pick_nb(d.Lz, fft_cap<T>(), 16);
         while (nb > 1 && sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz) > 65536) nb /= 2;
@@ -310,7 +315,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         launch_end(ctx, K_ZCONV);
     }
     // ---- backward y, x ----
-    {
+    if (!fast(3)) {
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
         launch_begin(ctx, K_INV_Y);
@@ -318,7 +323,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
                            twy, ws.plan[1], d, nb);
         launch_end(ctx, K_INV_Y);
     }
-    {
+    if (!fast(4)) {
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
         launch_begin(ctx, K_INV_X);

@@ -46,4 +46,10 @@ inline int pick_nb(int n, int cap, int nbmax)
 template <typename T>
 int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cathode);
 
+// Field-solve kernel `which` (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
+// 4 inv_x) on the register-resident engine; false when its transform length
+// is not one of the supported powers of two (nothing is launched then).
+template <typename T>
+bool launch_fast(Mesh *m, int which, int cathode);
+
 }  // namespace sc

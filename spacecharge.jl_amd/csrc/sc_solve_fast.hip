@@ -1,0 +1,295 @@
+// sc_solve_fast.hip -- register-resident transform kernels of the field solve
+// for power-of-two padded lengths (32..1024).
+//
+// Same data flow and layouts as the generic kernels in sc_solve.hip (which
+// remain the path for other lengths), with every pencil held in registers:
+// a lane loads its E elements straight from HBM, the pencil is transformed
+// with one LDS exchange per pass boundary, and the lane stores straight to
+// HBM.  Which inputs are zero (the padded half) and which outputs are kept
+// (the physical octant, the Hermitian half) depends only on the register
+// index, so the pruning is resolved at compile time.
+#include <hip/hip_runtime.h>
+
+#include "sc_fft_reg.h"
+#include "sc_solve_common.h"
+
+namespace sc {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+template <typename T, int L>
+struct Shape {
+    static constexpr int E = reg::elems<T>() < L ? reg::elems<T>() : L;
+    static constexpr int TPP = L / E;         // lanes per pencil
+    static constexpr int NB = kThreads / TPP;  // pencils per workgroup
+};
+
+// rho x pass: real rows (y, z), zero-padded to 2nx, kx in [0, nx] to A.
+template <typename T, int L>
+__global__ void __launch_bounds__(kThreads)
+k_rho_fwd_x_r(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg,
+              Dims d)
+{
+    using S = Shape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
+    const int r = blockIdx.x * NB + b;
+    const int nrows = d.ny * d.nz, nx = d.nx;
+    const bool live = r < nrows;
+    cpx<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        v[q] = cpx<T>{(T)0, (T)0};
+        if (q < E / 2 && live) v[q].x = rho[(size_t)r * nx + j + q * TPP];
+    }
+    reg::fft<T, L, E, -1>(v, j, lds + b * L, 1, tw);
+    if (live) {
+        const int y = r % d.ny, z = r / d.ny;
+        cpx<T> *out = A + ((size_t)z * d.Ly + y) * d.Kx;
+#pragma unroll
+        for (int q = 0; q <= E / 2; ++q) {
+            const int k = j + q * TPP;
+            if (q < E / 2 || k == nx) out[k] = v[q];
+        }
+    }
+}
+
+// rho y pass: pencils (kx, z < nz) along y, ny inputs zero-padded to 2ny.
+template <typename T, int L>
+__global__ void __launch_bounds__(kThreads)
+k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
+{
+    using S = Shape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    const int b = threadIdx.x % NB, j = threadIdx.x / NB;
+    const int kx = blockIdx.x * NB + b, z = blockIdx.y;
+    const int Kx = d.Kx;
+    const bool live = kx < Kx;
+    cpx<T> *Ap = A + (size_t)z * d.Ly * Kx + kx;
+    cpx<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        v[q] = cpx<T>{(T)0, (T)0};
+        if (q < E / 2 && live) v[q] = Ap[(size_t)(j + q * TPP) * Kx];
+    }
+    reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) Ap[(size_t)(j + q * TPP) * Kx] = v[q];
+    }
+}
+
+// Fused z convolution (see k_zconv in sc_solve.hip for the algebra).
+template <typename T, int L, bool CATH>
+__global__ void __launch_bounds__(kThreads)
+k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
+          const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d)
+{
+    using S = Shape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    const int b = threadIdx.x % NB, j = threadIdx.x / NB;
+    const int kx = blockIdx.x * NB + b, ky = blockIdx.y;
+    const int Kx = d.Kx, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
+    const bool live = kx < Kx;
+    const size_t zs = (size_t)Ly * Kx;
+    const cpx<T> *Ap = A + (size_t)ky * Kx + kx;
+    cpx<T> p[E], pi[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        p[q] = cpx<T>{(T)0, (T)0};
+        pi[q] = cpx<T>{(T)0, (T)0};
+        if (q < E / 2 && live) {
+            const int z = j + q * TPP;
+            p[q] = Ap[(size_t)z * zs];
+            if constexpr (CATH) {
+                const cpx<T> t = Ap[(size_t)(nz - 1 - z) * zs];
+                pi[q] = cpx<T>{-t.x, -t.y};
+            }
+        }
+    }
+    reg::fft<T, L, E, -1>(p, j, lds + b, NB, tw);
+    if constexpr (CATH) reg::fft<T, L, E, -1>(pi, j, lds + b, NB, tw);
+    const bool kyneg = ky > ny;
+    const int kyp = kyneg ? Ly - ky : ky;
+    const size_t M1 = (size_t)(nz + 1) * ny1 * Kx;
+    const size_t MI = (size_t)L * ny1 * Kx;
+    const int kxc = live ? kx : 0;  // keep dead lanes' loads in bounds
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
+        cpx<T> w[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int kz = j + q * TPP;
+            const bool kzneg = kz > nz;
+            const int kzp = kzneg ? L - kz : kz;
+            const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
+            const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kx + kxc];
+            w[q] = cpx<T>{-p[q].y * g, p[q].x * g};  // rho_hat * (i g)
+            if constexpr (CATH) {
+                cpx<T> gi = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kx + kxc];
+                gi.x *= sy;
+                gi.y *= sy;
+                if (c < 2) gi = cpx<T>{-gi.y, gi.x};
+                w[q] = c_add(w[q], c_mul(pi[q], gi));
+            }
+        }
+        reg::fft<T, L, E, +1>(w, j, lds + b, NB, tw);
+        if (live) {
+            cpx<T> *Bc = B + (size_t)c * nz * zs + (size_t)ky * Kx + kx;
+#pragma unroll
+            for (int q = 0; q < E / 2; ++q) Bc[(size_t)(j + q * TPP) * zs] = w[q];
+        }
+    }
+}
+
+// Backward y pass: pencils (kx, z < nz) of each field spectrum, y < ny kept.
+template <typename T, int L>
+__global__ void __launch_bounds__(kThreads)
+k_inv_y_r(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, Dims d)
+{
+    using S = Shape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    const int b = threadIdx.x % NB, j = threadIdx.x / NB;
+    const int kx = blockIdx.x * NB + b, z = blockIdx.y, c = blockIdx.z;
+    const int Kx = d.Kx;
+    const bool live = kx < Kx;
+    cpx<T> *Bp = B + ((size_t)c * d.nz + z) * d.Ly * Kx + (live ? kx : 0);
+    cpx<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = Bp[(size_t)(j + q * TPP) * Kx];
+    reg::fft<T, L, E, +1>(v, j, lds + b, NB, tw);
+    if (live) {
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q) Bp[(size_t)(j + q * TPP) * Kx] = v[q];
+    }
+}
+
+// Backward x pass (complex to real) with Hermitian completion.
+template <typename T, int L>
+__global__ void __launch_bounds__(kThreads)
+k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__restrict__ twg, Dims d)
+{
+    using S = Shape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
+    const int r = blockIdx.x * NB + b, c = blockIdx.y;
+    const int nrows = d.ny * d.nz, nx = d.nx;
+    const bool live = r < nrows;
+    const int rr = live ? r : 0;
+    const int y = rr % d.ny, z = rr / d.ny;
+    const cpx<T> *row = B + (size_t)c * d.nz * d.Ly * d.Kx + ((size_t)z * d.Ly + y) * d.Kx;
+    cpx<T> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int k = j + q * TPP;
+        if (q < E / 2 || k == nx) {
+            v[q] = row[k];
+        } else {
+            const cpx<T> t = row[L - k];
+            v[q] = cpx<T>{t.x, -t.y};
+        }
+    }
+    reg::fft<T, L, E, +1>(v, j, lds + b * L, 1, tw);
+    if (live) {
+        T *out = E_ + (size_t)c * nx * nrows + (size_t)r * nx;
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q) out[j + q * TPP] = v[q].x;
+    }
+}
+
+template <typename T, int L>
+void launch_all(Mesh *m, const Dims &d, int axis_kernel, int cath)
+{
+    Workspace &ws = m->ws;
+    Ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    using S = Shape<T, L>;
+    const cpx<T> *tw = static_cast<const cpx<T> *>(ws.twT[axis_kernel == 0 || axis_kernel == 4 ? 0
+                                                           : axis_kernel == 2         ? 2
+                                                                                      : 1]);
+    cpx<T> *A = static_cast<cpx<T> *>(ws.A);
+    cpx<T> *B = static_cast<cpx<T> *>(ws.B);
+    switch (axis_kernel) {
+    case 0:
+        launch_begin(ctx, K_RHO_FWD_X);
+        hipLaunchKernelGGL((k_rho_fwd_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB)), dim3(kThreads),
+                           0, st, static_cast<const T *>(m->rho), A, tw, d);
+        launch_end(ctx, K_RHO_FWD_X);
+        break;
+    case 1:
+        launch_begin(ctx, K_RHO_FWD_Y);
+        hipLaunchKernelGGL((k_rho_fwd_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz), dim3(kThreads),
+                           0, st, A, tw, d);
+        launch_end(ctx, K_RHO_FWD_Y);
+        break;
+    case 2:
+        launch_begin(ctx, K_ZCONV);
+        if (cath)
+            hipLaunchKernelGGL((k_zconv_r<T, L, true>), dim3(div_up(d.Kx, S::NB), d.Ly),
+                               dim3(kThreads), 0, st, (const cpx<T> *)A, B,
+                               static_cast<const T *>(ws.Gf), static_cast<const cpx<T> *>(ws.Gi),
+                               tw, d);
+        else
+            hipLaunchKernelGGL((k_zconv_r<T, L, false>), dim3(div_up(d.Kx, S::NB), d.Ly),
+                               dim3(kThreads), 0, st, (const cpx<T> *)A, B,
+                               static_cast<const T *>(ws.Gf), static_cast<const cpx<T> *>(ws.Gi),
+                               tw, d);
+        launch_end(ctx, K_ZCONV);
+        break;
+    case 3:
+        launch_begin(ctx, K_INV_Y);
+        hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, 3), dim3(kThreads), 0,
+                           st, B, tw, d);
+        launch_end(ctx, K_INV_Y);
+        break;
+    case 4:
+        launch_begin(ctx, K_INV_X);
+        hipLaunchKernelGGL((k_inv_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), 3), dim3(kThreads), 0,
+                           st, (const cpx<T> *)B, static_cast<T *>(m->efield), tw, d);
+        launch_end(ctx, K_INV_X);
+        break;
+    }
+}
+
+}  // namespace
+
+// Runs one field-solve kernel (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
+// 4 inv_x) with the register engine if its transform length is a supported
+// power of two; returns false (nothing launched) otherwise.
+template <typename T>
+bool launch_fast(Mesh *m, int which, int cath)
+{
+    const Dims d = make_dims(m);
+    const int L = (which == 0 || which == 4) ? d.Lx : (which == 2 ? d.Lz : d.Ly);
+    switch (L) {
+    case 32: launch_all<T, 32>(m, d, which, cath); return true;
+    case 64: launch_all<T, 64>(m, d, which, cath); return true;
+    case 128: launch_all<T, 128>(m, d, which, cath); return true;
+    case 256: launch_all<T, 256>(m, d, which, cath); return true;
+    case 512: launch_all<T, 512>(m, d, which, cath); return true;
+    default: return false;
+    }
+}
+
+template bool launch_fast<float>(Mesh *, int, int);
+template bool launch_fast<double>(Mesh *, int, int);
+
+}  // namespace sc

@@ -207,6 +207,10 @@ class Mesh3D:
         """Reuse the transformed Green function while delta/gamma/cathode are unchanged."""
         _check(_lib.load().sc_mesh_set_green_reuse(self._h, int(bool(enable))), None)
 
+    def force_generic_transforms(self, enable: bool = True) -> None:
+        """Diagnostic: run every transform on the generic mixed-radix engine."""
+        _check(_lib.load().sc_mesh_force_generic_transforms(self._h, int(bool(enable))), None)
+
 
 def _particles(mesh: Mesh3D, arrays, names) -> list:
     out = []

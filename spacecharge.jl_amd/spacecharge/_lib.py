@@ -41,6 +41,7 @@ PROTOTYPES = {
     "sc_mesh_efield": (_vp, [_vp]),
     "sc_mesh_workspace_bytes": (_i64, [_vp]),
     "sc_mesh_set_green_reuse": (_i, [_vp, _i]),
+    "sc_mesh_force_generic_transforms": (_i, [_vp, _i]),
     "sc_bounds_from_particles": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pi64, _i, _pd, _pd, _pd]),
     "sc_clear_mesh": (_i, [_vp]),
     "sc_deposit": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64, _pd, _pd, _i]),

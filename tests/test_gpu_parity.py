@@ -82,7 +82,7 @@ def test_interpolate_bit_identical(pt, mt):
 
 
 def _solve_pair(grid, T, gamma=1.0, at_cathode=False, n=20_000, seed=4, sigma=(1e-3, 1e-3, 2e-3),
-                zshift=0.0):
+                zshift=0.0, generic=False):
     """Deposit + solve on the GPU, and the Float64 oracle solve of the same rho."""
     sc = _sc()
     x, y, z = gaussian(n, sigma, seed)
@@ -90,6 +90,8 @@ def _solve_pair(grid, T, gamma=1.0, at_cathode=False, n=20_000, seed=4, sigma=(1
     q = np.full(n, 1e-9 / n)
     xt, yt, zt, qt = _tensors((x, y, z, q), torch.float64)
     mesh = sc.Mesh3D(grid, xt, yt, zt, T=T, gamma=gamma)
+    if generic:
+        mesh.force_generic_transforms()
     sc.deposit(mesh, xt, yt, zt, qt)
     sc.solve(mesh, at_cathode=at_cathode)
     torch.cuda.synchronize()
@@ -101,7 +103,8 @@ def _solve_pair(grid, T, gamma=1.0, at_cathode=False, n=20_000, seed=4, sigma=(1
 
 
 SOLVE_CASES = [((4, 4, 4), 1.0), ((6, 6, 6), 1.0), ((8, 8, 8), 2.0), ((16, 8, 12), 1.0),
-               ((10, 14, 20), 3.0), ((32, 32, 32), 1.0), ((24, 20, 40), 10.0)]
+               ((10, 14, 20), 3.0), ((32, 32, 32), 1.0), ((24, 20, 40), 10.0),
+               ((64, 32, 128), 2.0), ((128, 64, 32), 1.0)]
 
 
 @pytest.mark.parametrize("grid,gamma", SOLVE_CASES)
@@ -120,7 +123,18 @@ def test_solve_fp32_matches_fp64_oracle(grid, gamma):
     assert max(errs) <= 1e-5
 
 
-@pytest.mark.parametrize("grid,gamma", [((8, 8, 8), 1.0), ((16, 16, 32), 2.0), ((12, 10, 6), 1.0)])
+@pytest.mark.parametrize("T", [torch.float64, torch.float32])
+@pytest.mark.parametrize("grid,gamma", [((32, 32, 32), 1.0), ((64, 32, 128), 3.0)])
+def test_generic_engine_matches_oracle(grid, gamma, T):
+    """The generic mixed-radix engine on lengths the register engine also takes."""
+    got, ref = _solve_pair(grid, T, gamma, generic=True)
+    errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
+    print(f"generic engine {T} {grid}: rel L2 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= (1e-11 if T == torch.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("grid,gamma", [((8, 8, 8), 1.0), ((16, 16, 32), 2.0), ((12, 10, 6), 1.0),
+                                        ((32, 64, 32), 1.5)])
 @pytest.mark.parametrize("T", [torch.float64, torch.float32])
 def test_cathode_matches_oracle(grid, gamma, T):
     got, ref = _solve_pair(grid, T, gamma, at_cathode=True, zshift=4e-3)
