@@ -51,6 +51,7 @@ def _load():
     lib.sco_field_green_function_f32.argtypes = [ctypes.c_float] * 3
     lib.sco_field_green_function_f32.restype = ctypes.c_float
     lib.sco_green_lattice.argtypes = [i, pll, pd, d, i, pd, vp, i]
+    lib.sco_solve_extended.argtypes = [pll, pd, pd, pd, d, i, vp, vp, i]
     lib.sco_fft3d.argtypes = [i, pll, vp, i, i]
     lib.sco_fpei.restype = d
     _lib = lib
@@ -145,6 +146,19 @@ def solve(rho, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0):
                        float(gamma), int(bool(at_cathode)), _ptr(rho), _ptr(ef), nthreads)
     if rc != 0:
         raise OracleError(f"solve failed ({rc})")
+    return ef
+
+
+def solve_extended(rho, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0):
+    """The reference algorithm in x87 extended precision (64-bit mantissa),
+    rounded to Float64 at the end: a yardstick for Float64 round-off."""
+    lib = _load()
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    ef = np.zeros(rho.shape + (3,), dtype=np.float64, order="F")
+    rc = lib.sco_solve_extended(_ll3(rho.shape), _d3(lo), _d3(hi), _d3(delta), float(gamma),
+                                int(bool(at_cathode)), _ptr(rho), _ptr(ef), nthreads)
+    if rc != 0:
+        raise OracleError(f"solve_extended failed ({rc})")
     return ef
 
 

@@ -41,15 +41,15 @@
 #define SCO_CLIGHT 299792458.0
 #define SCO_FPEI (SCO_CLIGHT * SCO_CLIGHT * 1.0e-7)
 
-/* cos and sin of -2*pi*k/n, evaluated in x87 extended precision so that the
- * double result is within an ulp. */
-static void sco_twiddle(long long k, long long n, double *c, double *s)
+/* cos and sin of -2*pi*k/n in x87 extended precision (rounded by the caller
+ * to the transform's precision). */
+static void sco_twiddle(long long k, long long n, long double *c, long double *s)
 {
     const long double pi = 3.141592653589793238462643383279502884L;
     long long kk = k % n;
     long double a = (2.0L * pi * (long double)kk) / (long double)n;
-    *c = (double)cosl(a);
-    *s = (double)(-sinl(a));
+    *c = cosl(a);
+    *s = -sinl(a);
 }
 
 static int sco_threads(int nthreads)
@@ -73,6 +73,18 @@ static int sco_threads(int nthreads)
 #define sqrt__f32 sqrtf
 #define atan__f32 atanf
 #define log__f32 logf
+#include "sc_oracle_body.h"
+#undef R
+#undef SFX
+
+/* ---- x87 extended instantiation: the same algorithm with a 64-bit
+ * mantissa, used by the tests as the "truth" against which both the Float64
+ * oracle and the GPU's Float64 path are measured ---- */
+#define R long double
+#define SFX(name) name##_f80
+#define sqrt__f80 sqrtl
+#define atan__f80 atanl
+#define log__f80 logl
 #include "sc_oracle_body.h"
 #undef R
 #undef SFX
@@ -314,6 +326,28 @@ int sco_solve(int mdt, const long long grid[3], const double lo[3], const double
         return solve_f32(rho, efield, g, l, h, d, (float)gamma, at_cathode, nth);
     }
     return -4;
+}
+
+/* The solve in x87 extended precision: rho and efield cross as Float64
+ * arrays, bounds/spacings/gamma as the Float64 mesh values. */
+int sco_solve_extended(const long long grid[3], const double lo[3], const double hi[3],
+                       const double delta[3], double gamma, int at_cathode, const double *rho,
+                       double *efield, int nthreads)
+{
+    const int g[3] = {(int)grid[0], (int)grid[1], (int)grid[2]};
+    const long long M = grid[0] * grid[1] * grid[2];
+    long double *r = (long double *)malloc(sizeof(long double) * (size_t)M);
+    long double *e = (long double *)malloc(sizeof(long double) * 3 * (size_t)M);
+    if (!r || !e) { free(r); free(e); return -1; }
+    for (long long i = 0; i < M; ++i) r[i] = rho[i];
+    const long double l[3] = {lo[0], lo[1], lo[2]};
+    const long double h[3] = {hi[0], hi[1], hi[2]};
+    const long double d[3] = {delta[0], delta[1], delta[2]};
+    int rc = solve_f80(r, e, g, l, h, d, (long double)gamma, at_cathode, sco_threads(nthreads));
+    for (long long i = 0; i < 3 * M; ++i) efield[i] = (double)e[i];
+    free(r);
+    free(e);
+    return rc;
 }
 
 /* ---- internals exposed for the oracle's own tests ---- */

@@ -42,7 +42,7 @@ static void SFX(fftplan_init)(SFX(fftplan) *p, int n)
     }
     p->tw = (R *)malloc(sizeof(R) * 2 * (size_t)n);
     for (int k = 0; k < n; ++k) {
-        double c, s;
+        long double c, s;
         sco_twiddle(k, n, &c, &s);
         p->tw[2 * k] = (R)c;
         p->tw[2 * k + 1] = (R)s;

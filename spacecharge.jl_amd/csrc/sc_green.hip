@@ -27,6 +27,7 @@
 // has a complex z spectrum.
 #include <hip/hip_runtime.h>
 
+#include "sc_fft_reg.h"
 #include "sc_solve_common.h"
 
 namespace sc {
@@ -289,6 +290,223 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     }
 }
 
+// ---- register-resident versions for power-of-two lengths (sc_fft_reg.h) ----
+constexpr int kRThreads = 256;
+
+template <typename T, int L>
+struct RShape {
+    static constexpr int E = reg::elems<T>() < L ? reg::elems<T>() : L;
+    static constexpr int TPP = L / E;
+    static constexpr int NB = kRThreads / TPP;
+};
+
+template <typename T, int L>
+__global__ void __launch_bounds__(kRThreads)
+k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restrict__ twg,
+           GreenArgs a)
+{
+    using S = RShape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
+    const int q = blockIdx.y, c = q % 3, image = q / 3;
+    const int nx = a.d.nx, ny = a.d.ny, Kx = a.d.Kx;
+    const int bpairs = (ny + 1) / 2;
+    const int npencils = z_cells(a, image) * bpairs;
+    const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
+    const int P = blockIdx.x * NB + b;
+    const bool live = P < npencils;
+    const int k = live ? P / bpairs : 0, ya = live ? 2 * (P - k * bpairs) : 0;
+    const bool hasb = ya + 1 < ny;
+    const double *Fq = F + f_off(a, q);
+    const T sgn = (c == 0) ? (T)-1 : (T)1;
+    cpx<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int e = j + r * TPP;
+        v[r] = cpx<T>{(T)0, (T)0};
+        if (live && e != nx) {
+            const int i = e < nx ? e : L - e;  // symmetric extension
+            const T s = e < nx ? (T)1 : sgn;
+            v[r].x = s * (T)diff8(Fq, i, ya, k, nx + 1, ny + 1);
+            if (hasb) v[r].y = s * (T)diff8(Fq, i, ya + 1, k, nx + 1, ny + 1);
+        }
+    }
+    reg::fft<T, L, E, -1>(v, j, lds + b * L, 1, tw);
+    if (live) {
+        T *Rq = R + r_off(a, q);
+#pragma unroll
+        for (int r = 0; r <= E / 2; ++r) {
+            const int kx = j + r * TPP;
+            if (r < E / 2 || kx == nx) {
+                T ra, rb;
+                unpack_pair(v[r], c == 0, ra, rb);
+                Rq[((size_t)k * (ny + 1) + ya) * Kx + kx] = ra;
+                if (hasb) Rq[((size_t)k * (ny + 1) + ya + 1) * Kx + kx] = rb;
+            }
+        }
+    }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(kRThreads)
+k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
+{
+    using S = RShape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int k = blockIdx.y;
+    if (k >= z_cells(a, image)) return;  // uniform over the workgroup
+    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
+    const int ny = a.d.ny, Kx = a.d.Kx;
+    const int b = threadIdx.x % NB, j = threadIdx.x / NB;
+    const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
+    T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kx;
+    const T sgn = (c == 1) ? (T)-1 : (T)1;
+    cpx<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int e = j + r * TPP;
+        v[r] = cpx<T>{(T)0, (T)0};
+        if (e != ny) {
+            const int y = e < ny ? e : L - e;
+            const T s = e < ny ? (T)1 : sgn;
+            if (kxa < Kx) v[r].x = s * Rrow[(size_t)y * Kx + kxa];
+            if (kxb < Kx) v[r].y = s * Rrow[(size_t)y * Kx + kxb];
+        }
+    }
+    reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
+#pragma unroll
+    for (int r = 0; r <= E / 2; ++r) {
+        const int ky = j + r * TPP;
+        if (r < E / 2 || ky == ny) {
+            T ra, rb;
+            unpack_pair(v[r], c == 1, ra, rb);
+            if (kxa < Kx) Rrow[(size_t)ky * Kx + kxa] = ra;
+            if (kxb < Kx) Rrow[(size_t)ky * Kx + kxb] = rb;
+        }
+    }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(kRThreads)
+k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
+           const cpx<T> *__restrict__ twg, GreenArgs a)
+{
+    using S = RShape<T, L>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
+    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int ky = blockIdx.y;
+    const int nz = a.d.nz, Kx = a.d.Kx, ny1 = a.d.ny + 1;
+    const int b = threadIdx.x % NB, j = threadIdx.x / NB;
+    const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
+    const T *Rq = R + r_off(a, q);
+    const T sgn = (c == 2) ? (T)-1 : (T)1;
+    cpx<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int e = j + r * TPP;
+        int cell = -1;
+        T s = (T)1;
+        if (!image) {
+            if (e != nz) {
+                cell = e < nz ? e : L - e;
+                s = e < nz ? (T)1 : sgn;
+            }
+        } else if (e != nz) {
+            // slot e holds displacement e (e < nz) or e - L (e > nz); row = displacement + nz - 1
+            cell = e < nz ? e + nz - 1 : e - nz - 1;
+        }
+        v[r] = cpx<T>{(T)0, (T)0};
+        if (cell >= 0) {
+            const T *row = Rq + ((size_t)cell * ny1 + ky) * Kx;
+            if (kxa < Kx) v[r].x = s * row[kxa];
+            if (kxb < Kx) v[r].y = s * row[kxb];
+        }
+    }
+    reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
+    const T scale = (T)a.scale[c];
+    if (!image) {
+        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kx;
+#pragma unroll
+        for (int r = 0; r <= E / 2; ++r) {
+            const int kz = j + r * TPP;
+            if (r < E / 2 || kz == nz) {
+                T ra, rb;
+                unpack_pair(v[r], c == 2, ra, rb);
+                T *row = G + ((size_t)kz * ny1 + ky) * Kx;
+                if (kxa < Kx) row[kxa] = ra * scale;
+                if (kxb < Kx) row[kxb] = rb * scale;
+            }
+        }
+    } else {
+        // the separation needs X[-kz], held by another lane: go through LDS
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < E; ++r) lds[b + (j + r * TPP) * NB] = v[r];
+        __syncthreads();
+        cpx<T> *G = Gi + (size_t)c * L * ny1 * Kx;
+        const T h = (T)0.5 * scale;
+#pragma unroll
+        for (int r = 0; r < E; ++r) {
+            const int kz = j + r * TPP;
+            const cpx<T> X = v[r];
+            const cpx<T> Y = lds[b + ((L - kz) & (L - 1)) * NB];
+            cpx<T> *row = G + ((size_t)kz * ny1 + ky) * Kx;
+            if (kxa < Kx) row[kxa] = cpx<T>{(X.x + Y.x) * h, (X.y - Y.y) * h};
+            if (kxb < Kx) row[kxb] = cpx<T>{(X.y + Y.y) * h, (Y.x - X.x) * h};
+        }
+    }
+}
+
+template <typename T, int L>
+void green_pass_r(int pass, dim3 grid, hipStream_t st, const Workspace &ws, const GreenArgs &a,
+                  const cpx<T> *tw)
+{
+    T *R = static_cast<T *>(ws.R);
+    if (pass == 1)
+        hipLaunchKernelGGL((k_green_1r<T, L>), grid, dim3(kRThreads), 0, st, (const double *)ws.F, R,
+                           tw, a);
+    else if (pass == 2)
+        hipLaunchKernelGGL((k_green_2r<T, L>), grid, dim3(kRThreads), 0, st, R, tw, a);
+    else
+        hipLaunchKernelGGL((k_green_3r<T, L>), grid, dim3(kRThreads), 0, st, (const T *)R,
+                           static_cast<T *>(ws.Gf), static_cast<cpx<T> *>(ws.Gi), tw, a);
+}
+
+// One register-engine Green pass if its length is supported; false otherwise.
+template <typename T>
+bool green_pass_fast(int pass, int L, int ncomp, const Dims &d, int cath, hipStream_t st,
+                     const Workspace &ws, const GreenArgs &a, const cpx<T> *tw)
+{
+    auto grid_for = [&](int nb) -> dim3 {
+        const int zc = cath ? d.Lz - 1 : d.nz;
+        const int kp = (d.Kx + 1) / 2;
+        if (pass == 1) return dim3(div_up(zc * ((d.ny + 1) / 2), nb), ncomp);
+        if (pass == 2) return dim3(div_up(kp, nb), zc, ncomp);
+        return dim3(div_up(kp, nb), d.ny + 1, ncomp);
+    };
+    switch (L) {
+#define SC_GREEN_CASE(LL)                                                                   \
+    case LL:                                                                                \
+        green_pass_r<T, LL>(pass, grid_for(RShape<T, LL>::NB), st, ws, a, tw);             \
+        return true;
+        SC_GREEN_CASE(32)
+        SC_GREEN_CASE(64)
+        SC_GREEN_CASE(128)
+        SC_GREEN_CASE(256)
+        SC_GREEN_CASE(512)
+#undef SC_GREEN_CASE
+    default: return false;
+    }
+}
+
 }  // namespace
 
 template <typename T>
@@ -323,35 +541,36 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cath
                            ws.F, a);
         launch_end(ctx, K_GREEN_F);
     }
-    {
+    const bool fast_ok = !m->generic_transforms;
+    launch_begin(ctx, K_GREEN_X);
+    if (!(fast_ok && green_pass_fast<T>(1, d.Lx, ncomp, d, cath, st, ws, a, twx))) {
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const int zc = cath ? d.Lz - 1 : d.nz;
         const int np = zc * ((d.ny + 1) / 2);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
-        launch_begin(ctx, K_GREEN_X);
         hipLaunchKernelGGL(k_green_1<T>, dim3(div_up(np, nb), ncomp), dim3(kFftThreads), sh, st,
                            (const double *)ws.F, R, twx, ws.plan[0], a, nb);
-        launch_end(ctx, K_GREEN_X);
     }
+    launch_end(ctx, K_GREEN_X);
     const int kp = (d.Kx + 1) / 2;
-    {
+    launch_begin(ctx, K_GREEN_Y);
+    if (!(fast_ok && green_pass_fast<T>(2, d.Ly, ncomp, d, cath, st, ws, a, twy))) {
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const int zc = cath ? d.Lz - 1 : d.nz;
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
-        launch_begin(ctx, K_GREEN_Y);
         hipLaunchKernelGGL(k_green_2<T>, dim3(div_up(kp, nb), zc, ncomp), dim3(kFftThreads), sh, st,
                            R, twy, ws.plan[1], a, nb);
-        launch_end(ctx, K_GREEN_Y);
     }
-    {
+    launch_end(ctx, K_GREEN_Y);
+    launch_begin(ctx, K_GREEN_Z);
+    if (!(fast_ok && green_pass_fast<T>(3, d.Lz, ncomp, d, cath, st, ws, a, twz))) {
         const int nb = pick_nb(d.Lz, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lz + (size_t)nb * d.Lz);
-        launch_begin(ctx, K_GREEN_Z);
         hipLaunchKernelGGL(k_green_3<T>, dim3(div_up(kp, nb), d.ny + 1, ncomp), dim3(kFftThreads),
                            sh, st, (const T *)R, static_cast<T *>(ws.Gf),
                            static_cast<cpx<T> *>(ws.Gi), twz, ws.plan[2], a, nb);
-        launch_end(ctx, K_GREEN_Z);
     }
+    launch_end(ctx, K_GREEN_Z);
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
 }

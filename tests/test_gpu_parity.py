@@ -4,13 +4,15 @@ Tolerances, and why:
   * interpolate: bit-identical to the oracle for the same efield (both keep
     the reference's evaluation order and neither contracts into FMAs).
   * deposit: each particle's 8 contributions are bit-identical; only the
-    order of the atomic sums differs, so rho agrees to a few ulp of the
-    largest node value (1e-13 relative for Float64, 1e-5 for Float32).
-  * solve, Float64: relative L2 per component against the oracle's
-    reference-order computation.  The bound (1e-11 here, measured values
-    are printed) is set by the reference algorithm's own round-off: the
-    8-corner difference of the antiderivative cancels ~log2(N^3) bits at
-    far cells, and the two implementations round differently there.
+    order of the sums differs, so rho agrees to a few ulp of the largest node
+    value (1e-13 relative for Float64, 1e-5 for Float32).
+  * solve, Float64: the reference algorithm is itself only accurate to
+    ~1e-13..3e-12 in Float64 (the 8-corner difference of the antiderivative
+    cancels ~log2(N^3) bits at far cells).  The yardstick is therefore the
+    same algorithm in x87 extended precision (oracle.solve_extended): the GPU
+    must be at least as close to it as the Float64 oracle is (plus 1e-14),
+    and within 1e-12 or the oracle's own error, whichever is larger, of the
+    Float64 oracle.  Measured values are printed.
   * solve, Float32 mesh: relative L2 <= 1e-5 against the Float64 oracle
     (the Green function is built in Float64, transforms run in Float32).
 """
@@ -99,7 +101,23 @@ def _solve_pair(grid, T, gamma=1.0, at_cathode=False, n=20_000, seed=4, sigma=(1
     got = mesh_to_np(mesh._ef, grid)
     ref = orc.solve(np.asfortranarray(rho), mesh.min_bounds, mesh.max_bounds, mesh.delta,
                     mesh.gamma, at_cathode)
+    if T == torch.float64:
+        truth = orc.solve_extended(rho, mesh.min_bounds, mesh.max_bounds, mesh.delta, mesh.gamma,
+                                   at_cathode)
+        return got, ref, truth
     return got, ref
+
+
+def _check_fp64(label, got, ref, truth):
+    """GPU at least as close to the extended-precision result as the Float64 oracle."""
+    e_gpu = [rel_l2(got[..., c], truth[..., c]) for c in range(3)]
+    e_orc = [rel_l2(ref[..., c], truth[..., c]) for c in range(3)]
+    e_vs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
+    print(f"{label}: rel L2 GPU-vs-oracle {['%.2e' % e for e in e_vs]}; vs extended: "
+          f"GPU {['%.2e' % e for e in e_gpu]} oracle {['%.2e' % e for e in e_orc]}")
+    for c in range(3):
+        assert e_gpu[c] <= e_orc[c] + 1e-14
+        assert e_vs[c] <= max(1e-12, 2 * e_orc[c])
 
 
 SOLVE_CASES = [((4, 4, 4), 1.0), ((6, 6, 6), 1.0), ((8, 8, 8), 2.0), ((16, 8, 12), 1.0),
@@ -109,10 +127,7 @@ SOLVE_CASES = [((4, 4, 4), 1.0), ((6, 6, 6), 1.0), ((8, 8, 8), 2.0), ((16, 8, 12
 
 @pytest.mark.parametrize("grid,gamma", SOLVE_CASES)
 def test_solve_fp64_matches_oracle(grid, gamma):
-    got, ref = _solve_pair(grid, torch.float64, gamma)
-    errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
-    print(f"solve f64 {grid} gamma={gamma}: rel L2 {['%.2e' % e for e in errs]}")
-    assert max(errs) <= 1e-11
+    _check_fp64(f"solve f64 {grid} gamma={gamma}", *_solve_pair(grid, torch.float64, gamma))
 
 
 @pytest.mark.parametrize("grid,gamma", SOLVE_CASES)
@@ -127,20 +142,28 @@ def test_solve_fp32_matches_fp64_oracle(grid, gamma):
 @pytest.mark.parametrize("grid,gamma", [((32, 32, 32), 1.0), ((64, 32, 128), 3.0)])
 def test_generic_engine_matches_oracle(grid, gamma, T):
     """The generic mixed-radix engine on lengths the register engine also takes."""
-    got, ref = _solve_pair(grid, T, gamma, generic=True)
+    res = _solve_pair(grid, T, gamma, generic=True)
+    if T == torch.float64:
+        _check_fp64(f"generic engine f64 {grid}", *res)
+        return
+    got, ref = res
     errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
     print(f"generic engine {T} {grid}: rel L2 {['%.2e' % e for e in errs]}")
-    assert max(errs) <= (1e-11 if T == torch.float64 else 1e-5)
+    assert max(errs) <= 1e-5
 
 
 @pytest.mark.parametrize("grid,gamma", [((8, 8, 8), 1.0), ((16, 16, 32), 2.0), ((12, 10, 6), 1.0),
                                         ((32, 64, 32), 1.5)])
 @pytest.mark.parametrize("T", [torch.float64, torch.float32])
 def test_cathode_matches_oracle(grid, gamma, T):
-    got, ref = _solve_pair(grid, T, gamma, at_cathode=True, zshift=4e-3)
+    res = _solve_pair(grid, T, gamma, at_cathode=True, zshift=4e-3)
+    if T == torch.float64:
+        _check_fp64(f"cathode f64 {grid} gamma={gamma}", *res)
+        return
+    got, ref = res
     errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
     print(f"cathode {T} {grid} gamma={gamma}: rel L2 {['%.2e' % e for e in errs]}")
-    assert max(errs) <= (1e-11 if T == torch.float64 else 1e-5)
+    assert max(errs) <= 1e-5
 
 
 @pytest.mark.parametrize("grid", [(20, 33, 17), (1025, 1025, 18)])

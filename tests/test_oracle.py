@@ -349,3 +349,19 @@ def test_float32_reference_path_vs_float64():
     errs = [rel_l2(ef32[..., c], ef64[..., c]) for c in range(3)]
     print("reference Float32 path vs Float64 oracle, rel L2:", errs)
     assert max(errs) < 1e-2
+
+
+@pytest.mark.parametrize("grid,gamma", [((16, 16, 16), 1.0), ((24, 20, 40), 10.0)])
+def test_reference_float64_roundoff_floor(grid, gamma):
+    """The reference algorithm's own Float64 error, against the same algorithm
+    in x87 extended precision: nonzero and growing with the grid, ~1e-13 at
+    16^3 and ~1.5e-12 at 24x20x40 with gamma = 10.  The GPU Float64 parity
+    tests are measured against this floor."""
+    x, y, z, q = config_a(20000, seed=4)
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid)
+    rho = orc.deposit(x, y, z, q, grid, lo, dl)
+    e64 = orc.solve(rho, lo, hi, dl, gamma)
+    ext = orc.solve_extended(rho, lo, hi, dl, gamma)
+    errs = [rel_l2(e64[..., c], ext[..., c]) for c in range(3)]
+    print(f"oracle Float64 vs extended {grid}: {errs}")
+    assert 1e-16 < max(errs) < 1e-11
