@@ -197,6 +197,17 @@ def main():
     }
     breakdown = {k: round(v[0] / v[1] * 1e3, 2) for k, v in sorted(per_kernel.items(),
                                                                       key=lambda kv: -kv[1][0])}
+    # whole step: compulsory I/O of the reference decomposition, 10*Np*s + 168*M*s
+    # (deposit 4Np+M, interpolate 6Np+3M, solve 4M + 20P with P = 8M), cathode 339*M*s
+    M = grid[0] * grid[1] * grid[2]
+    b_step = 10 * n * s + (339 if args.cathode else 168) * M * s
+    step_roofline = {
+        "algorithmic_bytes": b_step,
+        "achieved": b_step / (ms * 1e-3) / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": b_step / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    }
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -225,6 +236,7 @@ def main():
             "parallelism": "single GPU",
         },
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "cpu_baseline": cpu,
         "kernel_us_per_launch": breakdown,
         "kernel_share": {k: round(v[0] / tot, 3) for k, v in per_kernel.items()},

@@ -365,3 +365,19 @@ def test_reference_float64_roundoff_floor(grid, gamma):
     errs = [rel_l2(e64[..., c], ext[..., c]) for c in range(3)]
     print(f"oracle Float64 vs extended {grid}: {errs}")
     assert 1e-16 < max(errs) < 1e-11
+
+
+def test_reference_self_field_is_roundoff():
+    """test/test_interpolation.jl:109-111 asserts |E| > 1e7 at a single charge's
+    own node.  The reference path (the oracle) meets it, but only through
+    round-off of its asymmetrically rounded lattice positions: the value is
+    ~13 orders below the field one cell away, and the exact self-field is 0."""
+    x = np.array([0.0])
+    g = (8, 8, 8)
+    lo, hi, dl = orc.bounds_auto(x, x, x, g)
+    rho = orc.deposit(x, x, x, np.array([1.0]), g, lo, dl)
+    ef = orc.solve(rho, lo, hi, dl)
+    at_charge = np.abs(ef[0, 0, 0, :])
+    one_cell = np.array([ef[1, 0, 0, 0], ef[0, 1, 0, 1], ef[0, 0, 1, 2]])
+    assert (at_charge > 1e7).all()            # the reference's assertion holds ...
+    assert (at_charge < 1e-12 * one_cell).all()  # ... at round-off level
