@@ -88,6 +88,31 @@ def kernel_bytes(name, n, grid, s, ps):
     return None
 
 
+def measured_traffic(name, n, grid, dtype_name, cathode):
+    """HBM bytes per launch of kernel `name` from a committed counter summary.
+
+    The summaries under profiles/hbm_traffic_*.md carry a fenced ```json block
+    with the workload they were measured on and, per kernel, FETCH_SIZE +
+    WRITE_SIZE per dispatch scaled by the copy-kernel yardstick of the same
+    runs.  Returns (bytes, file) for a summary of this exact workload, else
+    (None, None).
+    """
+    import glob
+    import re
+    want = {"particles": n, "grid": list(grid), "dtype": dtype_name, "at_cathode": bool(cathode)}
+    for path in sorted(glob.glob(os.path.join(REPO, "profiles", "hbm_traffic_*.md"))):
+        with open(path) as f:
+            text = f.read()
+        for block in re.findall(r"```json\n(.*?)\n```", text, re.S):
+            try:
+                d = json.loads(block)
+            except ValueError:
+                continue
+            if d.get("workload") == want and name in d.get("bytes_per_launch", {}):
+                return d["bytes_per_launch"][name], os.path.relpath(path, REPO)
+    return None, None
+
+
 def cpu_baseline(args, n, grid, dtype_name, max_seconds):
     """Time the CPU oracle on a bounded sample: whole steps of the same workload."""
     import numpy as np
@@ -134,6 +159,9 @@ def main():
     ap.add_argument("--cathode", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the CPU-oracle sample (0 disables it)")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="launch a plain copy kernel (read N MiB, write N MiB) three times "
+                         "before the warmup, as the yardstick of HBM counter runs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -156,6 +184,15 @@ def main():
         sc.deposit(mesh, x, y, z, q)
         sc.solve(mesh, at_cathode=args.cathode)
         return sc.interpolate_field(mesh, x, y, z)
+
+    if args.calibration_copy_mib > 0:
+        # elementwise negation: one read and one write of every byte, no reuse
+        src = torch.ones(args.calibration_copy_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     for _ in range(args.warmup):
         step()
@@ -184,6 +221,7 @@ def main():
     dom_name, (dom_ms, dom_n) = dom
     avg_s = dom_ms / dom_n * 1e-3
     nbytes = kernel_bytes(dom_name, n, grid, s, s)
+    traffic, traffic_src = measured_traffic(dom_name, n, grid, args.dtype, args.cathode)
     roofline = {
         "bound": "hbm",
         "kernel": dom_name,
@@ -191,7 +229,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": (nbytes / avg_s / 1e9 / HBM_PEAK_GBS) if nbytes else None,
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "algorithmic_bytes": nbytes,
         "avg_launch_us": avg_s * 1e6,
     }

@@ -10,12 +10,15 @@
 // copy of the tile's (17x17x17-node) sub-grid, and a node-parallel pass sums
 // the sub-grids into rho.  No global atomic touches rho.
 //
-//   bin_count   per-particle tile, workgroup histogram in LDS, one global
-//               atomic per (workgroup, non-empty tile)
+//   bin_count   per-particle tile; each workgroup histograms one contiguous
+//               slice of the particles in LDS and stores it as one column of
+//               a [tile][slice] count matrix (no global atomics)
+//   bin_rowscan one wave per tile: prefix of its row over the slices, and
+//               the tile's total
 //   bin_scan    one workgroup: tile offsets and the (tile, start, length)
 //               chunk table, chunks of at most kChunk particles
-//   bin_scatter recount with LDS ranks, reserve each tile's run with one
-//               global atomic, write (x, y, z, q) records in tile order
+//   bin_scatter each slice writes (x, y, z, q) records into its reserved run
+//               of each tile, slots from an LDS cursor per tile
 //   deposit_tiles  one workgroup per chunk: Float64 LDS sub-grid, CIC with
 //               the reference's arithmetic, sub-grid stored to `partials`
 //   gather_tiles   one lane per node: sum over the chunks of the (up to 8)
@@ -43,6 +46,9 @@ constexpr int kBinThreads = 256;
 constexpr int kBinPerThread = 16;    // particles per lane in the binning passes
 constexpr int kBinChunk = kBinThreads * kBinPerThread;
 constexpr int kChunk = 8192;         // particles per deposit workgroup
+constexpr int kBinMaxSlices = 2048;  // binning workgroups (columns of the count matrix)
+
+inline int div_up4(int a) { return (a + 3) / 4; }
 
 template <typename A, typename B>
 using wider_t = typename std::conditional<(sizeof(A) >= sizeof(B)), A, B>::type;
@@ -80,132 +86,160 @@ __device__ __forceinline__ int tile_of(PT px, PT py, PT pz, const Geo &g)
     return ix / kTile + g.tx * (iy / kTile + g.ty * (iz / kTile));
 }
 
+// The binning workgroups split the particles into G contiguous slices of
+// `slice` particles; all three binning kernels use the same split.
+struct Split {
+    int64_t n, slice;
+    int G;
+};
+
+// Per-slice tile histogram, written as column w of the [tile][G] matrix.
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
-            int64_t n, Geo g, unsigned *__restrict__ counts)
+            Split sp, Geo g, unsigned *__restrict__ cmat)
 {
     __shared__ unsigned hist[kMaxTiles];
     const int ntiles = g.tx * g.ty * g.tz;
     for (int t = threadIdx.x; t < ntiles; t += kBinThreads) hist[t] = 0u;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kBinChunk;
+    const int64_t first = (int64_t)blockIdx.x * sp.slice;
+    const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
+    for (int64_t base = first; base < end; base += kBinChunk) {
 #pragma unroll 4
-    for (int s = 0; s < kBinPerThread; ++s) {
-        const int64_t p = base + s * kBinThreads + threadIdx.x;
-        if (p < n) atomicAdd(&hist[tile_of<PT, MT>(px[p], py[p], pz[p], g)], 1u);
+        for (int s = 0; s < kBinPerThread; ++s) {
+            const int64_t p = base + s * kBinThreads + threadIdx.x;
+            if (p < end) atomicAdd(&hist[tile_of<PT, MT>(px[p], py[p], pz[p], g)], 1u);
+        }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads) {
-        const unsigned c = hist[t];
-        if (c) atomicAdd(&counts[t], c);
-    }
+    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
+        cmat[(size_t)t * sp.G + blockIdx.x] = hist[t];
 }
 
-// One workgroup.  counts -> cursor (exclusive offsets), each tile's first
-// chunk, and the chunk table.
-__global__ void __launch_bounds__(1024)
+// One wave per tile: exclusive prefix of the tile's row over the slices (in
+// place), and the tile's total.
+__global__ void __launch_bounds__(256)
+k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restrict__ tot)
+{
+    constexpr int kPer = kBinMaxSlices / 64;
+    const int lane = threadIdx.x & 63;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    unsigned *row = cmat + (size_t)t * G;
+    // lane l owns the V consecutive slices [l V, l V + V); all loads first
+    const int V = (G + 63) / 64;
+    const int w0 = lane * V;
+    unsigned v[kPer];
+    unsigned s = 0u;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        v[i] = (i < V && w0 + i < G) ? row[w0 + i] : 0u;
+        s += v[i];
+    }
+    unsigned incl = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    unsigned run = incl - s;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        if (i < V && w0 + i < G) row[w0 + i] = run;
+        run += v[i];
+    }
+    if (lane == 63) tot[t] = incl;
+}
+
+// One workgroup.  Per-tile totals -> tile start (exclusive offsets), each
+// tile's first chunk, and the chunk table.
+__global__ void __launch_bounds__(256)
 k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
            unsigned *__restrict__ chunk_base, int4 *__restrict__ chunks,
            unsigned *__restrict__ nchunks, int ntiles)
 {
-    __shared__ unsigned s_off[kMaxTiles];
-    __shared__ unsigned s_chk[kMaxTiles];
-    __shared__ unsigned s_sum[2][1024];
-    const int T = blockDim.x;
-    const int per = (ntiles + T - 1) / T;
+    constexpr int kPer = kMaxTiles / 256;
+    __shared__ unsigned s_w[2][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // thread i owns the `per` consecutive tiles [i per, i per + per)
+    const int per = (ntiles + 255) / 256;
     const int t0 = threadIdx.x * per;
-    unsigned a = 0, b = 0;
-    for (int i = 0; i < per; ++i) {
-        const int t = t0 + i;
-        if (t < ntiles) {
-            const unsigned c = counts[t];
-            a += c;
-            b += (c + kChunk - 1) / kChunk;
+    unsigned c[kPer];
+    unsigned a = 0u, b = 0u;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        c[i] = (i < per && t0 + i < ntiles) ? counts[t0 + i] : 0u;
+        a += c[i];
+        b += (c[i] + kChunk - 1) / kChunk;
+    }
+    unsigned ia = a, ib = b;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned oa = __shfl_up(ia, d, 64), ob = __shfl_up(ib, d, 64);
+        if (lane >= d) {
+            ia += oa;
+            ib += ob;
         }
     }
-    s_sum[0][threadIdx.x] = a;
-    s_sum[1][threadIdx.x] = b;
+    if (lane == 63) {
+        s_w[0][wv] = ia;
+        s_w[1][wv] = ib;
+    }
     __syncthreads();
-    // inclusive Hillis-Steele scan of the per-lane sums
-    for (int d = 1; d < T; d <<= 1) {
-        unsigned va = 0, vb = 0;
-        if ((int)threadIdx.x >= d) {
-            va = s_sum[0][threadIdx.x - d];
-            vb = s_sum[1][threadIdx.x - d];
-        }
-        __syncthreads();
-        s_sum[0][threadIdx.x] += va;
-        s_sum[1][threadIdx.x] += vb;
-        __syncthreads();
+    unsigned oa = ia - a, ob = ib - b;
+    for (int k = 0; k < wv; ++k) {
+        oa += s_w[0][k];
+        ob += s_w[1][k];
     }
-    unsigned oa = threadIdx.x ? s_sum[0][threadIdx.x - 1] : 0u;
-    unsigned ob = threadIdx.x ? s_sum[1][threadIdx.x - 1] : 0u;
-    for (int i = 0; i < per; ++i) {
+    if (threadIdx.x == 255) *nchunks = ob + b;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
         const int t = t0 + i;
-        if (t < ntiles) {
-            const unsigned c = counts[t];
-            s_off[t] = oa;
-            s_chk[t] = ob;
+        if (i < per && t < ntiles) {
             cursor[t] = oa;
             chunk_base[t] = ob;
-            oa += c;
-            ob += (c + kChunk - 1) / kChunk;
-        }
-    }
-    if (threadIdx.x == T - 1) *nchunks = s_sum[1][T - 1];
-    __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += T) {
-        const unsigned c = counts[t];
-        const unsigned k0 = s_chk[t];
-        const unsigned nk = (c + kChunk - 1) / kChunk;
-        for (unsigned k = 0; k < nk; ++k) {
-            const unsigned start = s_off[t] + k * kChunk;
-            const unsigned len = (c - k * kChunk) < (unsigned)kChunk ? (c - k * kChunk) : kChunk;
-            chunks[k0 + k] = make_int4(t, (int)start, (int)len, 0);
+            const unsigned nk = (c[i] + kChunk - 1) / kChunk;
+            for (unsigned k = 0; k < nk; ++k) {
+                const unsigned len = (c[i] - k * kChunk) < (unsigned)kChunk ? (c[i] - k * kChunk) : kChunk;
+                chunks[ob + k] = make_int4(t, (int)(oa + k * kChunk), (int)len, 0);
+            }
+            oa += c[i];
+            ob += nk;
         }
     }
 }
 
+// Each slice's particles go to its reserved run inside each tile: the LDS
+// cursor of tile t starts at start[t] + (particles of t in earlier slices),
+// and an LDS atomic hands out the slots.  The same split and the same
+// tile_of as k_bin_count make every run exactly as long as counted.
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
-              const PT *__restrict__ pq, int64_t n, Geo g, unsigned *__restrict__ cursor,
-              Rec<PT> *__restrict__ sorted)
+              const PT *__restrict__ pq, Split sp, Geo g, const unsigned *__restrict__ cmat,
+              const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted)
 {
-    __shared__ unsigned hist[kMaxTiles];
+    __shared__ unsigned cur[kMaxTiles];
     const int ntiles = g.tx * g.ty * g.tz;
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads) hist[t] = 0u;
+    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
+        cur[t] = start[t] + cmat[(size_t)t * sp.G + blockIdx.x];
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * kBinChunk;
-    int tl[kBinPerThread];
-    unsigned rk[kBinPerThread];
-#pragma unroll
-    for (int s = 0; s < kBinPerThread; ++s) {
-        const int64_t p = base + s * kBinThreads + threadIdx.x;
-        tl[s] = -1;
-        rk[s] = 0;
-        if (p < n) {
-            tl[s] = tile_of<PT, MT>(px[p], py[p], pz[p], g);
-            rk[s] = atomicAdd(&hist[tl[s]], 1u);
-        }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads) {
-        const unsigned c = hist[t];
-        if (c) hist[t] = atomicAdd(&cursor[t], c);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kBinPerThread; ++s) {
-        const int64_t p = base + s * kBinThreads + threadIdx.x;
-        if (tl[s] >= 0) {
-            Rec<PT> r;
-            r.x = px[p];
-            r.y = py[p];
-            r.z = pz[p];
-            r.q = pq[p];
-            sorted[hist[tl[s]] + rk[s]] = r;
+    const int64_t first = (int64_t)blockIdx.x * sp.slice;
+    const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
+    for (int64_t base = first; base < end; base += kBinChunk) {
+#pragma unroll 4
+        for (int s = 0; s < kBinPerThread; ++s) {
+            const int64_t p = base + s * kBinThreads + threadIdx.x;
+            if (p < end) {
+                Rec<PT> r;
+                r.x = px[p];
+                r.y = py[p];
+                r.z = pz[p];
+                r.q = pq[p];
+                const unsigned slot = atomicAdd(&cur[tile_of<PT, MT>(r.x, r.y, r.z, g)], 1u);
+                sorted[slot] = r;
+            }
         }
     }
 }
@@ -281,12 +315,13 @@ __global__ void __launch_bounds__(256)
 k_gather_tiles(const double *__restrict__ partials, const unsigned *__restrict__ counts,
                const unsigned *__restrict__ chunk_base, Geo g, MT *__restrict__ rho, int accumulate)
 {
-    const int64_t M = (int64_t)g.nx * g.ny * g.nz;
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // at most kMaxTiles tiles of 16^3 cells: node indices fit in 32 bits
+    const int M = g.nx * g.ny * g.nz;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= M) return;
-    const int gx = (int)(i % g.nx);
-    const int gy = (int)((i / g.nx) % g.ny);
-    const int gz = (int)(i / ((int64_t)g.nx * g.ny));
+    const int gx = i % g.nx;
+    const int gy = (i / g.nx) % g.ny;
+    const int gz = i / (g.nx * g.ny);
     int cx[2], cy[2], cz[2], nxc = 0, nyc = 0, nzc = 0;
     if (gx / kTile < g.tx) cx[nxc++] = gx / kTile;
     if (gx % kTile == 0 && gx >= kTile) cx[nxc++] = gx / kTile - 1;
@@ -303,19 +338,33 @@ k_gather_tiles(const double *__restrict__ partials, const unsigned *__restrict__
                 if (nk == 0) continue;
                 const int local = (gx - kTile * cx[a]) + kSub * ((gy - kTile * cy[b]) + kSub * (gz - kTile * cz[c]));
                 const double *p = partials + (size_t)chunk_base[t] * kSubVol + local;
-                // four independent partial sums keep several loads in flight
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                // eight independent partial sums keep eight loads in flight
+                // (the densest tiles hold a few dozen chunks)
+                double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
                 unsigned k = 0;
-                for (; k + 4 <= nk; k += 4) {
-                    s0 += p[(size_t)k * kSubVol];
-                    s1 += p[(size_t)(k + 1) * kSubVol];
-                    s2 += p[(size_t)(k + 2) * kSubVol];
-                    s3 += p[(size_t)(k + 3) * kSubVol];
+                for (; k + 8 <= nk; k += 8) {
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(k + u) * kSubVol];
                 }
-                for (; k < nk; ++k) s0 += p[(size_t)k * kSubVol];
-                s += (s0 + s1) + (s2 + s3);
+#pragma unroll
+                for (int u = 0; u < 7; ++u)
+                    if (k + u < nk) acc[u] += p[(size_t)(k + u) * kSubVol];
+                s += ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
             }
     rho[i] = accumulate ? (MT)((double)rho[i] + s) : (MT)s;
+}
+
+// At most kBinMaxSlices slices of whole rounds of kBinChunk particles.
+Split split_of(int64_t n)
+{
+    Split sp;
+    sp.n = n;
+    int64_t G = (n + kBinChunk - 1) / kBinChunk;
+    if (G > kBinMaxSlices) G = kBinMaxSlices;
+    if (G < 1) G = 1;
+    sp.G = (int)G;
+    sp.slice = (n + G - 1) / G;
+    return sp;
 }
 
 int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
@@ -346,9 +395,18 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
         B.chunk_cap = maxchunks;
     }
     if (!B.counts) {
-        // counts | cursor | chunk_base | nchunks
+        // totals | tile start | chunk_base | nchunks
         if (hipMalloc((void **)&B.counts, sizeof(unsigned) * (3 * kMaxTiles + 1)) != hipSuccess)
             return fail();
+    }
+    const int64_t need_cmat = (int64_t)ntiles * split_of(n).G;
+    if (B.cmat_cap < need_cmat) {
+        if (B.cmat) (void)hipFree(B.cmat);
+        B.cmat = nullptr;
+        B.cmat_cap = 0;
+        if (hipMalloc((void **)&B.cmat, sizeof(unsigned) * (size_t)need_cmat) != hipSuccess)
+            return fail();
+        B.cmat_cap = need_cmat;
     }
     return SC_OK;
 }
@@ -364,19 +422,23 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, 
     unsigned *counts = B.counts, *cursor = B.counts + kMaxTiles;
     unsigned *cbase = B.counts + 2 * kMaxTiles, *nch = B.counts + 3 * kMaxTiles;
     int4 *chunks = static_cast<int4 *>(B.chunks);
-    const unsigned nbin = (unsigned)((n + kBinChunk - 1) / kBinChunk);
-    if (int rc = launch_clear(ctx, counts, sizeof(unsigned) * ntiles)) return rc;
+    const Split sp = split_of(n);
     launch_begin(ctx, K_BIN_COUNT);
-    hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(nbin), dim3(kBinThreads), 0, st, (const PT *)x,
-                       (const PT *)y, (const PT *)z, n, g, counts);
+    hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
+                       (const PT *)y, (const PT *)z, sp, g, B.cmat);
     launch_end(ctx, K_BIN_COUNT);
+    launch_begin(ctx, K_BIN_ROWSCAN);
+    hipLaunchKernelGGL(k_bin_rowscan, dim3((unsigned)div_up4(ntiles)), dim3(256), 0, st, B.cmat,
+                       sp.G, ntiles, counts);
+    launch_end(ctx, K_BIN_ROWSCAN);
     launch_begin(ctx, K_BIN_SCAN);
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, st, (const unsigned *)counts, cursor,
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
                        cbase, chunks, nch, ntiles);
     launch_end(ctx, K_BIN_SCAN);
     launch_begin(ctx, K_BIN_SCATTER);
-    hipLaunchKernelGGL((k_bin_scatter<PT, MT>), dim3(nbin), dim3(kBinThreads), 0, st,
-                       (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q, n, g, cursor,
+    hipLaunchKernelGGL((k_bin_scatter<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st,
+                       (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q, sp, g,
+                       (const unsigned *)B.cmat, (const unsigned *)cursor,
                        static_cast<Rec<PT> *>(B.sorted));
     launch_end(ctx, K_BIN_SCATTER);
     const unsigned maxchunks = (unsigned)(ntiles + n / kChunk + 1);
@@ -404,6 +466,7 @@ void free_bins(Mesh *m)
     if (B.chunks) (void)hipFree(B.chunks);
     if (B.partials) (void)hipFree(B.partials);
     if (B.counts) (void)hipFree(B.counts);
+    if (B.cmat) (void)hipFree(B.cmat);
     B = DepositBins();
 }
 

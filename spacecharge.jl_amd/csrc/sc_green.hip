@@ -83,16 +83,19 @@ __device__ __forceinline__ void unpack_pair(cpx<T> X, bool odd, T &ra, T &rb)
     else { ra = X.x; rb = X.y; }
 }
 
+// grid (x-y plane blocks, z node planes, components); in-plane index math
+// in 32 bits (this kernel is bound by its Float64 instruction count)
 __global__ void __launch_bounds__(256) k_green_f(double *__restrict__ F, GreenArgs a)
 {
-    const int q = blockIdx.y, c = q % 3, image = q / 3;
+    const int q = blockIdx.z, c = q % 3, image = q / 3;
     const int nx1 = a.d.nx + 1, ny1 = a.d.ny + 1;
-    const int64_t nodes = (int64_t)nx1 * ny1 * z_nodes(a, image);
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= nodes) return;
-    const int in = (int)(idx % nx1);
-    const int jn = (int)((idx / nx1) % ny1);
-    const int kn = (int)(idx / ((int64_t)nx1 * ny1));
+    const int kn = blockIdx.y;
+    if (kn >= z_nodes(a, image)) return;
+    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pidx >= nx1 * ny1) return;
+    const int in = pidx % nx1;
+    const int jn = pidx / nx1;
+    const int64_t idx = (int64_t)kn * nx1 * ny1 + pidx;
     const double u = ((double)in - 0.5) * a.dx;
     const double v = ((double)jn - 0.5) * a.dy;
     const double w = image ? ((double)(kn - a.d.nz) + 0.5) * a.dz + a.offzg
@@ -126,7 +129,7 @@ k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restr
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + a.d.Lx;
     const int q = blockIdx.y, c = q % 3, image = q / 3;
-    const int nx = a.d.nx, ny = a.d.ny, Lx = a.d.Lx, Kx = a.d.Kx;
+    const int nx = a.d.nx, ny = a.d.ny, Lx = a.d.Lx, Kx = a.d.Kx, Kp = a.d.Kp;
     const int bpairs = (ny + 1) / 2;
     const int npencils = z_cells(a, image) * bpairs;
     const double *Fq = F + f_off(a, q);
@@ -164,8 +167,8 @@ k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restr
             const int k = P / bpairs, j = 2 * (P - k * bpairs);
             T ra, rb;
             unpack_pair(buf[p * Lx + kx], c == 0, ra, rb);
-            Rq[((size_t)k * (ny + 1) + j) * Kx + kx] = ra;
-            if (j + 1 < ny) Rq[((size_t)k * (ny + 1) + j + 1) * Kx + kx] = rb;
+            Rq[((size_t)k * (ny + 1) + j) * Kp + kx] = ra;
+            if (j + 1 < ny) Rq[((size_t)k * (ny + 1) + j + 1) * Kp + kx] = rb;
         }
     }
 }
@@ -180,15 +183,15 @@ k_green_2(T *__restrict__ R, const cpx<T> *__restrict__ twg, FftPlan pl, GreenAr
     const int q = blockIdx.z, c = q % 3, image = q / 3;
     const int k = blockIdx.y;
     if (k >= z_cells(a, image)) return;  // uniform over the workgroup
-    const int ny = a.d.ny, Ly = a.d.Ly, Kx = a.d.Kx;
-    T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kx;
+    const int ny = a.d.ny, Ly = a.d.Ly, Kx = a.d.Kx, Kp = a.d.Kp;
+    T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kp;
     load_twiddles(tw, twg, Ly);
     const int kp0 = blockIdx.x * nb;
     for (int idx = threadIdx.x; idx < nb * ny; idx += blockDim.x) {
         const int e = idx / nb, p = idx - e * nb;
         const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
-        const T va = kxa < Kx ? Rrow[(size_t)e * Kx + kxa] : (T)0;
-        const T vb = kxb < Kx ? Rrow[(size_t)e * Kx + kxb] : (T)0;
+        const T va = kxa < Kx ? Rrow[(size_t)e * Kp + kxa] : (T)0;
+        const T vb = kxb < Kx ? Rrow[(size_t)e * Kp + kxb] : (T)0;
         buf[e * nb + p] = cpx<T>{va, vb};
     }
     __syncthreads();
@@ -209,8 +212,8 @@ k_green_2(T *__restrict__ R, const cpx<T> *__restrict__ twg, FftPlan pl, GreenAr
         const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
         T ra, rb;
         unpack_pair(buf[ky * nb + p], c == 1, ra, rb);
-        if (kxa < Kx) Rrow[(size_t)ky * Kx + kxa] = ra;
-        if (kxb < Kx) Rrow[(size_t)ky * Kx + kxb] = rb;
+        if (kxa < Kx) Rrow[(size_t)ky * Kp + kxa] = ra;
+        if (kxb < Kx) Rrow[(size_t)ky * Kp + kxb] = rb;
     }
 }
 
@@ -224,7 +227,7 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     cpx<T> *buf = tw + a.d.Lz;
     const int q = blockIdx.z, c = q % 3, image = q / 3;
     const int ky = blockIdx.y;
-    const int nz = a.d.nz, Lz = a.d.Lz, Kx = a.d.Kx, ny1 = a.d.ny + 1;
+    const int nz = a.d.nz, Lz = a.d.Lz, Kx = a.d.Kx, Kp = a.d.Kp, ny1 = a.d.ny + 1;
     const T *Rq = R + r_off(a, q);
     load_twiddles(tw, twg, Lz);
     const int kp0 = blockIdx.x * nb;
@@ -241,7 +244,7 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
         }
         cpx<T> v{(T)0, (T)0};
         if (cell >= 0) {
-            const T *row = Rq + ((size_t)cell * ny1 + ky) * Kx;
+            const T *row = Rq + ((size_t)cell * ny1 + ky) * Kp;
             v.x = kxa < Kx ? row[kxa] : (T)0;
             v.y = kxb < Kx ? row[kxb] : (T)0;
         }
@@ -263,27 +266,27 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
     const T scale = (T)a.scale[c];
     if (!image) {
-        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kx;
+        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kp;
         for (int idx = threadIdx.x; idx < nb * (nz + 1); idx += blockDim.x) {
             const int kz = idx / nb, p = idx - kz * nb;
             const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
             T ra, rb;
             unpack_pair(buf[kz * nb + p], c == 2, ra, rb);
-            T *row = G + ((size_t)kz * ny1 + ky) * Kx;
+            T *row = G + ((size_t)kz * ny1 + ky) * Kp;
             if (kxa < Kx) row[kxa] = ra * scale;
             if (kxb < Kx) row[kxb] = rb * scale;
         }
     } else {
         // separate the two real sequences: A_k = (X_k + conj X_{-k})/2,
         // B_k = (X_k - conj X_{-k})/(2i)
-        cpx<T> *G = Gi + (size_t)c * Lz * ny1 * Kx;
+        cpx<T> *G = Gi + (size_t)c * Lz * ny1 * Kp;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
             const int kz = idx / nb, p = idx - kz * nb;
             const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
             const cpx<T> X = buf[kz * nb + p];
             const cpx<T> Y = buf[((Lz - kz) % Lz) * nb + p];
             const T h = (T)0.5 * scale;
-            cpx<T> *row = G + ((size_t)kz * ny1 + ky) * Kx;
+            cpx<T> *row = G + ((size_t)kz * ny1 + ky) * Kp;
             if (kxa < Kx) row[kxa] = cpx<T>{(X.x + Y.x) * h, (X.y - Y.y) * h};
             if (kxb < Kx) row[kxb] = cpx<T>{(X.y + Y.y) * h, (Y.x - X.x) * h};
         }
@@ -311,7 +314,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     __shared__ cpx<T> lds[NB * L];
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
     const int q = blockIdx.y, c = q % 3, image = q / 3;
-    const int nx = a.d.nx, ny = a.d.ny, Kx = a.d.Kx;
+    const int nx = a.d.nx, ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int bpairs = (ny + 1) / 2;
     const int npencils = z_cells(a, image) * bpairs;
     const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
@@ -342,8 +345,8 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
             if (r < E / 2 || kx == nx) {
                 T ra, rb;
                 unpack_pair(v[r], c == 0, ra, rb);
-                Rq[((size_t)k * (ny + 1) + ya) * Kx + kx] = ra;
-                if (hasb) Rq[((size_t)k * (ny + 1) + ya + 1) * Kx + kx] = rb;
+                Rq[((size_t)k * (ny + 1) + ya) * Kp + kx] = ra;
+                if (hasb) Rq[((size_t)k * (ny + 1) + ya + 1) * Kp + kx] = rb;
             }
         }
     }
@@ -361,10 +364,10 @@ k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
     const int k = blockIdx.y;
     if (k >= z_cells(a, image)) return;  // uniform over the workgroup
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
-    const int ny = a.d.ny, Kx = a.d.Kx;
+    const int ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
-    T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kx;
+    T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kp;
     const T sgn = (c == 1) ? (T)-1 : (T)1;
     cpx<T> v[E];
 #pragma unroll
@@ -374,8 +377,8 @@ k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
         if (e != ny) {
             const int y = e < ny ? e : L - e;
             const T s = e < ny ? (T)1 : sgn;
-            if (kxa < Kx) v[r].x = s * Rrow[(size_t)y * Kx + kxa];
-            if (kxb < Kx) v[r].y = s * Rrow[(size_t)y * Kx + kxb];
+            if (kxa < Kx) v[r].x = s * Rrow[(size_t)y * Kp + kxa];
+            if (kxb < Kx) v[r].y = s * Rrow[(size_t)y * Kp + kxb];
         }
     }
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
@@ -385,8 +388,8 @@ k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
         if (r < E / 2 || ky == ny) {
             T ra, rb;
             unpack_pair(v[r], c == 1, ra, rb);
-            if (kxa < Kx) Rrow[(size_t)ky * Kx + kxa] = ra;
-            if (kxb < Kx) Rrow[(size_t)ky * Kx + kxb] = rb;
+            if (kxa < Kx) Rrow[(size_t)ky * Kp + kxa] = ra;
+            if (kxb < Kx) Rrow[(size_t)ky * Kp + kxb] = rb;
         }
     }
 }
@@ -403,7 +406,7 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
     const int q = blockIdx.z, c = q % 3, image = q / 3;
     const int ky = blockIdx.y;
-    const int nz = a.d.nz, Kx = a.d.Kx, ny1 = a.d.ny + 1;
+    const int nz = a.d.nz, Kx = a.d.Kx, Kp = a.d.Kp, ny1 = a.d.ny + 1;
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
     const T *Rq = R + r_off(a, q);
@@ -425,7 +428,7 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
         }
         v[r] = cpx<T>{(T)0, (T)0};
         if (cell >= 0) {
-            const T *row = Rq + ((size_t)cell * ny1 + ky) * Kx;
+            const T *row = Rq + ((size_t)cell * ny1 + ky) * Kp;
             if (kxa < Kx) v[r].x = s * row[kxa];
             if (kxb < Kx) v[r].y = s * row[kxb];
         }
@@ -433,14 +436,14 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
     const T scale = (T)a.scale[c];
     if (!image) {
-        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kx;
+        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kp;
 #pragma unroll
         for (int r = 0; r <= E / 2; ++r) {
             const int kz = j + r * TPP;
             if (r < E / 2 || kz == nz) {
                 T ra, rb;
                 unpack_pair(v[r], c == 2, ra, rb);
-                T *row = G + ((size_t)kz * ny1 + ky) * Kx;
+                T *row = G + ((size_t)kz * ny1 + ky) * Kp;
                 if (kxa < Kx) row[kxa] = ra * scale;
                 if (kxb < Kx) row[kxb] = rb * scale;
             }
@@ -451,14 +454,14 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
 #pragma unroll
         for (int r = 0; r < E; ++r) lds[b + (j + r * TPP) * NB] = v[r];
         __syncthreads();
-        cpx<T> *G = Gi + (size_t)c * L * ny1 * Kx;
+        cpx<T> *G = Gi + (size_t)c * L * ny1 * Kp;
         const T h = (T)0.5 * scale;
 #pragma unroll
         for (int r = 0; r < E; ++r) {
             const int kz = j + r * TPP;
             const cpx<T> X = v[r];
             const cpx<T> Y = lds[b + ((L - kz) & (L - 1)) * NB];
-            cpx<T> *row = G + ((size_t)kz * ny1 + ky) * Kx;
+            cpx<T> *row = G + ((size_t)kz * ny1 + ky) * Kp;
             if (kxa < Kx) row[kxa] = cpx<T>{(X.x + Y.x) * h, (X.y - Y.y) * h};
             if (kxb < Kx) row[kxb] = cpx<T>{(X.y + Y.y) * h, (Y.x - X.x) * h};
         }
@@ -527,18 +530,19 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cath
     for (int c = 0; c < 3; ++c) a.scale[c] = kFPEI * ((c < 2 ? gamma : 1.0) / vol) / P;
     a.f_free = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     a.f_img = (int64_t)(d.nx + 1) * (d.ny + 1) * d.Lz;
-    a.r_free = (int64_t)d.nz * (d.ny + 1) * d.Kx;
-    a.r_img = (int64_t)(d.Lz - 1) * (d.ny + 1) * d.Kx;
+    a.r_free = (int64_t)d.nz * (d.ny + 1) * d.Kp;
+    a.r_img = (int64_t)(d.Lz - 1) * (d.ny + 1) * d.Kp;
     const int ncomp = cath ? 6 : 3;
     const cpx<T> *twx = static_cast<const cpx<T> *>(ws.twT[0]);
     const cpx<T> *twy = static_cast<const cpx<T> *>(ws.twT[1]);
     const cpx<T> *twz = static_cast<const cpx<T> *>(ws.twT[2]);
     T *R = static_cast<T *>(ws.R);
     {
-        const int64_t nodes = cath ? a.f_img : a.f_free;
+        const int plane = (d.nx + 1) * (d.ny + 1);
+        const int znodes = cath ? d.Lz : d.nz + 1;
         launch_begin(ctx, K_GREEN_F);
-        hipLaunchKernelGGL(k_green_f, dim3((unsigned)((nodes + 255) / 256), ncomp), dim3(256), 0, st,
-                           ws.F, a);
+        hipLaunchKernelGGL(k_green_f, dim3((unsigned)div_up(plane, 256), znodes, ncomp), dim3(256), 0,
+                           st, ws.F, a);
         launch_end(ctx, K_GREEN_F);
     }
     const bool fast_ok = !m->generic_transforms;

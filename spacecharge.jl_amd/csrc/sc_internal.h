@@ -38,6 +38,7 @@ enum KernelId : int {
     K_INTERLEAVE,
     K_GATHER_TILES,
     K_GREEN_F,
+    K_BIN_ROWSCAN,
     K_COUNT
 };
 extern const char *const kKernelNames[K_COUNT];
@@ -123,7 +124,9 @@ struct DepositBins {
     void *chunks = nullptr;       // int4 (tile, start, length, 0)
     void *partials = nullptr;     // one 17^3 sub-grid per chunk
     int64_t chunk_cap = 0;
-    unsigned *counts = nullptr;   // counts | cursor | chunk_base | nchunks
+    unsigned *counts = nullptr;   // per-tile totals | tile start | chunk_base | nchunks
+    unsigned *cmat = nullptr;     // [tile][binning workgroup] counts, then offsets
+    int64_t cmat_cap = 0;         // entries
 };
 
 struct Mesh {

@@ -62,7 +62,7 @@ k_rho_fwd_x(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *__r
         const int r = r0 + b;
         if (r < nrows) {
             const int y = r % d.ny, z = r / d.ny;
-            A[((size_t)z * d.Ly + y) * d.Kx + kx] = buf[b * Lx + kx];
+            A[((size_t)z * d.Ly + y) * d.Kp + kx] = buf[b * Lx + kx];
         }
     }
 }
@@ -75,15 +75,15 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + d.Ly;
-    const int Ly = d.Ly, Kx = d.Kx;
+    const int Ly = d.Ly, Kx = d.Kx, Kp = d.Kp;
     load_twiddles(tw, twg, Ly);
     const int kx0 = blockIdx.x * nb, z = blockIdx.y;
-    cpx<T> *Ap = A + (size_t)z * Ly * Kx;
+    cpx<T> *Ap = A + (size_t)z * Ly * Kp;
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
         cpx<T> v{(T)0, (T)0};
-        if (kx < Kx && e < d.ny) v = Ap[(size_t)e * Kx + kx];
+        if (kx < Kx && e < d.ny) v = Ap[(size_t)e * Kp + kx];
         buf[idx] = v;
     }
     __syncthreads();
@@ -91,7 +91,7 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
-        if (kx < Kx) Ap[(size_t)e * Kx + kx] = buf[idx];
+        if (kx < Kx) Ap[(size_t)e * Kp + kx] = buf[idx];
     }
 }
 
@@ -109,19 +109,19 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
     cpx<T> *P0 = tw + d.Lz;
     cpx<T> *W = P0 + nb * d.Lz;
     cpx<T> *P1 = W + nb * d.Lz;  // only with the cathode
-    const int Lz = d.Lz, nz = d.nz, Kx = d.Kx, Ly = d.Ly, ny = d.ny;
+    const int Lz = d.Lz, nz = d.nz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
     const int ny1 = ny + 1;
     load_twiddles(tw, twg, Lz);
     const int kx0 = blockIdx.x * nb, ky = blockIdx.y;
-    const size_t zs = (size_t)Ly * Kx;  // stride between z planes of A and B
+    const size_t zs = (size_t)Ly * Kp;  // stride between z planes of A and B
     for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
         cpx<T> v{(T)0, (T)0}, vi{(T)0, (T)0};
         if (kx < Kx && e < nz) {
-            v = A[(size_t)e * zs + (size_t)ky * Kx + kx];
+            v = A[(size_t)e * zs + (size_t)ky * Kp + kx];
             if (cathode) {
-                const cpx<T> t = A[(size_t)(nz - 1 - e) * zs + (size_t)ky * Kx + kx];
+                const cpx<T> t = A[(size_t)(nz - 1 - e) * zs + (size_t)ky * Kp + kx];
                 vi = cpx<T>{-t.x, -t.y};
             }
         }
@@ -133,8 +133,8 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
     if (cathode) lds_fft(P1, tw, pl, nb, nb, 1, true, -1);
     const bool kyneg = ky > ny;
     const int kyp = kyneg ? Ly - ky : ky;
-    const size_t M1 = (size_t)(nz + 1) * ny1 * Kx;  // one free-space Green spectrum
-    const size_t MI = (size_t)Lz * ny1 * Kx;        // one image Green spectrum
+    const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;  // one free-space Green spectrum
+    const size_t MI = (size_t)Lz * ny1 * Kp;        // one image Green spectrum
     for (int c = 0; c < 3; ++c) {
         const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
@@ -145,12 +145,12 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
                 const bool kzneg = kz > nz;
                 const int kzp = kzneg ? Lz - kz : kz;
                 const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
-                const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kx + kx];
+                const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kx];
                 const cpx<T> p = P0[idx];
                 // rho_hat * (i g)
                 out = cpx<T>{-p.y * g, p.x * g};
                 if (cathode) {
-                    cpx<T> q = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kx + kx];
+                    cpx<T> q = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kx];
                     q.x *= sy;
                     q.y *= sy;
                     if (c < 2) q = cpx<T>{-q.y, q.x};  // times i (odd in x or y)
@@ -165,7 +165,7 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
         for (int idx = threadIdx.x; idx < nb * nz; idx += blockDim.x) {
             const int z = idx / nb, b = idx - z * nb;
             const int kx = kx0 + b;
-            if (kx < Kx) Bc[(size_t)z * zs + (size_t)ky * Kx + kx] = W[idx];
+            if (kx < Kx) Bc[(size_t)z * zs + (size_t)ky * Kp + kx] = W[idx];
         }
         __syncthreads();
     }
@@ -179,15 +179,15 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + d.Ly;
-    const int Ly = d.Ly, Kx = d.Kx;
+    const int Ly = d.Ly, Kx = d.Kx, Kp = d.Kp;
     load_twiddles(tw, twg, Ly);
     const int kx0 = blockIdx.x * nb, z = blockIdx.y, c = blockIdx.z;
-    cpx<T> *Bp = B + ((size_t)c * d.nz + z) * Ly * Kx;
+    cpx<T> *Bp = B + ((size_t)c * d.nz + z) * Ly * Kp;
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
         cpx<T> v{(T)0, (T)0};
-        if (kx < Kx) v = Bp[(size_t)e * Kx + kx];
+        if (kx < Kx) v = Bp[(size_t)e * Kp + kx];
         buf[idx] = v;
     }
     __syncthreads();
@@ -195,7 +195,7 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
     for (int idx = threadIdx.x; idx < nb * d.ny; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
-        if (kx < Kx) Bp[(size_t)e * Kx + kx] = buf[idx];
+        if (kx < Kx) Bp[(size_t)e * Kp + kx] = buf[idx];
     }
 }
 
@@ -209,17 +209,17 @@ k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ E, const cpx<T> *__restric
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + d.Lx;
-    const int Lx = d.Lx, nx = d.nx, Kx = d.Kx, nrows = d.ny * d.nz;
+    const int Lx = d.Lx, nx = d.nx, Kx = d.Kx, Kp = d.Kp, nrows = d.ny * d.nz;
     load_twiddles(tw, twg, Lx);
     const int r0 = blockIdx.x * nb, c = blockIdx.y;
-    const cpx<T> *Bc = B + (size_t)c * d.nz * d.Ly * Kx;
+    const cpx<T> *Bc = B + (size_t)c * d.nz * d.Ly * Kp;
     for (int idx = threadIdx.x; idx < nb * Kx; idx += blockDim.x) {
         const int b = idx / Kx, kx = idx - b * Kx;
         const int r = r0 + b;
         cpx<T> v{(T)0, (T)0};
         if (r < nrows) {
             const int y = r % d.ny, z = r / d.ny;
-            v = Bc[((size_t)z * d.Ly + y) * Kx + kx];
+            v = Bc[((size_t)z * d.Ly + y) * Kp + kx];
         }
         buf[b * Lx + kx] = v;
     }
@@ -385,13 +385,13 @@ int ensure_workspace(Mesh *m)
                                std::to_string(cap / 2) + " per axis");
     }
     const size_t s = (m->dtype == SC_F32) ? 4 : 8;
-    const size_t Kx = nx + 1;
-    const size_t spec = (size_t)nz * 2 * ny * Kx;  // complex elements of one spectrum
-    const size_t gf = (size_t)(nz + 1) * (ny + 1) * Kx;
-    const size_t gi = (size_t)2 * nz * (ny + 1) * Kx;
+    const size_t Kp = make_dims(m).Kp;  // row stride of every kx-indexed array
+    const size_t spec = (size_t)nz * 2 * ny * Kp;  // complex elements of one spectrum
+    const size_t gf = (size_t)(nz + 1) * (ny + 1) * Kp;
+    const size_t gi = (size_t)2 * nz * (ny + 1) * Kp;
     // Green build scratch: F at the nodes (Float64), R rows (T), free + image
     const size_t fn = (size_t)(nx + 1) * (ny + 1) * ((nz + 1) + 2 * nz);
-    const size_t rn = (size_t)(ny + 1) * Kx * (nz + (2 * nz - 1));
+    const size_t rn = (size_t)(ny + 1) * Kp * (nz + (2 * nz - 1));
     auto fail = [&](void) {
         free_workspace(m);
         return set_err(ctx, SC_ERR_NOMEM, "device allocation of the solver workspace failed");

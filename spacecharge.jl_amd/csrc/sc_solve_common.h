@@ -14,6 +14,8 @@ constexpr double kFPEI = 299792458.0 * 299792458.0 * 1.0e-7;  // src/utils.jl:7-
 struct Dims {
     int nx, ny, nz;
     int Kx;          // nx + 1 kept x frequencies
+    int Kp;          // row stride of kx-indexed arrays: Kx rounded up to 32, so that the
+                     // 16-lane row segments of every transform start on a 128-byte line
     int Lx, Ly, Lz;  // padded lengths
 };
 
@@ -24,6 +26,7 @@ inline Dims make_dims(const Mesh *m)
     d.ny = (int)m->n[1];
     d.nz = (int)m->n[2];
     d.Kx = d.nx + 1;
+    d.Kp = (d.Kx + 31) / 32 * 32;
     d.Lx = 2 * d.nx;
     d.Ly = 2 * d.ny;
     d.Lz = 2 * d.nz;

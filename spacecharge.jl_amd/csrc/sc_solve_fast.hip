@@ -50,7 +50,7 @@ k_rho_fwd_x_r(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *_
     reg::fft<T, L, E, -1>(v, j, lds + b * L, 1, tw);
     if (live) {
         const int y = r % d.ny, z = r / d.ny;
-        cpx<T> *out = A + ((size_t)z * d.Ly + y) * d.Kx;
+        cpx<T> *out = A + ((size_t)z * d.Ly + y) * d.Kp;
 #pragma unroll
         for (int q = 0; q <= E / 2; ++q) {
             const int k = j + q * TPP;
@@ -71,19 +71,19 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
     for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kx = blockIdx.x * NB + b, z = blockIdx.y;
-    const int Kx = d.Kx;
+    const int Kx = d.Kx, Kp = d.Kp;
     const bool live = kx < Kx;
-    cpx<T> *Ap = A + (size_t)z * d.Ly * Kx + kx;
+    cpx<T> *Ap = A + (size_t)z * d.Ly * Kp + kx;
     cpx<T> v[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         v[q] = cpx<T>{(T)0, (T)0};
-        if (q < E / 2 && live) v[q] = Ap[(size_t)(j + q * TPP) * Kx];
+        if (q < E / 2 && live) v[q] = Ap[(size_t)(j + q * TPP) * Kp];
     }
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
     if (live) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) Ap[(size_t)(j + q * TPP) * Kx] = v[q];
+        for (int q = 0; q < E; ++q) Ap[(size_t)(j + q * TPP) * Kp] = v[q];
     }
 }
 
@@ -97,13 +97,13 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
     for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kx = blockIdx.x * NB + b, ky = blockIdx.y;
-    const int Kx = d.Kx, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
-    const bool live = kx < Kx;
-    const size_t zs = (size_t)Ly * Kx;
-    const cpx<T> *Ap = A + (size_t)ky * Kx + kx;
+    const bool live = kx < d.Kx;
+    const size_t zs = (size_t)Ly * Kp;
+    const cpx<T> *Ap = A + (size_t)ky * Kp + kx;
     cpx<T> p[E], pi[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -122,8 +122,8 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
     if constexpr (CATH) reg::fft<T, L, E, -1>(pi, j, lds + b, NB, tw);
     const bool kyneg = ky > ny;
     const int kyp = kyneg ? Ly - ky : ky;
-    const size_t M1 = (size_t)(nz + 1) * ny1 * Kx;
-    const size_t MI = (size_t)L * ny1 * Kx;
+    const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;
+    const size_t MI = (size_t)L * ny1 * Kp;
     const int kxc = live ? kx : 0;  // keep dead lanes' loads in bounds
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -135,10 +135,10 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
             const bool kzneg = kz > nz;
             const int kzp = kzneg ? L - kz : kz;
             const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
-            const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kx + kxc];
+            const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kxc];
             w[q] = cpx<T>{-p[q].y * g, p[q].x * g};  // rho_hat * (i g)
             if constexpr (CATH) {
-                cpx<T> gi = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kx + kxc];
+                cpx<T> gi = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kxc];
                 gi.x *= sy;
                 gi.y *= sy;
                 if (c < 2) gi = cpx<T>{-gi.y, gi.x};
@@ -147,7 +147,7 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
         }
         reg::fft<T, L, E, +1>(w, j, lds + b, NB, tw);
         if (live) {
-            cpx<T> *Bc = B + (size_t)c * nz * zs + (size_t)ky * Kx + kx;
+            cpx<T> *Bc = B + (size_t)c * nz * zs + (size_t)ky * Kp + kx;
 #pragma unroll
             for (int q = 0; q < E / 2; ++q) Bc[(size_t)(j + q * TPP) * zs] = w[q];
         }
@@ -166,16 +166,16 @@ k_inv_y_r(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, Dims d)
     for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kx = blockIdx.x * NB + b, z = blockIdx.y, c = blockIdx.z;
-    const int Kx = d.Kx;
+    const int Kx = d.Kx, Kp = d.Kp;
     const bool live = kx < Kx;
-    cpx<T> *Bp = B + ((size_t)c * d.nz + z) * d.Ly * Kx + (live ? kx : 0);
+    cpx<T> *Bp = B + ((size_t)c * d.nz + z) * d.Ly * Kp + (live ? kx : 0);
     cpx<T> v[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = Bp[(size_t)(j + q * TPP) * Kx];
+    for (int q = 0; q < E; ++q) v[q] = Bp[(size_t)(j + q * TPP) * Kp];
     reg::fft<T, L, E, +1>(v, j, lds + b, NB, tw);
     if (live) {
 #pragma unroll
-        for (int q = 0; q < E / 2; ++q) Bp[(size_t)(j + q * TPP) * Kx] = v[q];
+        for (int q = 0; q < E / 2; ++q) Bp[(size_t)(j + q * TPP) * Kp] = v[q];
     }
 }
 
@@ -195,7 +195,7 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
     const bool live = r < nrows;
     const int rr = live ? r : 0;
     const int y = rr % d.ny, z = rr / d.ny;
-    const cpx<T> *row = B + (size_t)c * d.nz * d.Ly * d.Kx + ((size_t)z * d.Ly + y) * d.Kx;
+    const cpx<T> *row = B + (size_t)c * d.nz * d.Ly * d.Kp + ((size_t)z * d.Ly + y) * d.Kp;
     cpx<T> v[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -240,20 +240,22 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, int cath)
                            0, st, A, tw, d);
         launch_end(ctx, K_RHO_FWD_Y);
         break;
-    case 2:
+    case 2: {
         launch_begin(ctx, K_ZCONV);
+        const dim3 zgrid(div_up(d.Kx, S::NB), d.Ly);
         if (cath)
-            hipLaunchKernelGGL((k_zconv_r<T, L, true>), dim3(div_up(d.Kx, S::NB), d.Ly),
+            hipLaunchKernelGGL((k_zconv_r<T, L, true>), zgrid,
                                dim3(kThreads), 0, st, (const cpx<T> *)A, B,
                                static_cast<const T *>(ws.Gf), static_cast<const cpx<T> *>(ws.Gi),
                                tw, d);
         else
-            hipLaunchKernelGGL((k_zconv_r<T, L, false>), dim3(div_up(d.Kx, S::NB), d.Ly),
+            hipLaunchKernelGGL((k_zconv_r<T, L, false>), zgrid,
                                dim3(kThreads), 0, st, (const cpx<T> *)A, B,
                                static_cast<const T *>(ws.Gf), static_cast<const cpx<T> *>(ws.Gi),
                                tw, d);
         launch_end(ctx, K_ZCONV);
         break;
+    }
     case 3:
         launch_begin(ctx, K_INV_Y);
         hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, 3), dim3(kThreads), 0,
