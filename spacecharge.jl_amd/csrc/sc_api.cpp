@@ -20,7 +20,7 @@ const char *const kKernelNames[K_COUNT] = {
     "green_x",      "green_y",     "green_z",     "rho_fwd_x",      "rho_fwd_y",
     "zconv",        "inv_y",       "inv_x",       "bin_count",      "bin_scan",
     "bin_scatter",  "deposit_tiles", "interleave",  "gather_tiles",   "green_f",
-    "bin_rowscan",
+    "bin_rowscan",  "tile_reduce",
 };
 
 namespace {

@@ -21,8 +21,10 @@
 //               of each tile, slots from an LDS cursor per tile
 //   deposit_tiles  one workgroup per chunk: Float64 LDS sub-grid, CIC with
 //               the reference's arithmetic, sub-grid stored to `partials`
-//   gather_tiles   one lane per node: sum over the chunks of the (up to 8)
-//               tiles whose sub-grid holds the node, in a fixed order
+//   tile_reduce    sums the chunk sub-grids of each multi-chunk tile into its
+//               first chunk's slot, one lane per sub-grid node
+//   gather_tiles   one lane per node: sum over the (up to 8) tiles whose
+//               sub-grid holds the node, in a fixed order
 //
 // A particle's eight contributions are computed exactly as in the reference
 // (same operation order, no FMA contraction); only the order of the sums
@@ -43,9 +45,11 @@ constexpr int kSub = kTile + 1;      // nodes per sub-grid edge
 constexpr int kSubVol = kSub * kSub * kSub;
 constexpr int kMaxTiles = 4096;      // LDS histogram size
 constexpr int kBinThreads = 256;
-constexpr int kBinPerThread = 16;    // particles per lane in the binning passes
+constexpr int kBinPerThread = 16;    // particles per lane and slice round (slice sizing)
 constexpr int kBinChunk = kBinThreads * kBinPerThread;
+constexpr int kBatch = 8;            // particles a lane loads before computing any
 constexpr int kChunk = 8192;         // particles per deposit workgroup
+static_assert(kChunk <= 8192, "the fixed-point deposit scale assumes at most 2^13 particles per chunk");
 constexpr int kBinMaxSlices = 2048;  // binning workgroups (columns of the count matrix)
 
 inline int div_up4(int a) { return (a + 3) / 4; }
@@ -74,8 +78,10 @@ __device__ __forceinline__ int cell_of(C norm, int n)
 }
 
 // Tile of a particle; the arithmetic matches the deposit's normalisation.
+// `ext` is set when a weight leaves [0, 1] (a coordinate outside the mesh
+// extent: the clamped cell extrapolates) or is NaN.
 template <typename PT, typename MT>
-__device__ __forceinline__ int tile_of(PT px, PT py, PT pz, const Geo &g)
+__device__ __forceinline__ int tile_of(PT px, PT py, PT pz, const Geo &g, bool &ext)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
@@ -83,7 +89,16 @@ __device__ __forceinline__ int tile_of(PT px, PT py, PT pz, const Geo &g)
     const C nyn = ((C)py - (C)(MT)g.lo[1]) / (C)(MT)g.dl[1];
     const C nzn = ((C)pz - (C)(MT)g.lo[2]) / (C)(MT)g.dl[2];
     const int ix = cell_of(nxn, g.nx), iy = cell_of(nyn, g.ny), iz = cell_of(nzn, g.nz);
+    ext = !(nxn >= (C)0 && nxn <= (C)(g.nx - 1) && nyn >= (C)0 && nyn <= (C)(g.ny - 1) &&
+            nzn >= (C)0 && nzn <= (C)(g.nz - 1));
     return ix / kTile + g.tx * (iy / kTile + g.ty * (iz / kTile));
+}
+
+template <typename PT, typename MT>
+__device__ __forceinline__ int tile_of(PT px, PT py, PT pz, const Geo &g)
+{
+    bool ext;
+    return tile_of<PT, MT>(px, py, pz, g, ext);
 }
 
 // The binning workgroups split the particles into G contiguous slices of
@@ -94,6 +109,8 @@ struct Split {
 };
 
 // Per-slice tile histogram, written as column w of the [tile][G] matrix.
+// Each lane loads kBatch particles before it computes any, so the loads of
+// a wave overlap.
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
@@ -105,11 +122,21 @@ k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__re
     __syncthreads();
     const int64_t first = (int64_t)blockIdx.x * sp.slice;
     const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
-    for (int64_t base = first; base < end; base += kBinChunk) {
-#pragma unroll 4
-        for (int s = 0; s < kBinPerThread; ++s) {
+    for (int64_t base = first; base < end; base += kBatch * kBinThreads) {
+        PT x[kBatch], y[kBatch], z[kBatch];
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
             const int64_t p = base + s * kBinThreads + threadIdx.x;
-            if (p < end) atomicAdd(&hist[tile_of<PT, MT>(px[p], py[p], pz[p], g)], 1u);
+            if (p < end) {
+                x[s] = px[p];
+                y[s] = py[p];
+                z[s] = pz[p];
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
+            if (base + s * kBinThreads + threadIdx.x < end)
+                atomicAdd(&hist[tile_of<PT, MT>(x[s], y[s], z[s], g)], 1u);
         }
     }
     __syncthreads();
@@ -153,11 +180,13 @@ k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restri
 }
 
 // One workgroup.  Per-tile totals -> tile start (exclusive offsets), each
-// tile's first chunk, and the chunk table.
+// tile's first chunk, the chunk table; clears the extrapolation flags.
+// (Folding this into k_bin_rowscan as "last workgroup done" costs more on
+// MI355X than the launch it saves: the device-scope fences write back L2.)
 __global__ void __launch_bounds__(256)
 k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
            unsigned *__restrict__ chunk_base, int4 *__restrict__ chunks,
-           unsigned *__restrict__ nchunks, int ntiles)
+           unsigned *__restrict__ nchunks, unsigned *__restrict__ ext_flags, int ntiles)
 {
     constexpr int kPer = kMaxTiles / 256;
     __shared__ unsigned s_w[2][4];
@@ -197,6 +226,7 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
     for (int i = 0; i < kPer; ++i) {
         const int t = t0 + i;
         if (i < per && t < ntiles) {
+            ext_flags[t] = 0u;
             cursor[t] = oa;
             chunk_base[t] = ob;
             const unsigned nk = (c[i] + kChunk - 1) / kChunk;
@@ -213,45 +243,90 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
 // Each slice's particles go to its reserved run inside each tile: the LDS
 // cursor of tile t starts at start[t] + (particles of t in earlier slices),
 // and an LDS atomic hands out the slots.  The same split and the same
-// tile_of as k_bin_count make every run exactly as long as counted.
+// tile_of as k_bin_count make every run exactly as long as counted.  Also
+// flags the tiles that hold a particle with weights outside [0, 1] and
+// takes max|q| (both steer the deposit's accumulator).
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
               const PT *__restrict__ pq, Split sp, Geo g, const unsigned *__restrict__ cmat,
-              const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted)
+              const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted,
+              unsigned *__restrict__ ext_flags, unsigned long long *__restrict__ slice_qmax)
 {
     __shared__ unsigned cur[kMaxTiles];
+    __shared__ unsigned char ext_lds[kMaxTiles];
     const int ntiles = g.tx * g.ty * g.tz;
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
+    for (int t = threadIdx.x; t < ntiles; t += kBinThreads) {
         cur[t] = start[t] + cmat[(size_t)t * sp.G + blockIdx.x];
+        ext_lds[t] = 0;
+    }
     __syncthreads();
+    double qm = 0.0;
     const int64_t first = (int64_t)blockIdx.x * sp.slice;
     const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
-    for (int64_t base = first; base < end; base += kBinChunk) {
-#pragma unroll 4
-        for (int s = 0; s < kBinPerThread; ++s) {
+    for (int64_t base = first; base < end; base += kBatch * kBinThreads) {
+        Rec<PT> r[kBatch];
+        unsigned slot[kBatch];
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
             const int64_t p = base + s * kBinThreads + threadIdx.x;
             if (p < end) {
-                Rec<PT> r;
-                r.x = px[p];
-                r.y = py[p];
-                r.z = pz[p];
-                r.q = pq[p];
-                const unsigned slot = atomicAdd(&cur[tile_of<PT, MT>(r.x, r.y, r.z, g)], 1u);
-                sorted[slot] = r;
+                r[s].x = px[p];
+                r[s].y = py[p];
+                r[s].z = pz[p];
+                r[s].q = pq[p];
             }
         }
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
+            if (base + s * kBinThreads + threadIdx.x < end) {
+                bool ext;
+                const int t = tile_of<PT, MT>(r[s].x, r[s].y, r[s].z, g, ext);
+                slot[s] = atomicAdd(&cur[t], 1u);
+                if (ext) ext_lds[t] = 1;  // same value from every writer
+                const double aq = fabs((double)r[s].q);
+                qm = (aq > qm || aq != aq) ? aq : qm;  // NaN sticks
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
+            if (base + s * kBinThreads + threadIdx.x < end) sorted[slot[s]] = r[s];
+        }
     }
+    // |q| >= 0, so its bit pattern orders like its value (NaN above +inf);
+    // one plain store per slice (the deposit reduces them)
+    __shared__ unsigned long long wmax[kBinThreads / 64];
+    unsigned long long bits = (unsigned long long)__double_as_longlong(qm);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        const unsigned long long o = __shfl_xor(bits, d, 64);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = wmax[0];
+        for (int k = 1; k < kBinThreads / 64; ++k) m = wmax[k] > m ? wmax[k] : m;
+        slice_qmax[blockIdx.x] = m;
+    }
+    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
+        if (ext_lds[t]) atomicOr(&ext_flags[t], 1u);  // rare: particles outside the mesh
 }
 
 // CIC contributions of one record, in the reference's arithmetic
-// (src/deposition.jl:38-84), added to the Float64 LDS sub-grid.  The LDS
-// accumulator is Float64 for every mesh type: gfx950 executes LDS atomic
-// adds on doubles several times faster than on floats, and each node's
-// chunk sum is then exact to double rounding before it meets the mesh type.
-template <typename PT, typename MT>
+// (src/deposition.jl:38-84), added to the chunk's LDS sub-grid.  Two
+// accumulators, chosen per chunk:
+//  * Float64 LDS atomics (FIXED = false): any mesh type, any weights;
+//  * 64-bit fixed point (FIXED = true, Float32 meshes): the contribution is
+//    rounded to a multiple of 1/S (S a power of two with
+//    kChunk * max|q| * S < 2^62, so no sum can overflow) and added with an
+//    integer LDS atomic, which gfx950 executes about twice as fast as the
+//    Float64 one.  The sum is exact on that grid; the rounding is at most
+//    2^-49 max|q| per contribution, far below Float32 resolution.  Only
+//    chunks whose weights all lie in [0, 1] take it.
+template <typename PT, typename MT, bool FIXED>
 __device__ __forceinline__ void deposit_one(const Rec<PT> &r, const Geo &g, int ox, int oy, int oz,
-                                            double *__restrict__ sub)
+                                            double *__restrict__ sub, double S)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
@@ -262,54 +337,135 @@ __device__ __forceinline__ void deposit_one(const Rec<PT> &r, const Geo &g, int 
     const C dx = nxn - (C)ix, dy = nyn - (C)iy, dz = nzn - (C)iz;
     const C wx0 = (C)1 - dx, wx1 = dx, wy0 = (C)1 - dy, wy1 = dy, wz0 = (C)1 - dz, wz1 = dz;
     const C c = (C)r.q;
-    double *b = sub + (ix - ox) + kSub * ((iy - oy) + kSub * (iz - oz));
+    const int o = (ix - ox) + kSub * ((iy - oy) + kSub * (iz - oz));
     constexpr int sy = kSub, sz = kSub * kSub;
-    atomicAdd(b, (double)(((c * wx0) * wy0) * wz0));
-    atomicAdd(b + 1, (double)(((c * wx1) * wy0) * wz0));
-    atomicAdd(b + sy, (double)(((c * wx0) * wy1) * wz0));
-    atomicAdd(b + sy + 1, (double)(((c * wx1) * wy1) * wz0));
-    atomicAdd(b + sz, (double)(((c * wx0) * wy0) * wz1));
-    atomicAdd(b + sz + 1, (double)(((c * wx1) * wy0) * wz1));
-    atomicAdd(b + sz + sy, (double)(((c * wx0) * wy1) * wz1));
-    atomicAdd(b + sz + sy + 1, (double)(((c * wx1) * wy1) * wz1));
+    const C v[8] = {((c * wx0) * wy0) * wz0, ((c * wx1) * wy0) * wz0,
+                    ((c * wx0) * wy1) * wz0, ((c * wx1) * wy1) * wz0,
+                    ((c * wx0) * wy0) * wz1, ((c * wx1) * wy0) * wz1,
+                    ((c * wx0) * wy1) * wz1, ((c * wx1) * wy1) * wz1};
+    constexpr int off[8] = {0, 1, sy, sy + 1, sz, sz + 1, sz + sy, sz + sy + 1};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        if constexpr (FIXED) {
+            unsigned long long *u = reinterpret_cast<unsigned long long *>(sub + o + off[k]);
+            atomicAdd(u, (unsigned long long)__double2ll_rn((double)v[k] * S));
+        } else {
+            atomicAdd(sub + o + off[k], (double)v[k]);
+        }
+    }
+}
+
+template <typename PT, typename MT, bool FIXED>
+__device__ __forceinline__ void deposit_chunk(const Rec<PT> *__restrict__ src, int len, const Geo &g,
+                                              int ox, int oy, int oz, double *__restrict__ sub,
+                                              double S)
+{
+    // records are fetched kBatch at a time so their loads overlap
+    constexpr int kBatch = 8;
+    for (int i0 = 0; i0 < len; i0 += kBatch * 256) {
+        Rec<PT> rb[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int i = i0 + u * 256 + (int)threadIdx.x;
+            if (i < len) rb[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (i0 + u * 256 + (int)threadIdx.x < len)
+                deposit_one<PT, MT, FIXED>(rb[u], g, ox, oy, oz, sub, S);
+        }
+    }
 }
 
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(256)
 k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chunks,
-                const unsigned *__restrict__ nchunks, Geo g, double *__restrict__ partials)
+                const unsigned *__restrict__ nchunks, Geo g, double *__restrict__ partials,
+                const unsigned *__restrict__ ext_flags,
+                const unsigned long long *__restrict__ slice_qmax, int nslices)
 {
     __shared__ double sub[kSubVol];
+    __shared__ unsigned long long wmax[4];
     if (blockIdx.x >= *nchunks) return;
     const int4 ch = chunks[blockIdx.x];
     const int t = ch.x;
     const int ox = (t % g.tx) * kTile, oy = ((t / g.tx) % g.ty) * kTile, oz = (t / (g.tx * g.ty)) * kTile;
-    for (int i = threadIdx.x; i < kSubVol; i += blockDim.x) sub[i] = 0.0;
-    __syncthreads();
-    // records are fetched kBatch at a time so their loads overlap
-    constexpr int kBatch = 8;
-    const Rec<PT> *src = sorted + ch.y;
-    for (int i0 = 0; i0 < ch.z; i0 += kBatch * 256) {
-        Rec<PT> rb[kBatch];
+    // fixed point for Float32 meshes when every weight of the tile is in
+    // [0, 1] and max|q| is finite; the branch is uniform over the workgroup
+    bool fixed = false;
+    double S = 1.0, invS = 1.0;
+    if constexpr (sizeof(MT) == 4) {
+        // max|q| over all particles: the maximum of the per-slice maxima
+        unsigned long long m = 0ull;
+        for (int i = threadIdx.x; i < nslices; i += blockDim.x) m = slice_qmax[i] > m ? slice_qmax[i] : m;
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            const int i = i0 + u * 256 + (int)threadIdx.x;
-            if (i < ch.z) rb[u] = src[i];
+        for (int d = 32; d > 0; d >>= 1) {
+            const unsigned long long o = __shfl_xor(m, d, 64);
+            m = o > m ? o : m;
         }
-#pragma unroll
-        for (int u = 0; u < kBatch; ++u) {
-            if (i0 + u * 256 + (int)threadIdx.x < ch.z) deposit_one<PT, MT>(rb[u], g, ox, oy, oz, sub);
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+        __syncthreads();
+        m = wmax[0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = wmax[k] > m ? wmax[k] : m;
+        const double qm = __longlong_as_double((long long)m);
+        // finite max|q| not so small that the scale 2^(48 - ilogb) overflows
+        if (ext_flags[t] == 0u && qm <= 1.7976931348623157e308 && (qm == 0.0 || qm >= 1e-270)) {
+            fixed = true;
+            if (qm > 0.0) {
+                const int e = 48 - ilogb(qm);  // kChunk (2^13) * qm * 2^e < 2^62
+                S = ldexp(1.0, e);
+                invS = ldexp(1.0, -e);
+            }
         }
     }
+    for (int i = threadIdx.x; i < kSubVol; i += blockDim.x) sub[i] = 0.0;  // also integer zero
+    __syncthreads();
+    const Rec<PT> *src = sorted + ch.y;
+    if (fixed) deposit_chunk<PT, MT, true>(src, ch.z, g, ox, oy, oz, sub, S);
+    else deposit_chunk<PT, MT, false>(src, ch.z, g, ox, oy, oz, sub, S);
     __syncthreads();
     double *out = partials + (size_t)blockIdx.x * kSubVol;
-    for (int i = threadIdx.x; i < kSubVol; i += blockDim.x) out[i] = sub[i];
+    if (fixed) {
+        const long long *si = reinterpret_cast<const long long *>(sub);
+        for (int i = threadIdx.x; i < kSubVol; i += blockDim.x) out[i] = (double)si[i] * invS;
+    } else {
+        for (int i = threadIdx.x; i < kSubVol; i += blockDim.x) out[i] = sub[i];
+    }
 }
 
-// Node-parallel sum of the chunk sub-grids: node g lies in at most two tiles
-// per axis (two when it is on a tile face), each tile contributes the sum
-// over its chunks.  Plain stores, fixed summation order; with accumulate
-// the sum is added to the existing rho (deposit!(...; clear=false)).
+// Sum of a multi-chunk tile's sub-grids, in place into its first chunk's
+// slot: grid (ceil(kSubVol / 256), tiles), one lane per sub-grid node.
+// Tiles with at most one chunk exit at once (their only sub-grid is
+// already the sum).
+__global__ void __launch_bounds__(256)
+k_tile_reduce(double *__restrict__ partials, const unsigned *__restrict__ counts,
+              const unsigned *__restrict__ chunk_base)
+{
+    const int t = blockIdx.y;
+    const unsigned nk = (counts[t] + kChunk - 1) / kChunk;
+    if (nk <= 1) return;
+    const int local = blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= kSubVol) return;
+    double *p = partials + (size_t)chunk_base[t] * kSubVol + local;
+    // eight independent partial sums keep eight loads in flight (the
+    // densest tiles hold a few dozen chunks)
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    unsigned k = 0;
+    for (; k + 8 <= nk; k += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(k + u) * kSubVol];
+    }
+#pragma unroll
+    for (int u = 0; u < 7; ++u)
+        if (k + u < nk) acc[u] += p[(size_t)(k + u) * kSubVol];
+    p[0] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
+// Node-parallel sum over the (up to 8) tiles whose sub-grid holds the node:
+// a node lies in two tiles along an axis when it is on a tile face.  Each
+// tile's sum is in its first chunk's slot (k_tile_reduce).  Plain stores,
+// fixed summation order; with accumulate the sum is added to the existing
+// rho (deposit!(...; clear=false)).
 template <typename MT>
 __global__ void __launch_bounds__(256)
 k_gather_tiles(const double *__restrict__ partials, const unsigned *__restrict__ counts,
@@ -334,22 +490,9 @@ k_gather_tiles(const double *__restrict__ partials, const unsigned *__restrict__
         for (int b = 0; b < nyc; ++b)
             for (int a = 0; a < nxc; ++a) {
                 const int t = cx[a] + g.tx * (cy[b] + g.ty * cz[c]);
-                const unsigned nk = (counts[t] + kChunk - 1) / kChunk;
-                if (nk == 0) continue;
+                if (counts[t] == 0u) continue;
                 const int local = (gx - kTile * cx[a]) + kSub * ((gy - kTile * cy[b]) + kSub * (gz - kTile * cz[c]));
-                const double *p = partials + (size_t)chunk_base[t] * kSubVol + local;
-                // eight independent partial sums keep eight loads in flight
-                // (the densest tiles hold a few dozen chunks)
-                double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-                unsigned k = 0;
-                for (; k + 8 <= nk; k += 8) {
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(k + u) * kSubVol];
-                }
-#pragma unroll
-                for (int u = 0; u < 7; ++u)
-                    if (k + u < nk) acc[u] += p[(size_t)(k + u) * kSubVol];
-                s += ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+                s += partials[(size_t)chunk_base[t] * kSubVol + local];
             }
     rho[i] = accumulate ? (MT)((double)rho[i] + s) : (MT)s;
 }
@@ -395,9 +538,10 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
         B.chunk_cap = maxchunks;
     }
     if (!B.counts) {
-        // totals | tile start | chunk_base | nchunks
-        if (hipMalloc((void **)&B.counts, sizeof(unsigned) * (3 * kMaxTiles + 1)) != hipSuccess)
-            return fail();
+        // totals | tile start | chunk_base | extrapolation flags | nchunks, pad |
+        // per-slice max|q| (u64 bit patterns)
+        const size_t bytes = sizeof(unsigned) * (4 * kMaxTiles + 4 + 2 * kBinMaxSlices);
+        if (hipMalloc((void **)&B.counts, bytes) != hipSuccess) return fail();
     }
     const int64_t need_cmat = (int64_t)ntiles * split_of(n).G;
     if (B.cmat_cap < need_cmat) {
@@ -420,7 +564,9 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, 
     DepositBins &B = m->bins;
     const int ntiles = g.tx * g.ty * g.tz;
     unsigned *counts = B.counts, *cursor = B.counts + kMaxTiles;
-    unsigned *cbase = B.counts + 2 * kMaxTiles, *nch = B.counts + 3 * kMaxTiles;
+    unsigned *cbase = B.counts + 2 * kMaxTiles, *ext = B.counts + 3 * kMaxTiles;
+    unsigned *nch = B.counts + 4 * kMaxTiles;
+    auto *slice_qmax = reinterpret_cast<unsigned long long *>(B.counts + 4 * kMaxTiles + 4);
     int4 *chunks = static_cast<int4 *>(B.chunks);
     const Split sp = split_of(n);
     launch_begin(ctx, K_BIN_COUNT);
@@ -433,21 +579,27 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, 
     launch_end(ctx, K_BIN_ROWSCAN);
     launch_begin(ctx, K_BIN_SCAN);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
-                       cbase, chunks, nch, ntiles);
+                       cbase, chunks, nch, ext, ntiles);
     launch_end(ctx, K_BIN_SCAN);
     launch_begin(ctx, K_BIN_SCATTER);
     hipLaunchKernelGGL((k_bin_scatter<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st,
                        (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q, sp, g,
                        (const unsigned *)B.cmat, (const unsigned *)cursor,
-                       static_cast<Rec<PT> *>(B.sorted));
+                       static_cast<Rec<PT> *>(B.sorted), ext, slice_qmax);
     launch_end(ctx, K_BIN_SCATTER);
     const unsigned maxchunks = (unsigned)(ntiles + n / kChunk + 1);
     launch_begin(ctx, K_DEPOSIT_TILES);
     hipLaunchKernelGGL((k_deposit_tiles<PT, MT>), dim3(maxchunks), dim3(256), 0, st,
                        (const Rec<PT> *)B.sorted, (const int4 *)chunks, (const unsigned *)nch, g,
-                       static_cast<double *>(B.partials));
+                       static_cast<double *>(B.partials), (const unsigned *)ext,
+                       (const unsigned long long *)slice_qmax, sp.G);
     launch_end(ctx, K_DEPOSIT_TILES);
     const int64_t M = (int64_t)g.nx * g.ny * g.nz;
+    launch_begin(ctx, K_TILE_REDUCE);
+    hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((kSubVol + 255) / 256), (unsigned)ntiles), dim3(256),
+                       0, st, static_cast<double *>(B.partials), (const unsigned *)counts,
+                       (const unsigned *)cbase);
+    launch_end(ctx, K_TILE_REDUCE);
     launch_begin(ctx, K_GATHER_TILES);
     hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
                        (const double *)B.partials, (const unsigned *)counts, (const unsigned *)cbase, g,

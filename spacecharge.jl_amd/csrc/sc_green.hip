@@ -120,6 +120,21 @@ __device__ __forceinline__ double diff8(const double *__restrict__ F, int i, int
     return s;
 }
 
+// The same difference on preloaded values f<y><z><x> (x, y, z in {0, 1}
+// relative to the cell's lower corner), same order of operations.
+__device__ __forceinline__ double diff8v(double f000, double f001, double f010, double f011,
+                                         double f100, double f101, double f110, double f111)
+{
+    double s = f111 - f110;
+    s = s - f011;
+    s = s - f101;
+    s = s - f000;
+    s = s + f010;
+    s = s + f100;
+    s = s + f001;
+    return s;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
 k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restrict__ twg,
@@ -324,19 +339,53 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     const bool hasb = ya + 1 < ny;
     const double *Fq = F + f_off(a, q);
     const T sgn = (c == 0) ? (T)-1 : (T)1;
+    // The extension's upper half (e > nx) mirrors cells nx - 1 .. 1, so each
+    // cell row is differenced once (e < nx, the first E/2 registers when
+    // L = 2 nx) and the mirror is read back through this pencil's LDS slice.
+    cpx<T> *row = lds + b * L;
     cpx<T> v[E];
+    // F rows ya, ya+1, ya+2 (clamped: without a second row the third is
+    // unused) of planes k, k+1, loaded without branches two cells at a time
+    // so that their 24 loads are in flight together
+    const int64_t sy = nx + 1, sz = (int64_t)(nx + 1) * (ny + 1);
+    const int yr[3] = {ya, ya + 1, ya + 2 <= ny ? ya + 2 : ny};
+    constexpr int kCells = 2;
+    static_assert((E / 2) % kCells == 0, "cells per batch");
 #pragma unroll
-    for (int r = 0; r < E; ++r) {
-        const int e = j + r * TPP;
-        v[r] = cpx<T>{(T)0, (T)0};
-        if (live && e != nx) {
-            const int i = e < nx ? e : L - e;  // symmetric extension
-            const T s = e < nx ? (T)1 : sgn;
-            v[r].x = s * (T)diff8(Fq, i, ya, k, nx + 1, ny + 1);
-            if (hasb) v[r].y = s * (T)diff8(Fq, i, ya + 1, k, nx + 1, ny + 1);
+    for (int r0 = 0; r0 < E / 2; r0 += kCells) {
+        double f[kCells][3][2][2];  // [cell][y][z][x]
+#pragma unroll
+        for (int u = 0; u < kCells; ++u) {
+            const int e = j + (r0 + u) * TPP;
+#pragma unroll
+            for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+                for (int zz = 0; zz < 2; ++zz) {
+                    const double *p = Fq + e + sy * yr[yy] + sz * (k + zz);
+                    f[u][yy][zz][0] = p[0];
+                    f[u][yy][zz][1] = p[1];
+                }
+        }
+#pragma unroll
+        for (int u = 0; u < kCells; ++u) {
+            const int r = r0 + u;
+            const double ga = diff8v(f[u][0][0][0], f[u][0][0][1], f[u][0][1][0], f[u][0][1][1],
+                                     f[u][1][0][0], f[u][1][0][1], f[u][1][1][0], f[u][1][1][1]);
+            const double gb = diff8v(f[u][1][0][0], f[u][1][0][1], f[u][1][1][0], f[u][1][1][1],
+                                     f[u][2][0][0], f[u][2][0][1], f[u][2][1][0], f[u][2][1][1]);
+            v[r].x = live ? (T)ga : (T)0;
+            v[r].y = (live && hasb) ? (T)gb : (T)0;
+            row[j + r * TPP] = v[r];
         }
     }
-    reg::fft<T, L, E, -1>(v, j, lds + b * L, 1, tw);
+    __syncthreads();
+#pragma unroll
+    for (int r = E / 2; r < E; ++r) {
+        const int e = j + r * TPP;
+        const cpx<T> m = row[(L - e) & (L - 1)];  // e == nx reads row[nx], unused
+        v[r] = e == nx ? cpx<T>{(T)0, (T)0} : cpx<T>{sgn * m.x, sgn * m.y};
+    }
+    reg::fft<T, L, E, -1>(v, j, row, 1, tw);
     if (live) {
         T *Rq = R + r_off(a, q);
 #pragma unroll
@@ -369,17 +418,25 @@ k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
     const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
     T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kp;
     const T sgn = (c == 1) ? (T)-1 : (T)1;
+    // every load is in range (clamped row and column) and issued without a
+    // branch, so a lane's loads are all in flight together; the selects
+    // below apply the extension and the bounds
+    const int kxa_c = kxa < Kx ? kxa : Kx - 1, kxb_c = kxb < Kx ? kxb : Kx - 1;
+    T ra[E], rb[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        const int e = j + r * TPP;
+        const int y = e < ny ? e : (e == ny ? 0 : L - e);
+        ra[r] = Rrow[(size_t)y * Kp + kxa_c];
+        rb[r] = Rrow[(size_t)y * Kp + kxb_c];
+    }
     cpx<T> v[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         const int e = j + r * TPP;
-        v[r] = cpx<T>{(T)0, (T)0};
-        if (e != ny) {
-            const int y = e < ny ? e : L - e;
-            const T s = e < ny ? (T)1 : sgn;
-            if (kxa < Kx) v[r].x = s * Rrow[(size_t)y * Kp + kxa];
-            if (kxb < Kx) v[r].y = s * Rrow[(size_t)y * Kp + kxb];
-        }
+        const T s = e < ny ? (T)1 : sgn;
+        v[r].x = (kxa < Kx && e != ny) ? s * ra[r] : (T)0;
+        v[r].y = (kxb < Kx && e != ny) ? s * rb[r] : (T)0;
     }
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
 #pragma unroll
@@ -411,12 +468,16 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
     const T *Rq = R + r_off(a, q);
     const T sgn = (c == 2) ? (T)-1 : (T)1;
-    cpx<T> v[E];
+    // loads clamped in range and issued without branches (see k_green_2r)
+    const int kxa_c = kxa < Kx ? kxa : Kx - 1, kxb_c = kxb < Kx ? kxb : Kx - 1;
+    T ra[E], rb[E], sc[E];
+    bool on[E];
 #pragma unroll
     for (int r = 0; r < E; ++r) {
         const int e = j + r * TPP;
-        int cell = -1;
+        int cell = 0;
         T s = (T)1;
+        on[r] = e != nz;  // slot e == nz is zero
         if (!image) {
             if (e != nz) {
                 cell = e < nz ? e : L - e;
@@ -426,12 +487,16 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
             // slot e holds displacement e (e < nz) or e - L (e > nz); row = displacement + nz - 1
             cell = e < nz ? e + nz - 1 : e - nz - 1;
         }
-        v[r] = cpx<T>{(T)0, (T)0};
-        if (cell >= 0) {
-            const T *row = Rq + ((size_t)cell * ny1 + ky) * Kp;
-            if (kxa < Kx) v[r].x = s * row[kxa];
-            if (kxb < Kx) v[r].y = s * row[kxb];
-        }
+        const T *row = Rq + ((size_t)cell * ny1 + ky) * Kp;
+        ra[r] = row[kxa_c];
+        rb[r] = row[kxb_c];
+        sc[r] = s;
+    }
+    cpx<T> v[E];
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+        v[r].x = (kxa < Kx && on[r]) ? sc[r] * ra[r] : (T)0;
+        v[r].y = (kxb < Kx && on[r]) ? sc[r] * rb[r] : (T)0;
     }
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
     const T scale = (T)a.scale[c];

@@ -39,6 +39,7 @@ enum KernelId : int {
     K_GATHER_TILES,
     K_GREEN_F,
     K_BIN_ROWSCAN,
+    K_TILE_REDUCE,
     K_COUNT
 };
 extern const char *const kKernelNames[K_COUNT];

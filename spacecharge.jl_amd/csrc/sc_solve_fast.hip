@@ -19,12 +19,22 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <typename T, int L>
+template <typename T, int L, int THREADS = kThreads>
 struct Shape {
     static constexpr int E = reg::elems<T>() < L ? reg::elems<T>() : L;
     static constexpr int TPP = L / E;         // lanes per pencil
-    static constexpr int NB = kThreads / TPP;  // pencils per workgroup
+    static constexpr int NB = THREADS / TPP;  // pencils per workgroup
 };
+
+// Workgroup size of the z convolution.  (512 threads, which make each
+// workgroup's Green row segment a whole 128-byte line, cut its fetched bytes
+// by a quarter on MI355X but ran slower: the kernel is bound by its
+// transform arithmetic, not by memory.)
+template <typename T, int L>
+constexpr int zconv_threads()
+{
+    return kThreads;
+}
 
 // rho x pass: real rows (y, z), zero-padded to 2nx, kx in [0, nx] to A.
 template <typename T, int L>
@@ -88,17 +98,17 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
 }
 
 // Fused z convolution (see k_zconv in sc_solve.hip for the algebra).
-template <typename T, int L, bool CATH>
-__global__ void __launch_bounds__(kThreads)
+template <typename T, int L, bool CATH, int THREADS>
+__global__ void __launch_bounds__(THREADS)
 k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
           const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d)
 {
-    using S = Shape<T, L>;
+    using S = Shape<T, L, THREADS>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
     const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
-    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kx = blockIdx.x * NB + b, ky = blockIdx.y;
     const bool live = kx < d.Kx;
@@ -196,16 +206,19 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
     const int rr = live ? r : 0;
     const int y = rr % d.ny, z = rr / d.ny;
     const cpx<T> *row = B + (size_t)c * d.nz * d.Ly * d.Kp + ((size_t)z * d.Ly + y) * d.Kp;
+    // one unconditional load per element (the Hermitian half reads the
+    // mirror), so all of a lane's loads are in flight together
     cpx<T> v[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         const int k = j + q * TPP;
-        if (q < E / 2 || k == nx) {
-            v[q] = row[k];
-        } else {
-            const cpx<T> t = row[L - k];
-            v[q] = cpx<T>{t.x, -t.y};
-        }
+        const bool direct = q < E / 2 || k == nx;
+        v[q] = row[direct ? k : L - k];
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        const int k = j + q * TPP;
+        if (!(q < E / 2 || k == nx)) v[q].y = -v[q].y;
     }
     reg::fft<T, L, E, +1>(v, j, lds + b * L, 1, tw);
     if (live) {
@@ -241,18 +254,18 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, int cath)
         launch_end(ctx, K_RHO_FWD_Y);
         break;
     case 2: {
+        constexpr int ZT = zconv_threads<T, L>();
+        using SZ = Shape<T, L, ZT>;
         launch_begin(ctx, K_ZCONV);
-        const dim3 zgrid(div_up(d.Kx, S::NB), d.Ly);
+        const dim3 zgrid(div_up(d.Kx, SZ::NB), d.Ly);
         if (cath)
-            hipLaunchKernelGGL((k_zconv_r<T, L, true>), zgrid,
-                               dim3(kThreads), 0, st, (const cpx<T> *)A, B,
-                               static_cast<const T *>(ws.Gf), static_cast<const cpx<T> *>(ws.Gi),
-                               tw, d);
+            hipLaunchKernelGGL((k_zconv_r<T, L, true, ZT>), zgrid, dim3(ZT), 0, st,
+                               (const cpx<T> *)A, B, static_cast<const T *>(ws.Gf),
+                               static_cast<const cpx<T> *>(ws.Gi), tw, d);
         else
-            hipLaunchKernelGGL((k_zconv_r<T, L, false>), zgrid,
-                               dim3(kThreads), 0, st, (const cpx<T> *)A, B,
-                               static_cast<const T *>(ws.Gf), static_cast<const cpx<T> *>(ws.Gi),
-                               tw, d);
+            hipLaunchKernelGGL((k_zconv_r<T, L, false, ZT>), zgrid, dim3(ZT), 0, st,
+                               (const cpx<T> *)A, B, static_cast<const T *>(ws.Gf),
+                               static_cast<const cpx<T> *>(ws.Gi), tw, d);
         launch_end(ctx, K_ZCONV);
         break;
     }

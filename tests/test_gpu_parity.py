@@ -188,6 +188,31 @@ def test_deposit_clamped_and_fallback(grid):
     assert abs(got.sum() - q.sum()) <= 1e-12 * np.abs(got).sum()
 
 
+def test_deposit_float32_fixed_and_float64_tiles():
+    """Float32 mesh with manual bounds: tiles whose particles all have CIC
+    weights in [0, 1] accumulate in 64-bit fixed point, tiles holding
+    particles outside the mesh (extrapolating weights) in Float64.  Both
+    agree with the Float64 oracle on the same Float32 inputs."""
+    sc = _sc()
+    rng = np.random.default_rng(13)
+    n = 200_000
+    x, y, z = (rng.normal(0.0, 0.35e-3, n).astype(np.float32) for _ in range(3))
+    q = (rng.uniform(0.5, 1.5, n) * 1e-15).astype(np.float32)
+    outside = np.abs(np.stack([x, y, z])).max(axis=0) > 1e-3
+    assert 0 < outside.sum() < n // 10
+    xt, yt, zt, qt = _tensors((x, y, z, q), torch.float32)
+    grid = (64, 64, 64)
+    mesh = sc.Mesh3D(grid, (-1e-3, -1e-3, -1e-3), (1e-3, 1e-3, 1e-3), T=torch.float32)
+    sc.deposit(mesh, xt, yt, zt, qt)
+    torch.cuda.synchronize()
+    got = mesh_to_np(mesh._rho, grid).astype(np.float64)
+    xs, ys, zs, qs = (a.astype(np.float64) for a in (x, y, z, q))
+    ref = orc.deposit(xs, ys, zs, qs, grid, mesh.min_bounds, mesh.delta)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"float32 fixed/Float64 tiles: max rel err vs Float64 {err:.3e}")
+    assert err <= 1e-5
+
+
 def test_deposit_dense_core():
     """300k particles inside a few cells: heavy contention on a handful of nodes.
     Float32 is compared with the Float64 oracle: a serial Float32 sum of 600k
