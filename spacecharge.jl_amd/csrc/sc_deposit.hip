@@ -21,8 +21,9 @@
 //               of each tile, slots from an LDS cursor per tile
 //   deposit_tiles  one workgroup per chunk: Float64 LDS sub-grid, CIC with
 //               the reference's arithmetic, sub-grid stored to `partials`
-//   tile_reduce    sums the chunk sub-grids of each multi-chunk tile into its
-//               first chunk's slot, one lane per sub-grid node
+//   tile_reduce    adds the further chunks of each multi-chunk tile to its
+//               first chunk's sub-grid (slot = tile index), zeroes empty
+//               tiles' slots; one lane per sub-grid node
 //   gather_tiles   one lane per node: sum over the (up to 8) tiles whose
 //               sub-grid holds the node, in a fixed order
 //
@@ -49,6 +50,7 @@ constexpr int kBinPerThread = 16;    // particles per lane and slice round (slic
 constexpr int kBinChunk = kBinThreads * kBinPerThread;
 constexpr int kBatch = 8;            // particles a lane loads before computing any
 constexpr int kChunk = 8192;         // particles per deposit workgroup
+constexpr int kDepThreads = 512;     // deposit workgroup size
 static_assert(kChunk <= 8192, "the fixed-point deposit scale assumes at most 2^13 particles per chunk");
 constexpr int kBinMaxSlices = 2048;  // binning workgroups (columns of the count matrix)
 
@@ -126,17 +128,16 @@ k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__re
         PT x[kBatch], y[kBatch], z[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            const int64_t p = base + s * kBinThreads + threadIdx.x;
-            if (p < end) {
-                x[s] = px[p];
-                y[s] = py[p];
-                z[s] = pz[p];
-            }
+            int64_t p = base + s * kBinThreads + threadIdx.x;
+            p = p < end ? p : end - 1;  // unconditional loads stay in flight together
+            x[s] = px[p];
+            y[s] = py[p];
+            z[s] = pz[p];
         }
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            if (base + s * kBinThreads + threadIdx.x < end)
-                atomicAdd(&hist[tile_of<PT, MT>(x[s], y[s], z[s], g)], 1u);
+            const bool ok = base + s * kBinThreads + threadIdx.x < end;
+            atomicAdd(&hist[tile_of<PT, MT>(x[s], y[s], z[s], g)], ok ? 1u : 0u);
         }
     }
     __syncthreads();
@@ -185,41 +186,46 @@ k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restri
 // MI355X than the launch it saves: the device-scope fences write back L2.)
 __global__ void __launch_bounds__(256)
 k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
-           unsigned *__restrict__ chunk_base, int4 *__restrict__ chunks,
+           unsigned *__restrict__ extra_base, int4 *__restrict__ chunks,
            unsigned *__restrict__ nchunks, unsigned *__restrict__ ext_flags, int ntiles)
 {
     constexpr int kPer = kMaxTiles / 256;
-    __shared__ unsigned s_w[2][4];
+    __shared__ unsigned s_w[3][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     // thread i owns the `per` consecutive tiles [i per, i per + per)
     const int per = (ntiles + 255) / 256;
     const int t0 = threadIdx.x * per;
     unsigned c[kPer];
-    unsigned a = 0u, b = 0u;
+    unsigned a = 0u, b = 0u, x = 0u;  // particles, chunks, chunks after each tile's first
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
         c[i] = (i < per && t0 + i < ntiles) ? counts[t0 + i] : 0u;
+        const unsigned nk = (c[i] + kChunk - 1) / kChunk;
         a += c[i];
-        b += (c[i] + kChunk - 1) / kChunk;
+        b += nk;
+        x += nk ? nk - 1 : 0u;
     }
-    unsigned ia = a, ib = b;
+    unsigned ia = a, ib = b, ix = x;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const unsigned oa = __shfl_up(ia, d, 64), ob = __shfl_up(ib, d, 64);
+        const unsigned oa = __shfl_up(ia, d, 64), ob = __shfl_up(ib, d, 64), ox = __shfl_up(ix, d, 64);
         if (lane >= d) {
             ia += oa;
             ib += ob;
+            ix += ox;
         }
     }
     if (lane == 63) {
         s_w[0][wv] = ia;
         s_w[1][wv] = ib;
+        s_w[2][wv] = ix;
     }
     __syncthreads();
-    unsigned oa = ia - a, ob = ib - b;
+    unsigned oa = ia - a, ob = ib - b, ox = ix - x;
     for (int k = 0; k < wv; ++k) {
         oa += s_w[0][k];
         ob += s_w[1][k];
+        ox += s_w[2][k];
     }
     if (threadIdx.x == 255) *nchunks = ob + b;
 #pragma unroll
@@ -228,14 +234,18 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
         if (i < per && t < ntiles) {
             ext_flags[t] = 0u;
             cursor[t] = oa;
-            chunk_base[t] = ob;
+            // sub-grid slots: the tile's first chunk uses slot t (which then
+            // holds the tile's sum), its further chunks ntiles + ox, ...
+            extra_base[t] = (unsigned)ntiles + ox;
             const unsigned nk = (c[i] + kChunk - 1) / kChunk;
             for (unsigned k = 0; k < nk; ++k) {
                 const unsigned len = (c[i] - k * kChunk) < (unsigned)kChunk ? (c[i] - k * kChunk) : kChunk;
-                chunks[ob + k] = make_int4(t, (int)(oa + k * kChunk), (int)len, 0);
+                const unsigned slot = k == 0 ? (unsigned)t : (unsigned)ntiles + ox + k - 1;
+                chunks[ob + k] = make_int4(t, (int)(oa + k * kChunk), (int)len, (int)slot);
             }
             oa += c[i];
             ob += nk;
+            ox += nk ? nk - 1 : 0u;
         }
     }
 }
@@ -269,28 +279,34 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
         unsigned slot[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            const int64_t p = base + s * kBinThreads + threadIdx.x;
-            if (p < end) {
-                r[s].x = px[p];
-                r[s].y = py[p];
-                r[s].z = pz[p];
-                r[s].q = pq[p];
-            }
+            int64_t p = base + s * kBinThreads + threadIdx.x;
+            p = p < end ? p : end - 1;  // unconditional loads stay in flight together
+            r[s].x = px[p];
+            r[s].y = py[p];
+            r[s].z = pz[p];
+            r[s].q = pq[p];
         }
+        // tiles first, then the slot atomics back to back, then the flags
+        // and stores (an LDS store between the atomics would order them)
+        int tl[kBatch];
+        bool ex[kBatch], ok[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            if (base + s * kBinThreads + threadIdx.x < end) {
-                bool ext;
-                const int t = tile_of<PT, MT>(r[s].x, r[s].y, r[s].z, g, ext);
-                slot[s] = atomicAdd(&cur[t], 1u);
-                if (ext) ext_lds[t] = 1;  // same value from every writer
+            ok[s] = base + s * kBinThreads + threadIdx.x < end;
+            tl[s] = tile_of<PT, MT>(r[s].x, r[s].y, r[s].z, g, ex[s]);
+        }
+        // no branch around the atomics (a lane past the slice end adds 0 to
+        // the valid tile of its clamped particle)
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) slot[s] = atomicAdd(&cur[tl[s]], ok[s] ? 1u : 0u);
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
+            if (ok[s]) {
+                if (ex[s]) ext_lds[tl[s]] = 1;  // same value from every writer
                 const double aq = fabs((double)r[s].q);
                 qm = (aq > qm || aq != aq) ? aq : qm;  // NaN sticks
+                sorted[slot[s]] = r[s];
             }
-        }
-#pragma unroll
-        for (int s = 0; s < kBatch; ++s) {
-            if (base + s * kBinThreads + threadIdx.x < end) sorted[slot[s]] = r[s];
         }
     }
     // |q| >= 0, so its bit pattern orders like its value (NaN above +inf);
@@ -362,30 +378,32 @@ __device__ __forceinline__ void deposit_chunk(const Rec<PT> *__restrict__ src, i
 {
     // records are fetched kBatch at a time so their loads overlap
     constexpr int kBatch = 8;
-    for (int i0 = 0; i0 < len; i0 += kBatch * 256) {
+    for (int i0 = 0; i0 < len; i0 += kBatch * kDepThreads) {
+        // unconditional loads (index clamped into the chunk) so that all
+        // kBatch are in flight together; len >= 1 for every chunk
         Rec<PT> rb[kBatch];
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            const int i = i0 + u * 256 + (int)threadIdx.x;
-            if (i < len) rb[u] = src[i];
+            const int i = i0 + u * kDepThreads + (int)threadIdx.x;
+            rb[u] = src[i < len ? i : len - 1];
         }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
-            if (i0 + u * 256 + (int)threadIdx.x < len)
+            if (i0 + u * kDepThreads + (int)threadIdx.x < len)
                 deposit_one<PT, MT, FIXED>(rb[u], g, ox, oy, oz, sub, S);
         }
     }
 }
 
 template <typename PT, typename MT>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kDepThreads)
 k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chunks,
                 const unsigned *__restrict__ nchunks, Geo g, double *__restrict__ partials,
                 const unsigned *__restrict__ ext_flags,
                 const unsigned long long *__restrict__ slice_qmax, int nslices)
 {
     __shared__ double sub[kSubVol];
-    __shared__ unsigned long long wmax[4];
+    __shared__ unsigned long long wmax[kDepThreads / 64];
     if (blockIdx.x >= *nchunks) return;
     const int4 ch = chunks[blockIdx.x];
     const int t = ch.x;
@@ -424,7 +442,7 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chu
     if (fixed) deposit_chunk<PT, MT, true>(src, ch.z, g, ox, oy, oz, sub, S);
     else deposit_chunk<PT, MT, false>(src, ch.z, g, ox, oy, oz, sub, S);
     __syncthreads();
-    double *out = partials + (size_t)blockIdx.x * kSubVol;
+    double *out = partials + (size_t)ch.w * kSubVol;
     if (fixed) {
         const long long *si = reinterpret_cast<const long long *>(sub);
         for (int i = threadIdx.x; i < kSubVol; i += blockDim.x) out[i] = (double)si[i] * invS;
@@ -433,43 +451,47 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chu
     }
 }
 
-// Sum of a multi-chunk tile's sub-grids, in place into its first chunk's
-// slot: grid (ceil(kSubVol / 256), tiles), one lane per sub-grid node.
-// Tiles with at most one chunk exit at once (their only sub-grid is
-// already the sum).
+// Completes the per-tile sums in sub-grid slots [0, ntiles): adds a tile's
+// further chunks (slots extra_base[t], ...) to slot t, or zeroes slot t of
+// an empty tile.  Grid (ceil(kSubVol / 256), tiles), one lane per node.
 __global__ void __launch_bounds__(256)
 k_tile_reduce(double *__restrict__ partials, const unsigned *__restrict__ counts,
-              const unsigned *__restrict__ chunk_base)
+              const unsigned *__restrict__ extra_base)
 {
     const int t = blockIdx.y;
     const unsigned nk = (counts[t] + kChunk - 1) / kChunk;
-    if (nk <= 1) return;
+    if (nk == 1) return;  // slot t already holds the sum
     const int local = blockIdx.x * blockDim.x + threadIdx.x;
     if (local >= kSubVol) return;
-    double *p = partials + (size_t)chunk_base[t] * kSubVol + local;
+    double *dst = partials + (size_t)t * kSubVol + local;
+    if (nk == 0) {
+        *dst = 0.0;
+        return;
+    }
+    const double *p = partials + (size_t)extra_base[t] * kSubVol + local;
     // eight independent partial sums keep eight loads in flight (the
     // densest tiles hold a few dozen chunks)
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[8] = {*dst, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const unsigned nx = nk - 1;  // further chunks
     unsigned k = 0;
-    for (; k + 8 <= nk; k += 8) {
+    for (; k + 8 <= nx; k += 8) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) acc[u] += p[(size_t)(k + u) * kSubVol];
     }
 #pragma unroll
     for (int u = 0; u < 7; ++u)
-        if (k + u < nk) acc[u] += p[(size_t)(k + u) * kSubVol];
-    p[0] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        if (k + u < nx) acc[u] += p[(size_t)(k + u) * kSubVol];
+    *dst = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 // Node-parallel sum over the (up to 8) tiles whose sub-grid holds the node:
-// a node lies in two tiles along an axis when it is on a tile face.  Each
-// tile's sum is in its first chunk's slot (k_tile_reduce).  Plain stores,
-// fixed summation order; with accumulate the sum is added to the existing
-// rho (deposit!(...; clear=false)).
+// a node lies in two tiles along an axis when it is on a tile face.  Tile
+// t's sum is in sub-grid slot t (k_tile_reduce), so every address follows
+// from the node index alone.  Plain stores, fixed summation order; with
+// accumulate the sum is added to the existing rho (deposit!(...; clear=false)).
 template <typename MT>
 __global__ void __launch_bounds__(256)
-k_gather_tiles(const double *__restrict__ partials, const unsigned *__restrict__ counts,
-               const unsigned *__restrict__ chunk_base, Geo g, MT *__restrict__ rho, int accumulate)
+k_gather_tiles(const double *__restrict__ partials, Geo g, MT *__restrict__ rho, int accumulate)
 {
     // at most kMaxTiles tiles of 16^3 cells: node indices fit in 32 bits
     const int M = g.nx * g.ny * g.nz;
@@ -490,9 +512,8 @@ k_gather_tiles(const double *__restrict__ partials, const unsigned *__restrict__
         for (int b = 0; b < nyc; ++b)
             for (int a = 0; a < nxc; ++a) {
                 const int t = cx[a] + g.tx * (cy[b] + g.ty * cz[c]);
-                if (counts[t] == 0u) continue;
                 const int local = (gx - kTile * cx[a]) + kSub * ((gy - kTile * cy[b]) + kSub * (gz - kTile * cz[c]));
-                s += partials[(size_t)chunk_base[t] * kSubVol + local];
+                s += partials[(size_t)t * kSubVol + local];
             }
     rho[i] = accumulate ? (MT)((double)rho[i] + s) : (MT)s;
 }
@@ -538,7 +559,7 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
         B.chunk_cap = maxchunks;
     }
     if (!B.counts) {
-        // totals | tile start | chunk_base | extrapolation flags | nchunks, pad |
+        // totals | tile start | first extra sub-grid slot | extrapolation flags | nchunks, pad |
         // per-slice max|q| (u64 bit patterns)
         const size_t bytes = sizeof(unsigned) * (4 * kMaxTiles + 4 + 2 * kBinMaxSlices);
         if (hipMalloc((void **)&B.counts, bytes) != hipSuccess) return fail();
@@ -589,7 +610,7 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, 
     launch_end(ctx, K_BIN_SCATTER);
     const unsigned maxchunks = (unsigned)(ntiles + n / kChunk + 1);
     launch_begin(ctx, K_DEPOSIT_TILES);
-    hipLaunchKernelGGL((k_deposit_tiles<PT, MT>), dim3(maxchunks), dim3(256), 0, st,
+    hipLaunchKernelGGL((k_deposit_tiles<PT, MT>), dim3(maxchunks), dim3(kDepThreads), 0, st,
                        (const Rec<PT> *)B.sorted, (const int4 *)chunks, (const unsigned *)nch, g,
                        static_cast<double *>(B.partials), (const unsigned *)ext,
                        (const unsigned long long *)slice_qmax, sp.G);
@@ -602,8 +623,7 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, 
     launch_end(ctx, K_TILE_REDUCE);
     launch_begin(ctx, K_GATHER_TILES);
     hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
-                       (const double *)B.partials, (const unsigned *)counts, (const unsigned *)cbase, g,
-                       static_cast<MT *>(m->rho), clear ? 0 : 1);
+                       (const double *)B.partials, g, static_cast<MT *>(m->rho), clear ? 0 : 1);
     launch_end(ctx, K_GATHER_TILES);
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
