@@ -146,6 +146,10 @@ int sc_ctx_destroy(sc_ctx *ctx)
     }
     for (auto e : c->pool) (void)hipEventDestroy(e);
     if (c->red) (void)hipFree(c->red);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
     delete ctx;
     return SC_OK;
 }

@@ -89,6 +89,10 @@ struct Ctx {
     int64_t launches[K_COUNT] = {};
     // bounds-reduction partials, allocated on first use
     double *red = nullptr;
+    // second stream (default priority, non-blocking), created on first use:
+    // the Green-function build of a solve runs on it, concurrently with the
+    // deposit still executing on `stream`
+    hipStream_t side = nullptr;
 };
 
 // Device workspace of one mesh (lazy, like src/mesh.jl:49-72).
@@ -112,6 +116,16 @@ struct Workspace {
     // twiddle tables in T for the lengths 2nx, 2ny, 2nz
     void *twT[3] = {nullptr, nullptr, nullptr};
     FftPlan plan[3];
+    // ordering of the Green build on the side stream: green_done after the
+    // build (the z convolution waits for it), gf_free after the z
+    // convolution (the next build waits for it)
+    hipEvent_t green_done = nullptr, gf_free = nullptr;
+    bool gf_free_recorded = false;
+    // recorded where each deposit into this mesh starts: the next build
+    // waits for it too, so that it overlaps the deposit kernels (latency
+    // bound) rather than the previous step's interpolation
+    hipEvent_t deposit_start = nullptr;
+    bool deposit_start_recorded = false;
     // cached Green key for the opt-in reuse
     bool g_valid = false;
     double g_key[6] = {};
@@ -174,6 +188,7 @@ int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z
 int solve_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
                int at_cathode);
 int ensure_workspace(Mesh *m);
+int ensure_side_stream(Mesh *m);
 void free_workspace(Mesh *m);
 
 }  // namespace sc

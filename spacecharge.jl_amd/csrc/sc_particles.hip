@@ -261,6 +261,12 @@ int launch_clear(Ctx *ctx, void *p, int64_t bytes)
 int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
                    int64_t n, const double lo[3], const double dl[3], int clear)
 {
+    // marks where this deposit starts on the stream; a following solve's
+    // Green build (side stream) waits for it, see run_solve
+    if (m->ws.deposit_start) {
+        SC_HIP(m->ctx, hipEventRecord(m->ws.deposit_start, m->ctx->stream));
+        m->ws.deposit_start_recorded = true;
+    }
     const int64_t rho_bytes = m->n[0] * m->n[1] * m->n[2] * m->dtype;
     if (n <= 0) return clear ? launch_clear(m->ctx, m->rho, rho_bytes) : SC_OK;
     const int rc = launch_deposit_binned(m, pdt, x, y, z, q, n, lo, dl, clear);

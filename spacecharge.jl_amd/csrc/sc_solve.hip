@@ -271,8 +271,27 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     const double key[6] = {dl[0], dl[1], dl[2], gamma, cath ? offz : 0.0, 0.0};
     const bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == cath &&
                        std::memcmp(key, ws.g_key, sizeof(key)) == 0;
+    // The Green spectra depend on the spacing, gamma and the cathode offset
+    // only, not on rho: built on the context's side stream they overlap the
+    // deposit kernels still running on `st`.  Under stream capture the
+    // build stays on `st`.
+    bool overlap = false;
     if (!reuse) {
-        if (int rc = build_green<T>(m, dl, gamma, offz, cath)) return rc;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        (void)hipStreamIsCapturing(st, &cs);
+        overlap = cs == hipStreamCaptureStatusNone && ensure_side_stream(m) == SC_OK;
+        if (overlap) {
+            if (ws.gf_free_recorded) SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.gf_free, 0));
+            if (ws.deposit_start_recorded)
+                SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.deposit_start, 0));
+            ctx->stream = ctx->side;  // the build launches (and times) on ctx->stream
+            const int rc = build_green<T>(m, dl, gamma, offz, cath);
+            ctx->stream = st;
+            if (rc) return rc;
+            SC_HIP(ctx, hipEventRecord(ws.green_done, ctx->side));
+        } else {
+            if (int rc = build_green<T>(m, dl, gamma, offz, cath)) return rc;
+        }
         ws.g_valid = true;
         ws.g_cath = cath;
         std::memcpy(ws.g_key, key, sizeof(key));
@@ -303,6 +322,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         launch_end(ctx, K_RHO_FWD_Y);
     }
     // ---- fused z convolution ----
+    if (overlap) SC_HIP(ctx, hipStreamWaitEvent(st, ws.green_done, 0));
     if (!fast(2)) {
         const int nbuf = cath ? 3 : 2;
         int nb = pick_nb(d.Lz, fft_cap<T>(), 16);
@@ -313,6 +333,10 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
                            (const cpx<T> *)A, B, static_cast<const T *>(ws.Gf),
                            static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, cath);
         launch_end(ctx, K_ZCONV);
+    }
+    if (ws.green_done) {  // the side stream exists: the next build waits for this point
+        SC_HIP(ctx, hipEventRecord(ws.gf_free, st));
+        ws.gf_free_recorded = true;
     }
     // ---- backward y, x ----
     if (!fast(3)) {
@@ -355,9 +379,38 @@ bool make_plan(int n, FftPlan *p)
     return true;
 }
 
+int ensure_side_stream(Mesh *m)
+{
+    Ctx *ctx = m->ctx;
+    Workspace &ws = m->ws;
+    if (!ctx->side && hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) {
+        ctx->side = nullptr;
+        return SC_ERR_DEVICE;
+    }
+    if (!ws.green_done && hipEventCreateWithFlags(&ws.green_done, hipEventDisableTiming) != hipSuccess) {
+        ws.green_done = nullptr;
+        return SC_ERR_DEVICE;
+    }
+    if (!ws.gf_free && hipEventCreateWithFlags(&ws.gf_free, hipEventDisableTiming) != hipSuccess) {
+        ws.gf_free = nullptr;
+        return SC_ERR_DEVICE;
+    }
+    if (!ws.deposit_start &&
+        hipEventCreateWithFlags(&ws.deposit_start, hipEventDisableTiming) != hipSuccess) {
+        ws.deposit_start = nullptr;
+        return SC_ERR_DEVICE;
+    }
+    return SC_OK;
+}
+
 void free_workspace(Mesh *m)
 {
     Workspace &ws = m->ws;
+    // a Green build may still be running on the side stream
+    if (m->ctx && m->ctx->side) (void)hipStreamSynchronize(m->ctx->side);
+    if (ws.green_done) (void)hipEventDestroy(ws.green_done);
+    if (ws.gf_free) (void)hipEventDestroy(ws.gf_free);
+    if (ws.deposit_start) (void)hipEventDestroy(ws.deposit_start);
     void *ptrs[] = {ws.A, ws.B, ws.Gf, ws.Gi, ws.F, ws.R, ws.twT[0], ws.twT[1], ws.twT[2]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
