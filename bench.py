@@ -67,10 +67,16 @@ def kernel_bytes(name, n, grid, s, ps):
         return 3 * n * ps
     if name == "bin_scatter":
         return 8 * n * ps
+    tiles = 1
+    for g_ in grid:
+        tiles *= (g_ - 1 + 15) // 16
+    if name == "bin_rowscan":
+        slices = min((n + 4095) // 4096, 2048)
+        return 2 * 4 * tiles * slices
     if name == "deposit_tiles":
         return 4 * n * ps
     if name == "gather_tiles":
-        return M * s
+        return tiles * 17 ** 3 * 8 + M * s
     if name == "interleave":
         return 7 * M * s
     if name == "interpolate":

@@ -98,8 +98,11 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
 }
 
 // Fused z convolution (see k_zconv in sc_solve.hip for the algebra).
+// Three waves per SIMD: the forward pencil stays in registers across the
+// three inverse transforms, and unconstrained the kernel takes 169 VGPRs
+// (two waves); 168 fit without spilling.
 template <typename T, int L, bool CATH, int THREADS>
-__global__ void __launch_bounds__(THREADS)
+__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
           const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d)
 {
