@@ -9,6 +9,11 @@ timing, as in the reference benchmark (benchmark/full_pipeline_benchmark.jl:
 25-30); each timed step is deposit! + solve! + interpolate_field with inputs
 resident in HBM.
 
+--workload cathode runs the cathode-emission configuration instead: 2,000,000
+particles, uniform disk of radius 1 mm with z ~ Exp(mean 0.2 mm), 100 pC,
+64x64x128 mesh from the particle extrema, image charge on (solve!(mesh;
+at_cathode=true)).
+
 Prints one JSON line.  `value` is particles/s of the whole job (one GPU).
 The `roofline` object is for the kernel with the largest share of the step
 time, measured with the library's HIP-event timers on the launching stream
@@ -52,6 +57,22 @@ def make_particles(n, dtype, device, seed=42):
     p = torch.cat([gauss, ell]).to(dtype)
     x, y, z = (p[:, i].contiguous() for i in range(3))
     q = torch.full((n,), 1e-9 / n, dtype=dtype, device=device)
+    return x, y, z, q
+
+
+def make_cathode_particles(n, dtype, device, seed=7):
+    """Cathode emission bunch: r = 1 mm sqrt(U) (uniform disk), z ~ Exp(mean 0.2 mm),
+    100 pC total in equal macro-charges, generated on the GPU."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    u = torch.rand((n, 3), generator=g, dtype=torch.float64, device=device)
+    r = 1e-3 * torch.sqrt(u[:, 0])
+    th = 2 * torch.pi * u[:, 1]
+    x = (r * torch.cos(th)).to(dtype).contiguous()
+    y = (r * torch.sin(th)).to(dtype).contiguous()
+    z = (-0.2e-3 * torch.log1p(-u[:, 2])).to(dtype).contiguous()
+    q = torch.full((n,), 100e-12 / n, dtype=dtype, device=device)
     return x, y, z, q
 
 
@@ -119,16 +140,15 @@ def measured_traffic(name, n, grid, dtype_name, cathode):
     return None, None
 
 
-def cpu_baseline(args, n, grid, dtype_name, max_seconds):
-    """Time the CPU oracle on a bounded sample: whole steps of the same workload."""
+def cpu_baseline(args, parts, grid, dtype_name, max_seconds):
+    """Time the CPU oracle on a bounded sample: whole steps of the same workload
+    (the same particles, copied to the host)."""
     import numpy as np
-    import torch
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
     npdt = np.float32 if dtype_name == "f32" else np.float64
-    x, y, z, q = make_particles(n, torch.float32 if dtype_name == "f32" else torch.float64,
-                                "cuda", seed=42)
-    x, y, z, q = (t.cpu().numpy() for t in (x, y, z, q))
+    x, y, z, q = (t.cpu().numpy() for t in parts)
+    n = x.shape[0]
     threads = os.cpu_count() or 1
     lo, hi, dl = orc.bounds_auto(x, y, z, grid, npdt)
     times = []
@@ -136,7 +156,7 @@ def cpu_baseline(args, n, grid, dtype_name, max_seconds):
     while True:
         t0 = time.perf_counter()
         rho = orc.deposit(x, y, z, q, grid, lo, dl, npdt)
-        ef = orc.solve(rho, lo, hi, dl, 1.0, False, nthreads=threads)
+        ef = orc.solve(rho, lo, hi, dl, args.gamma, bool(args.cathode), nthreads=threads)
         orc.interpolate(ef, x, y, z, lo, dl, nthreads=threads)
         times.append(time.perf_counter() - t0)
         if time.perf_counter() - t_start > max_seconds or len(times) >= 10:
@@ -147,7 +167,9 @@ def cpu_baseline(args, n, grid, dtype_name, max_seconds):
         "unit": "particles/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{len(times)} full steps ({dtype_name}, {n} particles, {grid[0]}^3), "
+        "sample": (f"{len(times)} full steps ({dtype_name}, {n} particles, "
+                   f"{grid[0]}x{grid[1]}x{grid[2]}, gamma={args.gamma}, "
+                   f"{'cathode' if args.cathode else 'free space'}), "
                    f"min {best * 1e3:.1f} ms, median {sorted(times)[len(times) // 2] * 1e3:.1f} ms; "
                    "deposit single-threaded, FFT/Green/interpolate on all cores"),
     }
@@ -163,6 +185,10 @@ def main():
     ap.add_argument("--particles", type=int, default=4_000_000)
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cathode", action="store_true")
+    ap.add_argument("--workload", choices=["bunch", "cathode"], default="bunch",
+                    help="bunch: Gaussian + uniform ellipsoid (defaults 4M particles, 128^3); "
+                         "cathode: uniform disk with exponential z emitted from a cathode, "
+                         "2M particles, 64x64x128, image charge on")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the CPU-oracle sample (0 disables it)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -181,10 +207,18 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     T = torch.float32 if args.dtype == "f32" else torch.float64
-    n = args.particles
-    grid = (args.grid, args.grid, args.grid)
-    x, y, z, q = make_particles(n, T, dev)
-    mesh = sc.Mesh3D(grid, x, y, z, T=T, gamma=args.gamma, total_charge=1e-9)
+    if args.workload == "cathode":
+        n = args.particles if args.particles != 4_000_000 else 2_000_000
+        grid = (64, 64, 128) if args.grid == 128 else (args.grid, args.grid, 2 * args.grid)
+        args.cathode = True
+        x, y, z, q = make_cathode_particles(n, T, dev)
+        total = 100e-12
+    else:
+        n = args.particles
+        grid = (args.grid, args.grid, args.grid)
+        x, y, z, q = make_particles(n, T, dev)
+        total = 1e-9
+    mesh = sc.Mesh3D(grid, x, y, z, T=T, gamma=args.gamma, total_charge=total)
 
     def step():
         sc.deposit(mesh, x, y, z, q)
@@ -256,7 +290,7 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(args, n, grid, args.dtype, args.cpu_seconds)
+        cpu = cpu_baseline(args, (x, y, z, q), grid, args.dtype, args.cpu_seconds)
 
     out = {
         "metric": "particles/s per full space-charge step (deposit+solve+interpolate)",
@@ -270,7 +304,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.dtype,
-        "data": "synthetic (seeded Gaussian + uniform-ellipsoid bunch generated on the GPU)",
+        "data": ("synthetic (seeded uniform-disk, exponential-z cathode bunch generated on the GPU)"
+                 if args.workload == "cathode" else
+                 "synthetic (seeded Gaussian + uniform-ellipsoid bunch generated on the GPU)"),
         "config": {
             "workload": (f"single bunch, {n} particles, {grid[0]}x{grid[1]}x{grid[2]} mesh, "
                          f"gamma={args.gamma}, {'cathode' if args.cathode else 'free space'}"),

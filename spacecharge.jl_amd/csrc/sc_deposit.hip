@@ -363,8 +363,11 @@ __device__ __forceinline__ void deposit_one(const Rec<PT> &r, const Geo &g, int 
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         if constexpr (FIXED) {
+            // round-to-nearest integer of v*S (|v S| < 2^49): adding 1.5 * 2^52
+            // leaves it in the low mantissa bits of the sum
             unsigned long long *u = reinterpret_cast<unsigned long long *>(sub + o + off[k]);
-            atomicAdd(u, (unsigned long long)__double2ll_rn((double)v[k] * S));
+            const double t = (double)v[k] * S + 6755399441055744.0;
+            atomicAdd(u, (unsigned long long)(__double_as_longlong(t) - 0x4338000000000000LL));
         } else {
             atomicAdd(sub + o + off[k], (double)v[k]);
         }
