@@ -14,6 +14,10 @@ particles, uniform disk of radius 1 mm with z ~ Exp(mean 0.2 mm), 100 pC,
 64x64x128 mesh from the particle extrema, image charge on (solve!(mesh;
 at_cathode=true)).
 
+--workload batched runs 32 independent bunches (Np log-uniform in [1e4, 2e6],
+meshes from 16^3 to 128^3) through the batched calls and reports, beside it,
+the same bunches through a loop of single-bunch calls.
+
 Prints one JSON line.  `value` is particles/s of the whole job (one GPU).
 The `roofline` object is for the kernel with the largest share of the step
 time, measured with the library's HIP-event timers on the launching stream
@@ -74,6 +78,88 @@ def make_cathode_particles(n, dtype, device, seed=7):
     z = (-0.2e-3 * torch.log1p(-u[:, 2])).to(dtype).contiguous()
     q = torch.full((n,), 100e-12 / n, dtype=dtype, device=device)
     return x, y, z, q
+
+
+def make_batch(dtype, device, nb=32, seed=11):
+    """Many independent bunches: Np = floor(10^U(4, log10 2e6)) Gaussians with
+    sigma ~ U(0.5, 3) mm per axis, each on a mesh drawn from 16^3, 32^3,
+    64x64x128, 128^3 (seeded)."""
+    import math
+    import torch
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    grids = [(16, 16, 16), (32, 32, 32), (64, 64, 128), (128, 128, 128)]
+    out = []
+    for b in range(nb):
+        u = torch.rand(5, generator=g, dtype=torch.float64)
+        n = int(10 ** (4 + float(u[0]) * (math.log10(2e6) - 4)))
+        sig = (0.5e-3 + 2.5e-3 * u[1:4]).to(device)
+        grid = grids[int(float(u[4]) * 4) % 4]
+        gd = torch.Generator(device=device)
+        gd.manual_seed(seed * 1000 + b)
+        p = (torch.randn((n, 3), generator=gd, dtype=torch.float64, device=device) * sig).to(dtype)
+        x, y, z = (p[:, i].contiguous() for i in range(3))
+        q = torch.full((n,), 1e-10 / n, dtype=dtype, device=device)
+        out.append((grid, (x, y, z, q)))
+    return out
+
+
+def bench_batched(args, sc, torch, dev, T):
+    """Batched step over many bunches against the loop of single-bunch calls."""
+    bunches = make_batch(T, dev)
+    ntot = sum(p[0].numel() for _, p in bunches)
+    meshes = [sc.Mesh3D(grid, p[0], p[1], p[2], T=T, gamma=args.gamma) for grid, p in bunches]
+    loop_meshes = [sc.Mesh3D(grid, p[0], p[1], p[2], T=T, gamma=args.gamma) for grid, p in bunches]
+    parts = [p for _, p in bunches]
+
+    def step_batched():
+        sc.deposit_batched(meshes, *[[p[k] for p in parts] for k in range(4)])
+        sc.solve_batched(meshes)
+        return sc.interpolate_batched(meshes, *[[p[k] for p in parts] for k in range(3)])
+
+    def step_loop():
+        for m, p in zip(loop_meshes, parts):
+            sc.deposit(m, *p)
+            sc.solve(m)
+            sc.interpolate_field(m, p[0], p[1], p[2])
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    ms = timed(step_batched)
+    ms_loop = timed(step_loop)
+    return {
+        "metric": "particles/s per full space-charge step (deposit+solve+interpolate), many bunches",
+        "value": ntot / (ms * 1e-3),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded Gaussian bunches generated on the GPU)",
+        "config": {
+            "workload": (f"{len(bunches)} bunches, {ntot} particles in total, Np log-uniform in "
+                         "[1e4, 2e6], meshes from 16^3, 32^3, 64x64x128, 128^3, free space"),
+            "particles": ntot,
+            "bunches": len(bunches),
+            "grids": [list(g) for g, _ in bunches],
+            "gamma": args.gamma,
+            "parallelism": "single GPU, bunches on a pool of streams",
+        },
+        "sequential_ms_per_step": ms_loop,
+        "batched_speedup": ms_loop / ms,
+    }
 
 
 def kernel_bytes(name, n, grid, s, ps):
@@ -185,10 +271,11 @@ def main():
     ap.add_argument("--particles", type=int, default=4_000_000)
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cathode", action="store_true")
-    ap.add_argument("--workload", choices=["bunch", "cathode"], default="bunch",
+    ap.add_argument("--workload", choices=["bunch", "cathode", "batched"], default="bunch",
                     help="bunch: Gaussian + uniform ellipsoid (defaults 4M particles, 128^3); "
                          "cathode: uniform disk with exponential z emitted from a cathode, "
-                         "2M particles, 64x64x128, image charge on")
+                         "2M particles, 64x64x128, image charge on; batched: 32 bunches of "
+                         "uneven size and mesh through the batched calls")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the CPU-oracle sample (0 disables it)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -207,6 +294,9 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     T = torch.float32 if args.dtype == "f32" else torch.float64
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args, sc, torch, dev, T)))
+        return
     if args.workload == "cathode":
         n = args.particles if args.particles != 4_000_000 else 2_000_000
         grid = (64, 64, 128) if args.grid == 128 else (args.grid, args.grid, 2 * args.grid)

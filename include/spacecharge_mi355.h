@@ -148,6 +148,30 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
                    int64_t n, const double min_bounds[3], const double delta[3], void *Ex,
                    void *Ey, void *Ez);
 
+/* ---- many bunches per call (no reference counterpart: the reference runs
+ * one mesh per call, e.g. benchmark/full_pipeline_benchmark.jl:25-47; these
+ * serve the batched configuration of many independent bunches) -------------
+ * Each performs its single-bunch counterpart (sc_deposit, sc_solve,
+ * sc_interpolate) for nb bunches, bunch b on meshes[b].  The meshes must be
+ * distinct and belong to one context.  The bunches run concurrently on the
+ * context's stream pool (sc_ctx_set_pool_size, default 4 streams, default
+ * priority): the pool streams wait for the work already on the context's
+ * stream, bunch b is launched on pool stream b mod P, and the context's
+ * stream then waits for all of them, so a batched call is ordered like a
+ * sequence of its single-bunch counterparts and computes the same results.
+ * Per-bunch arrays have nb entries; bounds, deltas have 3*nb (bunch-major). */
+int sc_ctx_set_pool_size(sc_ctx *ctx, int nstreams);
+int sc_deposit_batched(sc_mesh *const *meshes, int nb, int pdtype, const void *const *x,
+                       const void *const *y, const void *const *z, const void *const *q,
+                       const int64_t *n, const double *min_bounds, const double *delta, int clear);
+int sc_solve_batched(sc_mesh *const *meshes, int nb, const double *min_bounds,
+                     const double *max_bounds, const double *delta, const double *gamma,
+                     const int *at_cathode);
+int sc_interpolate_batched(sc_mesh *const *meshes, int nb, int pdtype, const void *const *x,
+                           const void *const *y, const void *const *z, const int64_t *n,
+                           const double *min_bounds, const double *delta, void *const *Ex,
+                           void *const *Ey, void *const *Ez);
+
 /* ---- per-kernel timing (no reference counterpart; the reference profiles
  * with CUDA.@profile, benchmark/full_pipeline_profile.jl:108-111) ----------
  * When enabled, every kernel launch on the context is bracketed by HIP

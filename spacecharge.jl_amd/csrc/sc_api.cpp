@@ -150,6 +150,12 @@ int sc_ctx_destroy(sc_ctx *ctx)
         (void)hipStreamSynchronize(c->side);
         (void)hipStreamDestroy(c->side);
     }
+    for (auto s : c->bstreams) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    for (auto e : c->bjoin) (void)hipEventDestroy(e);
+    if (c->bfork) (void)hipEventDestroy(c->bfork);
     delete ctx;
     return SC_OK;
 }
@@ -438,6 +444,162 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
     if (int rc = check_geom(c, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
     return launch_interp(&M, pdtype, x, y, z, n, lo, dl, Ex, Ey, Ez);
+}
+
+// ---- many bunches per call ----
+int sc_ctx_set_pool_size(sc_ctx *ctx, int nstreams)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    if (nstreams < 1 || nstreams > 16)
+        return set_err(&ctx->c, SC_ERR_INVALID, "pool size must be in [1, 16]");
+    ctx->c.pool_size = nstreams;
+    return SC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Validates a batch of meshes (non-NULL, distinct, one context) and returns
+// that context.
+Ctx *batch_ctx(sc_mesh *const *meshes, int nb, int *rc)
+{
+    *rc = SC_OK;
+    if (!meshes || nb < 1) {
+        *rc = set_err(nullptr, SC_ERR_INVALID, "a batch needs at least one mesh");
+        return nullptr;
+    }
+    if (!meshes[0]) {
+        *rc = set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+        return nullptr;
+    }
+    Ctx *c = meshes[0]->m.ctx;
+    for (int b = 0; b < nb; ++b) {
+        if (!meshes[b]) {
+            *rc = set_err(c, SC_ERR_INVALID, "mesh is NULL");
+            return nullptr;
+        }
+        if (meshes[b]->m.ctx != c) {
+            *rc = set_err(c, SC_ERR_INVALID, "all meshes of a batch must share one context");
+            return nullptr;
+        }
+        // the bunches run concurrently: each needs its own mesh (and workspace)
+        for (int a = 0; a < b; ++a) {
+            if (meshes[a] == meshes[b]) {
+                *rc = set_err(c, SC_ERR_INVALID, "a mesh appears twice in one batch");
+                return nullptr;
+            }
+        }
+    }
+    return c;
+}
+
+// Runs op(b) for every bunch with c->stream set to its pool stream, between
+// a fork from and a join back to the caller's stream.
+template <typename Op>
+int run_batched(Ctx *c, int nb, Op op)
+{
+    if (int rc = check_dev(c)) return rc;
+    const int P = nb < c->pool_size ? nb : c->pool_size;
+    while ((int)c->bstreams.size() < P) {
+        hipStream_t s = nullptr;
+        hipEvent_t e = nullptr;
+        SC_HIP(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));  // default priority
+        c->bstreams.push_back(s);
+        SC_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->bjoin.push_back(e);
+    }
+    if (!c->bfork) SC_HIP(c, hipEventCreateWithFlags(&c->bfork, hipEventDisableTiming));
+    hipStream_t home = c->stream;
+    SC_HIP(c, hipEventRecord(c->bfork, home));
+    for (int i = 0; i < P; ++i) SC_HIP(c, hipStreamWaitEvent(c->bstreams[i], c->bfork, 0));
+    int rc = SC_OK;
+    c->in_batch = true;
+    for (int b = 0; b < nb && rc == SC_OK; ++b) {
+        c->stream = c->bstreams[b % P];
+        rc = op(b);
+    }
+    c->stream = home;
+    c->in_batch = false;
+    // join every used stream, also after an error, so the caller's stream
+    // orders after whatever was launched
+    for (int i = 0; i < P; ++i) {
+        if (hipEventRecord(c->bjoin[i], c->bstreams[i]) != hipSuccess ||
+            hipStreamWaitEvent(home, c->bjoin[i], 0) != hipSuccess)
+            if (rc == SC_OK) rc = set_err(c, SC_ERR_DEVICE, "joining the batch streams failed");
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sc_deposit_batched(sc_mesh *const *meshes, int nb, int pdtype, const void *const *x,
+                       const void *const *y, const void *const *z, const void *const *q,
+                       const int64_t *n, const double *min_bounds, const double *delta, int clear)
+{
+    int rc;
+    Ctx *c = batch_ctx(meshes, nb, &rc);
+    if (!c) return rc;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (!x || !y || !z || !q || !n || !min_bounds || !delta)
+        return set_err(c, SC_ERR_INVALID, "NULL per-bunch array");
+    for (int b = 0; b < nb; ++b) {
+        if (n[b] < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+        if (n[b] > 0 && (!x[b] || !y[b] || !z[b] || !q[b]))
+            return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
+        if (int e = check_geom(c, min_bounds + 3 * b, delta + 3 * b)) return e;
+    }
+    return run_batched(c, nb, [&](int b) {
+        return launch_deposit(&meshes[b]->m, pdtype, x[b], y[b], z[b], q[b], n[b], min_bounds + 3 * b,
+                              delta + 3 * b, clear);
+    });
+}
+
+int sc_solve_batched(sc_mesh *const *meshes, int nb, const double *min_bounds,
+                     const double *max_bounds, const double *delta, const double *gamma,
+                     const int *at_cathode)
+{
+    int rc;
+    Ctx *c = batch_ctx(meshes, nb, &rc);
+    if (!c) return rc;
+    if (!min_bounds || !max_bounds || !delta || !gamma || !at_cathode)
+        return set_err(c, SC_ERR_INVALID, "NULL per-bunch array");
+    for (int b = 0; b < nb; ++b) {
+        if (int e = check_geom(c, min_bounds + 3 * b, delta + 3 * b)) return e;
+        if (!std::isfinite(gamma[b]) || !(gamma[b] > 0.0))
+            return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
+        if (at_cathode[b] && !std::isfinite(max_bounds[3 * b + 2]))
+            return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
+    }
+    return run_batched(c, nb, [&](int b) {
+        return solve_mesh(&meshes[b]->m, min_bounds + 3 * b, max_bounds + 3 * b, delta + 3 * b,
+                          gamma[b], at_cathode[b]);
+    });
+}
+
+int sc_interpolate_batched(sc_mesh *const *meshes, int nb, int pdtype, const void *const *x,
+                           const void *const *y, const void *const *z, const int64_t *n,
+                           const double *min_bounds, const double *delta, void *const *Ex,
+                           void *const *Ey, void *const *Ez)
+{
+    int rc;
+    Ctx *c = batch_ctx(meshes, nb, &rc);
+    if (!c) return rc;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (!x || !y || !z || !n || !min_bounds || !delta || !Ex || !Ey || !Ez)
+        return set_err(c, SC_ERR_INVALID, "NULL per-bunch array");
+    for (int b = 0; b < nb; ++b) {
+        if (n[b] < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+        if (n[b] > 0 && (!x[b] || !y[b] || !z[b] || !Ex[b] || !Ey[b] || !Ez[b]))
+            return set_err(c, SC_ERR_INVALID, "NULL particle or output pointer");
+        if (int e = check_geom(c, min_bounds + 3 * b, delta + 3 * b)) return e;
+    }
+    return run_batched(c, nb, [&](int b) {
+        return launch_interp(&meshes[b]->m, pdtype, x[b], y[b], z[b], n[b], min_bounds + 3 * b,
+                             delta + 3 * b, Ex[b], Ey[b], Ez[b]);
+    });
 }
 
 // ---- timers ----

@@ -93,6 +93,15 @@ struct Ctx {
     // the Green-function build of a solve runs on it, concurrently with the
     // deposit still executing on `stream`
     hipStream_t side = nullptr;
+    // batched calls: bunch b runs on bstreams[b % pool_size]; these fork
+    // from `stream` (fork event) and join back (one event per stream)
+    int pool_size = 4;
+    std::vector<hipStream_t> bstreams;
+    std::vector<hipEvent_t> bjoin;
+    hipEvent_t bfork = nullptr;
+    // set while a batched call launches: the solve keeps the Green build on
+    // the bunch's own stream (the bunches overlap each other instead)
+    bool in_batch = false;
 };
 
 // Device workspace of one mesh (lazy, like src/mesh.jl:49-72).

@@ -279,7 +279,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     if (!reuse) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         (void)hipStreamIsCapturing(st, &cs);
-        overlap = cs == hipStreamCaptureStatusNone && ensure_side_stream(m) == SC_OK;
+        overlap = cs == hipStreamCaptureStatusNone && !ctx->in_batch && ensure_side_stream(m) == SC_OK;
         if (overlap) {
             if (ws.gf_free_recorded) SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.gf_free, 0));
             if (ws.deposit_start_recorded)

@@ -10,6 +10,9 @@ Mirrors the SpaceCharge.jl API (src/SpaceCharge.jl:17):
     solve!(mesh; at_cathode=false)            solve(mesh, at_cathode=False)
     interpolate_field(mesh, x, y, z)          interpolate_field(mesh, x, y, z)
 
+and adds batched forms for many independent bunches (deposit_batched,
+solve_batched, interpolate_batched), which the reference does not have.
+
 Every call forwards to one function of the C ABI (include/spacecharge_mi355.h)
 through ctypes; the arithmetic runs in the hand-written gfx950 kernels of
 libspacecharge_mi355.so.  Device memory and streams come from torch: mesh
@@ -43,6 +46,10 @@ __all__ = [
     "FPEI",
     "synchronize",
     "KernelTimer",
+    "deposit_batched",
+    "solve_batched",
+    "interpolate_batched",
+    "set_stream_pool_size",
 ]
 
 # src/utils.jl:7-8
@@ -269,6 +276,98 @@ def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
                                         Ez.data_ptr())
     _check(rc, ctx)
     return Ex, Ey, Ez
+
+
+# ---- many bunches per call (no reference counterpart) ----
+
+def _batch_device(meshes) -> torch.device:
+    if len(meshes) == 0:
+        raise SpaceChargeError("a batch needs at least one mesh")
+    dev = meshes[0].device
+    if any(m.device != dev for m in meshes):
+        raise SpaceChargeError("all meshes of a batch must be on one device")
+    if len({id(m) for m in meshes}) != len(meshes):
+        raise SpaceChargeError("a mesh appears twice in one batch")
+    return dev
+
+
+def _ptrs(ts):
+    return (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
+def _flat3(vals):
+    return (ctypes.c_double * (3 * len(vals)))(*[float(a) for v in vals for a in v])
+
+
+def _batch_particles(meshes, groups, names):
+    """Per-bunch particle tensors on the device, one particle dtype for the batch."""
+    out = [[] for _ in names]
+    dtype = None
+    for b, m in enumerate(meshes):
+        arrs = [g[b] for g in groups]
+        lens = [len(a) for a in arrs]
+        if not all(n == lens[0] for n in lens):
+            raise SpaceChargeError("Particle coordinate and charge arrays must have the same length.")
+        for k, a in enumerate(arrs):
+            t = _as_device(a, m.device, dtype)
+            dtype = dtype or t.dtype
+            out[k].append(t)
+    return out
+
+
+def deposit_batched(meshes: Sequence[Mesh3D], xs, ys, zs, qs, *, clear: bool = True) -> None:
+    """deposit! for many bunches at once (bunch b into meshes[b]); the bunches run
+    concurrently on the library's stream pool.  Same results as a loop of deposit()."""
+    dev = _batch_device(meshes)
+    x, y, z, q = _batch_particles(meshes, (xs, ys, zs, qs), "xyzq")
+    n = (ctypes.c_int64 * len(meshes))(*[t.numel() for t in x])
+    ctx = _ctx(dev)
+    with torch.cuda.device(dev):
+        rc = _lib.load().sc_deposit_batched(
+            (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes]), len(meshes), _DT[x[0].dtype],
+            _ptrs(x), _ptrs(y), _ptrs(z), _ptrs(q), n, _flat3([m.min_bounds for m in meshes]),
+            _flat3([m.delta for m in meshes]), int(bool(clear)))
+    _check(rc, ctx)
+
+
+def solve_batched(meshes: Sequence[Mesh3D], *, at_cathode=False) -> None:
+    """solve! for many bunches at once; at_cathode is one flag or one per bunch."""
+    dev = _batch_device(meshes)
+    flags = list(at_cathode) if isinstance(at_cathode, (list, tuple)) else [at_cathode] * len(meshes)
+    if len(flags) != len(meshes):
+        raise SpaceChargeError("at_cathode needs one flag per mesh")
+    ctx = _ctx(dev)
+    with torch.cuda.device(dev):
+        rc = _lib.load().sc_solve_batched(
+            (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes]), len(meshes),
+            _flat3([m.min_bounds for m in meshes]), _flat3([m.max_bounds for m in meshes]),
+            _flat3([m.delta for m in meshes]),
+            (ctypes.c_double * len(meshes))(*[float(m.gamma) for m in meshes]),
+            (ctypes.c_int * len(meshes))(*[int(bool(f)) for f in flags]))
+    _check(rc, ctx)
+
+
+def interpolate_batched(meshes: Sequence[Mesh3D], xs, ys, zs):
+    """interpolate_field for many bunches at once: a list of (Ex, Ey, Ez) per bunch."""
+    dev = _batch_device(meshes)
+    x, y, z = _batch_particles(meshes, (xs, ys, zs), "xyz")
+    E = [[torch.empty_like(t) for t in x] for _ in range(3)]
+    n = (ctypes.c_int64 * len(meshes))(*[t.numel() for t in x])
+    ctx = _ctx(dev)
+    with torch.cuda.device(dev):
+        rc = _lib.load().sc_interpolate_batched(
+            (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes]), len(meshes), _DT[x[0].dtype],
+            _ptrs(x), _ptrs(y), _ptrs(z), n, _flat3([m.min_bounds for m in meshes]),
+            _flat3([m.delta for m in meshes]), _ptrs(E[0]), _ptrs(E[1]), _ptrs(E[2]))
+    _check(rc, ctx)
+    return [(E[0][b], E[1][b], E[2][b]) for b in range(len(meshes))]
+
+
+def set_stream_pool_size(nstreams: int, device=None) -> None:
+    """Number of streams the batched calls spread bunches over (default 4)."""
+    dev = torch.device(device) if device is not None else _default_device()
+    ctx = _ctx(dev)
+    _check(_lib.load().sc_ctx_set_pool_size(ctx, int(nstreams)), ctx)
 
 
 def synchronize(device=None) -> None:

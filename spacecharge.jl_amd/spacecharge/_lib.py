@@ -20,6 +20,8 @@ _vp, _i, _i64, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_doub
 _pd = ctypes.POINTER(ctypes.c_double)
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 _sz = ctypes.c_size_t
+_pvp = ctypes.POINTER(_vp)
+_pi = ctypes.POINTER(_i)
 PROTOTYPES = {
     "sc_ctx_create": (_i, [_i, ctypes.POINTER(_vp)]),
     "sc_ctx_destroy": (_i, [_vp]),
@@ -47,6 +49,11 @@ PROTOTYPES = {
     "sc_deposit": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64, _pd, _pd, _i]),
     "sc_solve": (_i, [_vp, _pd, _pd, _pd, _d, _i]),
     "sc_interpolate": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pd, _pd, _vp, _vp, _vp]),
+    "sc_ctx_set_pool_size": (_i, [_vp, _i]),
+    "sc_deposit_batched": (_i, [_pvp, _i, _i, _pvp, _pvp, _pvp, _pvp, _pi64, _pd, _pd, _i]),
+    "sc_solve_batched": (_i, [_pvp, _i, _pd, _pd, _pd, _pd, _pi]),
+    "sc_interpolate_batched": (_i, [_pvp, _i, _i, _pvp, _pvp, _pvp, _pi64, _pd, _pd, _pvp, _pvp,
+                                    _pvp]),
     "sc_timing_enable": (_i, [_vp, _i]),
     "sc_timing_reset": (_i, [_vp]),
     "sc_kernel_count": (_i, []),
