@@ -54,6 +54,10 @@ def _load():
     lib.sco_solve_extended.argtypes = [pll, pd, pd, pd, d, i, vp, vp, i]
     lib.sco_fft3d.argtypes = [i, pll, vp, i, i]
     lib.sco_fpei.restype = d
+    lib.sco_solve_potential.argtypes = [i, pll, pd, d, vp, vp, i]
+    lib.sco_solve_potential_extended.argtypes = [pll, pd, d, vp, vp, i]
+    lib.sco_potential_green_function_f64.argtypes = [d, d, d]
+    lib.sco_potential_green_function_f64.restype = d
     _lib = lib
     return lib
 
@@ -193,3 +197,33 @@ def fft3d(a, sign=-1, nthreads=0):
     if rc != 0:
         raise OracleError(f"fft3d failed ({rc})")
     return a
+
+
+def solve_potential(rho, delta, gamma=1.0, nthreads=0):
+    """Scalar potential (free space) of rho, shape (nx, ny, nz), Fortran order,
+    in rho's dtype: FPEI * gamma/(dx dy dz gamma) * (rho (*) cell integral of 1/r)."""
+    lib = _load()
+    rho = np.asfortranarray(rho)
+    phi = np.zeros(rho.shape, dtype=rho.dtype, order="F")
+    rc = lib.sco_solve_potential(_code(rho.dtype), _ll3(rho.shape), _d3(delta), float(gamma),
+                                 _ptr(rho), _ptr(phi), nthreads)
+    if rc:
+        raise RuntimeError(f"sco_solve_potential failed ({rc})")
+    return phi
+
+
+def solve_potential_extended(rho, delta, gamma=1.0, nthreads=0):
+    """solve_potential in x87 extended precision (Float64 in and out)."""
+    lib = _load()
+    rho = np.asfortranarray(rho, dtype=np.float64)
+    phi = np.zeros(rho.shape, dtype=np.float64, order="F")
+    rc = lib.sco_solve_potential_extended(_ll3(rho.shape), _d3(delta), float(gamma), _ptr(rho),
+                                          _ptr(phi), nthreads)
+    if rc:
+        raise RuntimeError(f"sco_solve_potential_extended failed ({rc})")
+    return phi
+
+
+def potential_green_function(x, y, z):
+    """src/green_functions.jl:13-22 in Float64."""
+    return _load().sco_potential_green_function_f64(x, y, z)

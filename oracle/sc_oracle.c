@@ -350,7 +350,44 @@ int sco_solve_extended(const long long grid[3], const double lo[3], const double
     return rc;
 }
 
+/* Scalar potential of rho in free space (no reference solver; the Green
+ * function is src/green_functions.jl:13-22).  phi has M elements. */
+int sco_solve_potential(int mdt, const long long grid[3], const double delta[3], double gamma,
+                        const void *rho, void *phi, int nthreads)
+{
+    const int g[3] = {(int)grid[0], (int)grid[1], (int)grid[2]};
+    int nth = sco_threads(nthreads);
+    if (mdt == SCO_F64) return solve_potential_f64(rho, phi, g, delta, gamma, nth);
+    if (mdt == SCO_F32) {
+        const float d[3] = {(float)delta[0], (float)delta[1], (float)delta[2]};
+        return solve_potential_f32(rho, phi, g, d, (float)gamma, nth);
+    }
+    return -4;
+}
+
+int sco_solve_potential_extended(const long long grid[3], const double delta[3], double gamma,
+                                 const double *rho, double *phi, int nthreads)
+{
+    const int g[3] = {(int)grid[0], (int)grid[1], (int)grid[2]};
+    const long long M = grid[0] * grid[1] * grid[2];
+    long double *r = (long double *)malloc(sizeof(long double) * (size_t)M);
+    long double *p = (long double *)malloc(sizeof(long double) * (size_t)M);
+    if (!r || !p) { free(r); free(p); return -1; }
+    for (long long i = 0; i < M; ++i) r[i] = rho[i];
+    const long double d[3] = {delta[0], delta[1], delta[2]};
+    int rc = solve_potential_f80(r, p, g, d, (long double)gamma, sco_threads(nthreads));
+    for (long long i = 0; i < M; ++i) phi[i] = (double)p[i];
+    free(r);
+    free(p);
+    return rc;
+}
+
 /* ---- internals exposed for the oracle's own tests ---- */
+
+double sco_potential_green_function_f64(double x, double y, double z)
+{
+    return potential_green_function_f64(x, y, z);
+}
 
 double sco_field_green_function_f64(double x, double y, double z)
 {

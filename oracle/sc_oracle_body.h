@@ -219,6 +219,24 @@ static R SFX(field_green_function)(R x, R y, R z)
     return (t1 - t2) + t3;
 }
 
+/* src/green_functions.jl:13-22 -- antiderivative of 1/r (the scalar
+ * potential), zero at r = 0:
+ *   -z^2 atan(xy/(zr))/2 - y^2 atan(xz/(yr))/2 - x^2 atan(yz/(xr))/2
+ *   + yz log(x+r) + xz log(y+r) + xy log(z+r) */
+static R SFX(potential_green_function)(R x, R y, R z)
+{
+    R r = SFX(sqrt_)((x * x + y * y) + z * z);
+    if (r == (R)0) return (R)0;
+    const R half = (R)1 / (R)2;
+    R t = -half * (z * z) * SFX(atan_)((x * y) / (z * r));
+    t = t - half * (y * y) * SFX(atan_)((x * z) / (y * r));
+    t = t - half * (x * x) * SFX(atan_)((y * z) / (x * r));
+    t = t + (y * z) * SFX(log_)(x + r);
+    t = t + (x * z) * SFX(log_)(y + r);
+    t = t + (x * y) * SFX(log_)(z + r);
+    return t;
+}
+
 /* src/green_functions.jl:41-112 -- get_green_function!: pointwise F*factor on
  * the (2nx,2ny,2nz) node lattice (get_green_kernel!, :69-101), then the
  * 8-point difference (apply_8point_differencing!, :103-112) written to a
@@ -233,7 +251,9 @@ static void SFX(get_green_function)(R *cgrn, R *temp, const int grid[3],
     /* get_green_kernel! (:69-101) */
     const R dx = delta[0], dy = delta[1], dz = delta[2] * gamma;
     R factor;
-    if (icomp == 1 || icomp == 2) factor = gamma / ((dx * dy) * dz);
+    /* icomp 4: the scalar potential (no solver in the reference uses it);
+     * the lab-frame potential gamma * phi_rest takes the transverse factor */
+    if (icomp == 1 || icomp == 2 || icomp == 4) factor = gamma / ((dx * dy) * dz);
     else factor = (R)1 / ((dx * dy) * dz);
     const R umin = ((R)(1 - isz) * dx) / (R)2 + offset[0];
     const R vmin = ((R)(1 - jsz) * dy) / (R)2 + offset[1];
@@ -249,6 +269,7 @@ static void SFX(get_green_function)(R *cgrn, R *temp, const int grid[3],
                 if (icomp == 1) g = SFX(field_green_function)(u, v, w) * factor;
                 else if (icomp == 2) g = SFX(field_green_function)(v, w, u) * factor;
                 else if (icomp == 3) g = SFX(field_green_function)(w, u, v) * factor;
+                else if (icomp == 4) g = SFX(potential_green_function)(u, v, w) * factor;
                 else g = (R)0;
                 long long idx = i + isz * (j + jsz * k);
                 cgrn[2 * idx] = g;
@@ -293,7 +314,7 @@ static void SFX(get_green_function)(R *cgrn, R *temp, const int grid[3],
  * lazily created workspace of src/mesh.jl:49-72. */
 static void SFX(solve_freespace)(const R *rho, R *efield, const int grid[3],
                                  const R delta[3], R gamma, const R offset[3],
-                                 R *crho, R *cgrn, R *temp, int nthreads)
+                                 R *crho, R *cgrn, R *temp, int nthreads, int c0, int c1)
 {
     const long long nx = grid[0], ny = grid[1], nz = grid[2];
     const long long isz = 2 * nx, jsz = 2 * ny, ksz = 2 * nz;
@@ -311,7 +332,7 @@ static void SFX(solve_freespace)(const R *rho, R *efield, const int grid[3],
     const R factr = (R)SCO_FPEI;
     /* AbstractFFTs normalisation of plan_ifft!: one(T)/prod(size) */
     const R scale = (R)1 / (R)P;
-    for (int icomp = 1; icomp <= 3; ++icomp) {
+    for (int icomp = c0; icomp <= c1; ++icomp) {
         SFX(get_green_function)(cgrn, temp, grid, delta, gamma, icomp, offset, nthreads);
         SFX(fft3d)(cgrn, dims, -1, nthreads);                       /* :89 */
         #pragma omp parallel for schedule(static) num_threads(nthreads)
@@ -328,7 +349,7 @@ static void SFX(solve_freespace)(const R *rho, R *efield, const int grid[3],
             temp[2 * e + 1] = temp[2 * e + 1] * scale;
         }
         /* efield[:,:,:,icomp] = factr*real(temp[nx:2nx-1, ...])  (:98-99) */
-        R *E = efield + (size_t)(icomp - 1) * M;
+        R *E = efield + (size_t)(icomp - c0) * M;
         for (long long k = 0; k < nz; ++k)
             for (long long j = 0; j < ny; ++j)
                 for (long long i = 0; i < nx; ++i) {
@@ -352,7 +373,7 @@ static int SFX(solve)(const R *rho, R *efield, const int grid[3], const R lo[3],
     R *temp = (R *)malloc(sizeof(R) * 2 * (size_t)P);
     if (!crho || !cgrn || !temp) { free(crho); free(cgrn); free(temp); return -1; }
     const R zero3[3] = {0, 0, 0};
-    SFX(solve_freespace)(rho, efield, grid, delta, gamma, zero3, crho, cgrn, temp, nthreads);
+    SFX(solve_freespace)(rho, efield, grid, delta, gamma, zero3, crho, cgrn, temp, nthreads, 1, 3);
     if (at_cathode) {
         R *irho = (R *)malloc(sizeof(R) * (size_t)M);
         R *ief = (R *)malloc(sizeof(R) * 3 * (size_t)M);
@@ -362,11 +383,32 @@ static int SFX(solve)(const R *rho, R *efield, const int grid[3], const R lo[3],
                     irho[i + nx * (j + ny * k)] = -rho[i + nx * (j + ny * (nz - 1 - k))];
         const R offz = (R)2 * lo[2] + (hi[2] - lo[2]);
         const R off[3] = {0, 0, offz};
-        SFX(solve_freespace)(irho, ief, grid, delta, gamma, off, crho, cgrn, temp, nthreads);
+        SFX(solve_freespace)(irho, ief, grid, delta, gamma, off, crho, cgrn, temp, nthreads, 1, 3);
         for (long long e = 0; e < 3 * M; ++e) efield[e] = efield[e] + ief[e];
         free(irho);
         free(ief);
     }
+    free(crho);
+    free(cgrn);
+    free(temp);
+    return 0;
+}
+
+/* Scalar potential, free space: phi = FPEI * gamma/(dx dy dz_rest) *
+ * sum_n rho[n] * (cell integral of 1/r), the same linear convolution as the
+ * field solve (src/solvers/free_space.jl:56-101) with the antiderivative of
+ * src/green_functions.jl:13-22.  The reference defines that function but no
+ * solver calls it. */
+static int SFX(solve_potential)(const R *rho, R *phi, const int grid[3], const R delta[3],
+                                R gamma, int nthreads)
+{
+    const long long P = 8LL * grid[0] * grid[1] * grid[2];
+    R *crho = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    R *cgrn = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    R *temp = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    if (!crho || !cgrn || !temp) { free(crho); free(cgrn); free(temp); return -1; }
+    const R zero3[3] = {0, 0, 0};
+    SFX(solve_freespace)(rho, phi, grid, delta, gamma, zero3, crho, cgrn, temp, nthreads, 4, 4);
     free(crho);
     free(cgrn);
     free(temp);

@@ -17,6 +17,11 @@ code or formula:
                            the adjoint identity <interp(E), q> = <E, deposit(q)>.
   * bounds                 the closed form of src/mesh.jl:95-174 and
                            test/test_mesh.jl's assertions.
+  * potential              40-digit mpmath values of the antiderivative of 1/r
+                           (tests/golden/pot_golden.json, whose generator checks
+                           d3P/dxdydz = 1/r) and cell integrals of 1/r computed
+                           without it; brute-force direct sum; the analytical
+                           potential of an isotropic Gaussian.
 """
 from __future__ import annotations
 
@@ -32,6 +37,7 @@ import oracle as orc
 from _common import config_a, gaussian, rel_l2
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "igf_golden.json")))
+PGOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "pot_golden.json")))
 FPEI = 299792458.0 ** 2 * 1.0e-7
 
 
@@ -381,3 +387,125 @@ def test_reference_self_field_is_roundoff():
     one_cell = np.array([ef[1, 0, 0, 0], ef[0, 1, 0, 1], ef[0, 0, 1, 2]])
     assert (at_charge > 1e7).all()            # the reference's assertion holds ...
     assert (at_charge < 1e-12 * one_cell).all()  # ... at round-off level
+
+
+# ---------------------------------------------------------------- scalar potential
+def test_potential_green_function_golden():
+    for e in PGOLD["P"]:
+        x, y, z = e["point"]
+        ref = float(e["P"])
+        got = orc.potential_green_function(x, y, z)
+        assert abs(got - ref) <= 4e-15 * max(abs(ref), 1.0), (e["point"], got, ref)
+    assert orc.potential_green_function(0.0, 0.0, 0.0) == 0.0
+
+
+@pytest.mark.parametrize("gamma", [1.0, 3.0])
+def test_potential_lattice_cell_integrals(gamma):
+    """Differenced lattice entry = gamma/vol * (integral of 1/r over the cell)."""
+    grid = (32, 32, 32)
+    for h in ({tuple(e["h"]) for e in PGOLD["cells"]}):
+        delta = (h[0], h[1], h[2] / gamma)
+        vol = h[0] * h[1] * h[2]
+        lat = orc.green_lattice(grid, delta, gamma, 4).real
+        factor = gamma / vol
+        for e in PGOLD["cells"]:
+            if tuple(e["h"]) != h:
+                continue
+            a, b, c = e["cell"]
+            got = lat[a + grid[0] - 1, b + grid[1] - 1, c + grid[2] - 1]
+            ref = factor * float(e["J"])
+            dist = max(1.0, float(np.sqrt(a * a + b * b + c * c)))
+            # the 8-point difference of P ~ r^2 cancels like distance^4
+            tol = 1e-14 * dist ** 4 * factor + 1e-12 * abs(ref)
+            assert abs(got - ref) <= tol, (h, e["cell"], got, ref)
+
+
+def _cell_integrals_inv_r(span, h):
+    """scipy integral of 1/r over every cell with displacement in span (z in
+    closed form, the x-y integral split at the axis)."""
+    out = {}
+    for a in range(-span[0], span[0] + 1):
+        for b in range(-span[1], span[1] + 1):
+            for c in range(-span[2], span[2] + 1):
+                lo = [(a - 0.5) * h[0], (b - 0.5) * h[1], (c - 0.5) * h[2]]
+                hi = [(a + 0.5) * h[0], (b + 0.5) * h[1], (c + 0.5) * h[2]]
+
+                def f(v, u):
+                    rho = np.hypot(u, v)
+                    return np.arcsinh(hi[2] / rho) - np.arcsinh(lo[2] / rho)
+
+                xs = [lo[0], 0.0, hi[0]] if lo[0] < 0 < hi[0] else [lo[0], hi[0]]
+                ys = [lo[1], 0.0, hi[1]] if lo[1] < 0 < hi[1] else [lo[1], hi[1]]
+                val = 0.0
+                for x0, x1 in zip(xs[:-1], xs[1:]):
+                    for y0, y1 in zip(ys[:-1], ys[1:]):
+                        val += integrate.dblquad(f, x0, x1, y0, y1, epsabs=1e-13,
+                                                 epsrel=1e-11)[0]
+                out[(a, b, c)] = val
+    return out
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_solve_potential_equals_brute_force_direct_sum(dt):
+    """phi(m) = FPEI * gamma/vol * sum_n rho(n) J(m - n) on a ragged grid, with
+    the cells integrated by scipy (not through the antiderivative)."""
+    grid = (4, 3, 5)
+    h = (0.5, 0.8, 0.6)
+    gamma = 1.5
+    rng = np.random.default_rng(6)
+    rho = np.asfortranarray(rng.uniform(-1, 1, grid) * 1e-12)
+    phi = orc.solve_potential(rho.astype(dt), h, gamma)
+    assert phi.dtype == dt
+    hz = (h[0], h[1], h[2] * gamma)
+    vol = hz[0] * hz[1] * hz[2]
+    J = _cell_integrals_inv_r(tuple(g - 1 for g in grid), hz)
+    ref = np.zeros(grid)
+    for m in np.ndindex(grid):
+        ref[m] = sum(rho[n] * J[(m[0] - n[0], m[1] - n[1], m[2] - n[2])]
+                     for n in np.ndindex(grid))
+    ref *= FPEI * gamma / vol
+    # Float32: the 8-point difference of P (which grows like r^2) cancels
+    # about log2(r^2/h^2) bits at the far lattice nodes
+    assert rel_l2(phi, ref) < (1e-9 if dt == np.float64 else 2e-5)
+    ext = orc.solve_potential_extended(rho, h, gamma)
+    assert rel_l2(ext, ref) < 1e-9
+
+
+def test_potential_analytical_isotropic_gaussian():
+    """phi(r) = Q erf(r / (sqrt(2) sigma)) / (4 pi eps0 r) for a spherical
+    Gaussian of charge Q: 1M particles, sigma = 1 mm, 1 nC, 32^3 auto-bounds,
+    within 2% of the closed form along the z axis (the mesh is cut at the
+    bunch edge, which truncates the far charge)."""
+    eps0 = 8.8541878128e-12
+    n, Q, sigma = 1_000_000, 1e-9, 1e-3
+    x, y, z = gaussian(n, (sigma, sigma, sigma), seed=123)
+    q = np.full(n, Q / n)
+    grid = (32, 32, 32)
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid)
+    rho = orc.deposit(x, y, z, q, grid, lo, dl)
+    phi = orc.solve_potential(rho, dl)
+    zc = np.array([lo[2] + k * dl[2] for k in range(grid[2])])
+    ix = int(np.argmin(np.abs([lo[0] + i * dl[0] for i in range(grid[0])])))
+    iy = int(np.argmin(np.abs([lo[1] + j * dl[1] for j in range(grid[1])])))
+    xc, yc = lo[0] + ix * dl[0], lo[1] + iy * dl[1]
+    r = np.sqrt(xc ** 2 + yc ** 2 + zc ** 2)
+    ana = Q / (4 * np.pi * eps0) * erf(r / (np.sqrt(2) * sigma)) / r
+    err = np.abs(phi[ix, iy, :] - ana).max() / np.abs(ana).max()
+    assert err < 0.02, err
+
+
+def test_potential_gradient_is_field():
+    """-grad(phi) by central differences equals the field solve (gamma = 1) in
+    the interior, to the O(h^2) of the difference."""
+    x, y, z, q = config_a(200_000, seed=4)
+    grid = (32, 32, 32)
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid)
+    rho = orc.deposit(x, y, z, q, grid, lo, dl)
+    phi = orc.solve_potential(rho, dl)
+    ef = orc.solve(rho, lo, hi, dl)
+    s = slice(4, -4)
+    for ax in range(3):
+        g = -(np.roll(phi, -1, ax) - np.roll(phi, 1, ax)) / (2 * dl[ax])
+        e = ef[..., ax]
+        err = rel_l2(g[s, s, s], e[s, s, s])
+        assert err < 0.015, (ax, err)
