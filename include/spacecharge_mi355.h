@@ -140,6 +140,18 @@ int sc_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y, const vo
 int sc_solve(sc_mesh *mesh, const double min_bounds[3], const double max_bounds[3],
              const double delta[3], double gamma, int at_cathode);
 
+/* Scalar potential in free space.  No reference solver computes it; its
+ * Green function is potential_green_function (src/green_functions.jl:13-22),
+ * the antiderivative of 1/r, used exactly as sc_solve uses the field's:
+ * phi = FPEI * gamma/(dx dy dz gamma) * (rho (*) cell integral of 1/r) on
+ * the same zero-padded linear convolution.  This is the lab-frame potential
+ * of a bunch moving along z (gamma times the rest-frame one), so that
+ * -dphi/dx and -dphi/dy approximate sc_solve's Ex and Ey, and
+ * -dphi/dz / gamma^2 its Ez.  phi: caller-owned device array of nx*ny*nz
+ * elements of the mesh type, in rho's order.  Uses (and may rebuild) the
+ * mesh's solver workspace. */
+int sc_solve_potential(sc_mesh *mesh, const double delta[3], double gamma, void *phi);
+
 /* Replaces interpolate_field (src/interpolation.jl:100-128) and
  * interpolate_kernel! (:17-86): Ex/Ey/Ez[p] = sum over the 8 surrounding
  * nodes of the trilinear weight times efield, written in the particle type

@@ -431,6 +431,20 @@ int sc_solve(sc_mesh *mesh, const double lo[3], const double hi[3], const double
     return solve_mesh(&M, lo, hi, dl, gamma, at_cathode);
 }
 
+int sc_solve_potential(sc_mesh *mesh, const double dl[3], double gamma, void *phi)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    static const double origin[3] = {0.0, 0.0, 0.0};  // free space: no bounds enter
+    if (int rc = check_geom(c, origin, dl)) return rc;
+    if (!std::isfinite(gamma) || !(gamma > 0.0))
+        return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
+    if (!phi) return set_err(c, SC_ERR_INVALID, "NULL phi");
+    if (int rc = check_dev(c)) return rc;
+    return solve_potential_mesh(&M, origin, origin, dl, gamma, phi);
+}
+
 int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
                    int64_t n, const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez)
 {

@@ -25,6 +25,10 @@
 // transform: the spectrum is i*R_c with R_c real on (nx+1)(ny+1)(nz+1)
 // points.  The image function (offset along z) keeps the x, y symmetry and
 // has a complex z spectrum.
+//
+// The scalar potential (GreenArgs::pot) runs the same passes on one
+// component, the antiderivative of 1/r (src/green_functions.jl:13-22), whose
+// cell integrals are even in every axis: its spectrum is real.
 #include <hip/hip_runtime.h>
 
 #include "sc_fft_reg.h"
@@ -44,19 +48,37 @@ __device__ __forceinline__ double field_green(double x, double y, double z)
     return (t1 - t2) + t3;
 }
 
-// Component permutation of get_green_kernel! (src/green_functions.jl:90-98).
+// src/green_functions.jl:13-22, in Float64 (zero at the origin, which is
+// never a lattice node: node coordinates are odd multiples of half a cell).
+__device__ __forceinline__ double potential_green(double x, double y, double z)
+{
+    const double r = sqrt((x * x + y * y) + z * z);
+    if (r == 0.0) return 0.0;
+    double t = -0.5 * (z * z) * atan((x * y) / (z * r));
+    t = t - 0.5 * (y * y) * atan((x * z) / (y * r));
+    t = t - 0.5 * (x * x) * atan((y * z) / (x * r));
+    t = t + (y * z) * log(x + r);
+    t = t + (x * z) * log(y + r);
+    t = t + (x * y) * log(z + r);
+    return t;
+}
+
+// Component permutation of get_green_kernel! (src/green_functions.jl:90-98);
+// c == 3 is the scalar potential.
 __device__ __forceinline__ double field_green_comp(int c, double u, double v, double w)
 {
     if (c == 0) return field_green(u, v, w);
     if (c == 1) return field_green(v, w, u);
-    return field_green(w, u, v);
+    if (c == 2) return field_green(w, u, v);
+    return potential_green(u, v, w);
 }
 
 struct GreenArgs {
     Dims d;
     double dx, dy, dz;  // dz = gamma * delta_z (green_functions.jl:73)
     double offzg;       // image z offset times gamma (:84)
-    double scale[3];    // FPEI * factor_c / (8 nx ny nz)
+    double scale[4];    // FPEI * factor_c / (8 nx ny nz); [3] for the potential
+    int pot;            // 1: one component, the scalar potential (c = 3)
     int64_t f_free, f_img;  // F elements per component
     int64_t r_free, r_img;  // R elements per component
 };
@@ -83,11 +105,15 @@ __device__ __forceinline__ void unpack_pair(cpx<T> X, bool odd, T &ra, T &rb)
     else { ra = X.x; rb = X.y; }
 }
 
+// The function computed by component set q: field component q % 3 (image
+// set q / 3), or the potential (c = 3, odd in no axis).
+__device__ __forceinline__ int comp_of(const GreenArgs &a, int q) { return a.pot ? 3 : q % 3; }
+
 // grid (x-y plane blocks, z node planes, components); in-plane index math
 // in 32 bits (this kernel is bound by its Float64 instruction count)
 __global__ void __launch_bounds__(256) k_green_f(double *__restrict__ F, GreenArgs a)
 {
-    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int nx1 = a.d.nx + 1, ny1 = a.d.ny + 1;
     const int kn = blockIdx.y;
     if (kn >= z_nodes(a, image)) return;
@@ -143,7 +169,7 @@ k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restr
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + a.d.Lx;
-    const int q = blockIdx.y, c = q % 3, image = q / 3;
+    const int q = blockIdx.y, c = comp_of(a, q), image = q / 3;
     const int nx = a.d.nx, ny = a.d.ny, Lx = a.d.Lx, Kx = a.d.Kx, Kp = a.d.Kp;
     const int bpairs = (ny + 1) / 2;
     const int npencils = z_cells(a, image) * bpairs;
@@ -195,7 +221,7 @@ k_green_2(T *__restrict__ R, const cpx<T> *__restrict__ twg, FftPlan pl, GreenAr
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + a.d.Ly;
-    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int k = blockIdx.y;
     if (k >= z_cells(a, image)) return;  // uniform over the workgroup
     const int ny = a.d.ny, Ly = a.d.Ly, Kx = a.d.Kx, Kp = a.d.Kp;
@@ -240,7 +266,7 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + a.d.Lz;
-    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int ky = blockIdx.y;
     const int nz = a.d.nz, Lz = a.d.Lz, Kx = a.d.Kx, Kp = a.d.Kp, ny1 = a.d.ny + 1;
     const T *Rq = R + r_off(a, q);
@@ -281,7 +307,7 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
     const T scale = (T)a.scale[c];
     if (!image) {
-        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kp;
+        T *G = Gf + (size_t)(q % 3) * (nz + 1) * ny1 * Kp;
         for (int idx = threadIdx.x; idx < nb * (nz + 1); idx += blockDim.x) {
             const int kz = idx / nb, p = idx - kz * nb;
             const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
@@ -294,7 +320,7 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     } else {
         // separate the two real sequences: A_k = (X_k + conj X_{-k})/2,
         // B_k = (X_k - conj X_{-k})/(2i)
-        cpx<T> *G = Gi + (size_t)c * Lz * ny1 * Kp;
+        cpx<T> *G = Gi + (size_t)(q % 3) * Lz * ny1 * Kp;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
             const int kz = idx / nb, p = idx - kz * nb;
             const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
@@ -328,7 +354,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
-    const int q = blockIdx.y, c = q % 3, image = q / 3;
+    const int q = blockIdx.y, c = comp_of(a, q), image = q / 3;
     const int nx = a.d.nx, ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int bpairs = (ny + 1) / 2;
     const int npencils = z_cells(a, image) * bpairs;
@@ -409,7 +435,7 @@ k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
-    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int k = blockIdx.y;
     if (k >= z_cells(a, image)) return;  // uniform over the workgroup
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
@@ -461,7 +487,7 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
-    const int q = blockIdx.z, c = q % 3, image = q / 3;
+    const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int ky = blockIdx.y;
     const int nz = a.d.nz, Kx = a.d.Kx, Kp = a.d.Kp, ny1 = a.d.ny + 1;
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
@@ -501,7 +527,7 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
     const T scale = (T)a.scale[c];
     if (!image) {
-        T *G = Gf + (size_t)c * (nz + 1) * ny1 * Kp;
+        T *G = Gf + (size_t)(q % 3) * (nz + 1) * ny1 * Kp;
 #pragma unroll
         for (int r = 0; r <= E / 2; ++r) {
             const int kz = j + r * TPP;
@@ -519,7 +545,7 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
 #pragma unroll
         for (int r = 0; r < E; ++r) lds[b + (j + r * TPP) * NB] = v[r];
         __syncthreads();
-        cpx<T> *G = Gi + (size_t)c * L * ny1 * Kp;
+        cpx<T> *G = Gi + (size_t)(q % 3) * L * ny1 * Kp;
         const T h = (T)0.5 * scale;
 #pragma unroll
         for (int r = 0; r < E; ++r) {
@@ -578,8 +604,9 @@ bool green_pass_fast(int pass, int L, int ncomp, const Dims &d, int cath, hipStr
 }  // namespace
 
 template <typename T>
-int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cath)
+int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode)
 {
+    const int cath = mode == kFieldCathode;
     Ctx *ctx = m->ctx;
     Workspace &ws = m->ws;
     hipStream_t st = ctx->stream;
@@ -593,11 +620,14 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cath
     const double vol = (a.dx * a.dy) * a.dz;
     const double P = (double)d.Lx * (double)d.Ly * (double)d.Lz;
     for (int c = 0; c < 3; ++c) a.scale[c] = kFPEI * ((c < 2 ? gamma : 1.0) / vol) / P;
+    // the lab-frame potential gamma * phi_rest takes the transverse factor
+    a.scale[3] = kFPEI * (gamma / vol) / P;
+    a.pot = mode == kPotential;
     a.f_free = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     a.f_img = (int64_t)(d.nx + 1) * (d.ny + 1) * d.Lz;
     a.r_free = (int64_t)d.nz * (d.ny + 1) * d.Kp;
     a.r_img = (int64_t)(d.Lz - 1) * (d.ny + 1) * d.Kp;
-    const int ncomp = cath ? 6 : 3;
+    const int ncomp = a.pot ? 1 : (cath ? 6 : 3);
     const cpx<T> *twx = static_cast<const cpx<T> *>(ws.twT[0]);
     const cpx<T> *twy = static_cast<const cpx<T> *>(ws.twT[1]);
     const cpx<T> *twz = static_cast<const cpx<T> *>(ws.twT[2]);

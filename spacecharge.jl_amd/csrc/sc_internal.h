@@ -138,7 +138,7 @@ struct Workspace {
     // cached Green key for the opt-in reuse
     bool g_valid = false;
     double g_key[6] = {};
-    int g_cath = 0;
+    int g_cath = 0;  // solve mode the cached spectra were built for
 };
 
 // Scratch of the tile-binned deposit, grown on demand.
@@ -196,6 +196,9 @@ int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z
                   double mn[3], double mx[3]);
 int solve_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
                int at_cathode);
+// Scalar potential (free space) of m->rho into phi (nx*ny*nz of the mesh type).
+int solve_potential_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3],
+                         double gamma, void *phi);
 int ensure_workspace(Mesh *m);
 int ensure_side_stream(Mesh *m);
 void free_workspace(Mesh *m);

@@ -97,13 +97,15 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
 
 // Fused z convolution.  Per (kx, ky): forward z transform of rho (and of the
 // image charge -rho[.., nz-1-z]), products with the three Green spectra,
-// backward z transforms, z < nz of each field spectrum to B.
+// backward z transforms, z < nz of each field spectrum to B.  The potential
+// (mode kPotential) has one Green spectrum, real and even in every axis.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
 k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
         const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb,
-        int cathode)
+        int mode)
 {
+    const bool cathode = mode == kFieldCathode, pot = mode == kPotential;
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *P0 = tw + d.Lz;
@@ -135,8 +137,9 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
     const int kyp = kyneg ? Ly - ky : ky;
     const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;  // one free-space Green spectrum
     const size_t MI = (size_t)Lz * ny1 * Kp;        // one image Green spectrum
-    for (int c = 0; c < 3; ++c) {
-        const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
+    const int ncomp = pot ? 1 : 3;
+    for (int c = 0; c < ncomp; ++c) {
+        const T sy = (!pot && c == 1 && kyneg) ? (T)-1 : (T)1;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
             const int kz = idx / nb, b = idx - kz * nb;
             const int kx = kx0 + b;
@@ -144,11 +147,11 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
             if (kx < Kx) {
                 const bool kzneg = kz > nz;
                 const int kzp = kzneg ? Lz - kz : kz;
-                const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
+                const T sz = (!pot && c == 2 && kzneg) ? (T)-1 : (T)1;
                 const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kx];
                 const cpx<T> p = P0[idx];
-                // rho_hat * (i g)
-                out = cpx<T>{-p.y * g, p.x * g};
+                // rho_hat * (i g), or rho_hat * g for the potential
+                out = pot ? cpx<T>{p.x * g, p.y * g} : cpx<T>{-p.y * g, p.x * g};
                 if (cathode) {
                     cpx<T> q = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kx];
                     q.x *= sy;
@@ -256,8 +259,10 @@ int upload_twiddles(Ctx *ctx, int n, void **dst)
 
 template <typename T>
 int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
-              int cath)
+              int mode, void *out)
 {
+    const int cath = mode == kFieldCathode;
+    const int ncomp = mode == kPotential ? 1 : 3;
     Ctx *ctx = m->ctx;
     Workspace &ws = m->ws;
     hipStream_t st = ctx->stream;
@@ -268,8 +273,8 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
 
     // ---- Green spectra ----
     const double offz = lo[2] + hi[2];  // 2*zmin + (zmax - zmin), free_space.jl:39
-    const double key[6] = {dl[0], dl[1], dl[2], gamma, cath ? offz : 0.0, 0.0};
-    const bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == cath &&
+    const double key[6] = {dl[0], dl[1], dl[2], gamma, cath ? offz : 0.0, (double)mode};
+    const bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == mode &&
                        std::memcmp(key, ws.g_key, sizeof(key)) == 0;
     // The Green spectra depend on the spacing, gamma and the cathode offset
     // only, not on rho: built on the context's side stream they overlap the
@@ -285,22 +290,22 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
             if (ws.deposit_start_recorded)
                 SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.deposit_start, 0));
             ctx->stream = ctx->side;  // the build launches (and times) on ctx->stream
-            const int rc = build_green<T>(m, dl, gamma, offz, cath);
+            const int rc = build_green<T>(m, dl, gamma, offz, mode);
             ctx->stream = st;
             if (rc) return rc;
             SC_HIP(ctx, hipEventRecord(ws.green_done, ctx->side));
         } else {
-            if (int rc = build_green<T>(m, dl, gamma, offz, cath)) return rc;
+            if (int rc = build_green<T>(m, dl, gamma, offz, mode)) return rc;
         }
         ws.g_valid = true;
-        ws.g_cath = cath;
+        ws.g_cath = mode;
         std::memcpy(ws.g_key, key, sizeof(key));
     }
 
     // Each transform kernel runs on the register-resident engine when its
     // length is a supported power of two, else on the generic LDS engine.
     const bool fast_ok = !m->generic_transforms;
-    auto fast = [&](int which) { return fast_ok && launch_fast<T>(m, which, cath); };
+    auto fast = [&](int which) { return fast_ok && launch_fast<T>(m, which, mode, out); };
 
     // ---- rho forward x, y ----
     cpx<T> *A = static_cast<cpx<T> *>(ws.A);
@@ -331,7 +336,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         launch_begin(ctx, K_ZCONV);
         hipLaunchKernelGGL(k_zconv<T>, dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh, st,
                            (const cpx<T> *)A, B, static_cast<const T *>(ws.Gf),
-                           static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, cath);
+                           static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, mode);
         launch_end(ctx, K_ZCONV);
     }
     if (ws.green_done) {  // the side stream exists: the next build waits for this point
@@ -343,7 +348,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
         launch_begin(ctx, K_INV_Y);
-        hipLaunchKernelGGL(k_inv_y<T>, dim3(div_up(d.Kx, nb), d.nz, 3), dim3(kFftThreads), sh, st, B,
+        hipLaunchKernelGGL(k_inv_y<T>, dim3(div_up(d.Kx, nb), d.nz, ncomp), dim3(kFftThreads), sh, st, B,
                            twy, ws.plan[1], d, nb);
         launch_end(ctx, K_INV_Y);
     }
@@ -351,8 +356,8 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
         launch_begin(ctx, K_INV_X);
-        hipLaunchKernelGGL(k_inv_x<T>, dim3(div_up(d.ny * d.nz, nb), 3), dim3(kFftThreads), sh, st,
-                           (const cpx<T> *)B, static_cast<T *>(m->efield), twx, ws.plan[0], d, nb);
+        hipLaunchKernelGGL(k_inv_x<T>, dim3(div_up(d.ny * d.nz, nb), ncomp), dim3(kFftThreads), sh,
+                           st, (const cpx<T> *)B, static_cast<T *>(out), twx, ws.plan[0], d, nb);
         launch_end(ctx, K_INV_X);
     }
     SC_HIP(ctx, hipGetLastError());
@@ -471,8 +476,18 @@ int solve_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[
 {
     int rc = ensure_workspace(m);
     if (rc != SC_OK) return rc;
-    if (m->dtype == SC_F32) return run_solve<float>(m, lo, hi, dl, gamma, at_cathode ? 1 : 0);
-    return run_solve<double>(m, lo, hi, dl, gamma, at_cathode ? 1 : 0);
+    const int mode = at_cathode ? kFieldCathode : kFieldFree;
+    if (m->dtype == SC_F32) return run_solve<float>(m, lo, hi, dl, gamma, mode, m->efield);
+    return run_solve<double>(m, lo, hi, dl, gamma, mode, m->efield);
+}
+
+int solve_potential_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3],
+                         double gamma, void *phi)
+{
+    int rc = ensure_workspace(m);
+    if (rc != SC_OK) return rc;
+    if (m->dtype == SC_F32) return run_solve<float>(m, lo, hi, dl, gamma, kPotential, phi);
+    return run_solve<double>(m, lo, hi, dl, gamma, kPotential, phi);
 }
 
 }  // namespace sc

@@ -44,15 +44,21 @@ inline int pick_nb(int n, int cap, int nbmax)
     return nb;
 }
 
-// Builds the transformed Green functions of one solve into ws.Gf (and
-// ws.Gi when cathode != 0).
-template <typename T>
-int build_green(Mesh *m, const double dl[3], double gamma, double offz, int cathode);
+// What one solve computes: the three field components in free space or with
+// the cathode image charge, or the scalar potential in free space.
+enum SolveMode { kFieldFree = 0, kFieldCathode = 1, kPotential = 2 };
 
-// Field-solve kernel `which` (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
+// Builds the transformed Green functions of one solve into ws.Gf (and
+// ws.Gi for kFieldCathode): three field components, or the potential's one
+// (stored in the first component's slot).
+template <typename T>
+int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode);
+
+// Solve kernel `which` (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
 // 4 inv_x) on the register-resident engine; false when its transform length
 // is not one of the supported powers of two (nothing is launched then).
+// inv_x writes the solve's components to `out` (ncomp * nx*ny*nz elements).
 template <typename T>
-bool launch_fast(Mesh *m, int which, int cathode);
+bool launch_fast(Mesh *m, int which, int mode, void *out);
 
 }  // namespace sc

@@ -97,15 +97,17 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
     }
 }
 
-// Fused z convolution (see k_zconv in sc_solve.hip for the algebra).
-// Three waves per SIMD: the forward pencil stays in registers across the
-// three inverse transforms, and unconstrained the kernel takes 169 VGPRs
-// (two waves); 168 fit without spilling.
-template <typename T, int L, bool CATH, int THREADS>
+// Fused z convolution (see k_zconv in sc_solve.hip for the algebra), MODE
+// a SolveMode.  Three waves per SIMD: the forward pencil stays in registers
+// across the three inverse transforms, and unconstrained the kernel takes
+// 169 VGPRs (two waves); 168 fit without spilling.
+template <typename T, int L, int MODE, int THREADS>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
           const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d)
 {
+    constexpr bool CATH = MODE == kFieldCathode, POT = MODE == kPotential;
+    constexpr int NCOMP = POT ? 1 : 3;
     using S = Shape<T, L, THREADS>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
@@ -139,17 +141,20 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
     const size_t MI = (size_t)L * ny1 * Kp;
     const int kxc = live ? kx : 0;  // keep dead lanes' loads in bounds
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
+    for (int c = 0; c < NCOMP; ++c) {
+        const T sy = (!POT && c == 1 && kyneg) ? (T)-1 : (T)1;
         cpx<T> w[E];
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             const int kz = j + q * TPP;
             const bool kzneg = kz > nz;
             const int kzp = kzneg ? L - kz : kz;
-            const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
+            const T sz = (!POT && c == 2 && kzneg) ? (T)-1 : (T)1;
             const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kxc];
-            w[q] = cpx<T>{-p[q].y * g, p[q].x * g};  // rho_hat * (i g)
+            if constexpr (POT)
+                w[q] = cpx<T>{p[q].x * g, p[q].y * g};  // rho_hat * g
+            else
+                w[q] = cpx<T>{-p[q].y * g, p[q].x * g};  // rho_hat * (i g)
             if constexpr (CATH) {
                 cpx<T> gi = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kxc];
                 gi.x *= sy;
@@ -232,8 +237,9 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
 }
 
 template <typename T, int L>
-void launch_all(Mesh *m, const Dims &d, int axis_kernel, int cath)
+void launch_all(Mesh *m, const Dims &d, int axis_kernel, int mode, void *out)
 {
+    const int ncomp = mode == kPotential ? 1 : 3;
     Workspace &ws = m->ws;
     Ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
@@ -261,27 +267,30 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, int cath)
         using SZ = Shape<T, L, ZT>;
         launch_begin(ctx, K_ZCONV);
         const dim3 zgrid(div_up(d.Kx, SZ::NB), d.Ly);
-        if (cath)
-            hipLaunchKernelGGL((k_zconv_r<T, L, true, ZT>), zgrid, dim3(ZT), 0, st,
-                               (const cpx<T> *)A, B, static_cast<const T *>(ws.Gf),
-                               static_cast<const cpx<T> *>(ws.Gi), tw, d);
+        const T *Gf = static_cast<const T *>(ws.Gf);
+        const cpx<T> *Gi = static_cast<const cpx<T> *>(ws.Gi);
+        if (mode == kFieldCathode)
+            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldCathode, ZT>), zgrid, dim3(ZT), 0, st,
+                               (const cpx<T> *)A, B, Gf, Gi, tw, d);
+        else if (mode == kPotential)
+            hipLaunchKernelGGL((k_zconv_r<T, L, kPotential, ZT>), zgrid, dim3(ZT), 0, st,
+                               (const cpx<T> *)A, B, Gf, Gi, tw, d);
         else
-            hipLaunchKernelGGL((k_zconv_r<T, L, false, ZT>), zgrid, dim3(ZT), 0, st,
-                               (const cpx<T> *)A, B, static_cast<const T *>(ws.Gf),
-                               static_cast<const cpx<T> *>(ws.Gi), tw, d);
+            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldFree, ZT>), zgrid, dim3(ZT), 0, st,
+                               (const cpx<T> *)A, B, Gf, Gi, tw, d);
         launch_end(ctx, K_ZCONV);
         break;
     }
     case 3:
         launch_begin(ctx, K_INV_Y);
-        hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, 3), dim3(kThreads), 0,
-                           st, B, tw, d);
+        hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, ncomp), dim3(kThreads),
+                           0, st, B, tw, d);
         launch_end(ctx, K_INV_Y);
         break;
     case 4:
         launch_begin(ctx, K_INV_X);
-        hipLaunchKernelGGL((k_inv_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), 3), dim3(kThreads), 0,
-                           st, (const cpx<T> *)B, static_cast<T *>(m->efield), tw, d);
+        hipLaunchKernelGGL((k_inv_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), ncomp), dim3(kThreads),
+                           0, st, (const cpx<T> *)B, static_cast<T *>(out), tw, d);
         launch_end(ctx, K_INV_X);
         break;
     }
@@ -289,25 +298,25 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, int cath)
 
 }  // namespace
 
-// Runs one field-solve kernel (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
+// Runs one solve kernel (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
 // 4 inv_x) with the register engine if its transform length is a supported
 // power of two; returns false (nothing launched) otherwise.
 template <typename T>
-bool launch_fast(Mesh *m, int which, int cath)
+bool launch_fast(Mesh *m, int which, int mode, void *out)
 {
     const Dims d = make_dims(m);
     const int L = (which == 0 || which == 4) ? d.Lx : (which == 2 ? d.Lz : d.Ly);
     switch (L) {
-    case 32: launch_all<T, 32>(m, d, which, cath); return true;
-    case 64: launch_all<T, 64>(m, d, which, cath); return true;
-    case 128: launch_all<T, 128>(m, d, which, cath); return true;
-    case 256: launch_all<T, 256>(m, d, which, cath); return true;
-    case 512: launch_all<T, 512>(m, d, which, cath); return true;
+    case 32: launch_all<T, 32>(m, d, which, mode, out); return true;
+    case 64: launch_all<T, 64>(m, d, which, mode, out); return true;
+    case 128: launch_all<T, 128>(m, d, which, mode, out); return true;
+    case 256: launch_all<T, 256>(m, d, which, mode, out); return true;
+    case 512: launch_all<T, 512>(m, d, which, mode, out); return true;
     default: return false;
     }
 }
 
-template bool launch_fast<float>(Mesh *, int, int);
-template bool launch_fast<double>(Mesh *, int, int);
+template bool launch_fast<float>(Mesh *, int, int, void *);
+template bool launch_fast<double>(Mesh *, int, int, void *);
 
 }  // namespace sc

@@ -16,7 +16,7 @@ module SpaceChargeMI355
 using SpaceCharge
 import SpaceCharge: deposit!, clear_mesh!, solve!, interpolate_field, Mesh3D
 
-export MIArray, mi355_mesh
+export MIArray, mi355_mesh, solve_potential
 
 const libsc = joinpath(@__DIR__, "..", "libspacecharge_mi355.so")
 const SC_F32 = Cint(4)
@@ -177,6 +177,17 @@ function interpolate_field(mesh::MIMesh, x::MIArray{P}, y::MIArray{P}, z::MIArra
                 handle(mesh), dtcode(P), x.ptr, y.ptr, z.ptr, length(x), d3(mesh.min_bounds),
                 d3(mesh.delta), Ex.ptr, Ey.ptr, Ez.ptr), ctx())
     return Ex, Ey, Ez
+end
+
+# Scalar potential in free space (no reference counterpart; its Green function
+# is SpaceCharge.potential_green_function, src/green_functions.jl:13-22):
+# the lab-frame potential, an MIArray of the mesh's grid size and eltype.
+function solve_potential(mesh::MIMesh{T}) where {T}
+    phi = MIArray{T}(undef, mesh.grid_size...)
+    check(ccall((:sc_solve_potential, libsc), Cint,
+                (Ptr{Cvoid}, Ptr{Float64}, Float64, Ptr{Cvoid}),
+                handle(mesh), d3(mesh.delta), Float64(mesh.gamma), phi.ptr), ctx())
+    return phi
 end
 
 end # module

@@ -11,7 +11,9 @@ Mirrors the SpaceCharge.jl API (src/SpaceCharge.jl:17):
     interpolate_field(mesh, x, y, z)          interpolate_field(mesh, x, y, z)
 
 and adds batched forms for many independent bunches (deposit_batched,
-solve_batched, interpolate_batched), which the reference does not have.
+solve_batched, interpolate_batched) and the scalar potential
+(solve_potential, on the reference's potential_green_function), which the
+reference does not have.
 
 Every call forwards to one function of the C ABI (include/spacecharge_mi355.h)
 through ctypes; the arithmetic runs in the hand-written gfx950 kernels of
@@ -40,6 +42,7 @@ __all__ = [
     "deposit",
     "clear_mesh",
     "solve",
+    "solve_potential",
     "interpolate_field",
     "SpaceChargeError",
     "CLIGHT",
@@ -259,6 +262,28 @@ def solve(mesh: Mesh3D, *, at_cathode: bool = False) -> None:
         rc = _lib.load().sc_solve(mesh._h, _lib.d3(mesh.min_bounds), _lib.d3(mesh.max_bounds),
                                   _lib.d3(mesh.delta), float(mesh.gamma), int(bool(at_cathode)))
     _check(rc, ctx)
+
+
+def solve_potential(mesh: Mesh3D, out=None):
+    """Scalar potential of mesh.rho in free space (no reference solver; the
+    Green function is potential_green_function, src/green_functions.jl:13-22):
+    FPEI * gamma/(dx dy dz gamma) * (rho (*) cell integral of 1/r), the
+    lab-frame potential, so -grad_x, -grad_y give Ex, Ey and -grad_z/gamma^2
+    gives Ez.  Returns a (nx, ny, nz) tensor with rho's column-major strides
+    (a view of ``out``, a contiguous tensor of nx*ny*nz elements, if given)."""
+    nx, ny, nz = mesh.grid_size
+    n = nx * ny * nz
+    if out is None:
+        out = torch.empty(n, dtype=mesh.rho.dtype, device=mesh.device)
+    elif (out.dtype != mesh.rho.dtype or out.device != mesh.rho.device or out.numel() != n
+          or not out.is_contiguous()):
+        raise SpaceChargeError("out must be a contiguous tensor of the mesh's dtype, device and size")
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_solve_potential(mesh._h, _lib.d3(mesh.delta), float(mesh.gamma),
+                                            out.data_ptr())
+    _check(rc, ctx)
+    return out.as_strided((nx, ny, nz), (1, nx, nx * ny))
 
 
 def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
