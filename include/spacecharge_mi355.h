@@ -160,6 +160,42 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
                    int64_t n, const double min_bounds[3], const double delta[3], void *Ex,
                    void *Ey, void *Ez);
 
+/* ---- vector-Jacobian products (no reference counterpart: the reference has
+ * no derivatives; these serve differentiable tracking, e.g. a ChainRulesCore
+ * rrule per function) -----------------------------------------------------
+ * Each returns the cotangents of its forward function's inputs given those
+ * of its outputs, with the mesh geometry (min_bounds, delta, gamma, bounds)
+ * held fixed.  The CIC weights are linear in the position inside a cell;
+ * position derivatives hold the cell index fixed, so a particle on a cell
+ * face takes the derivative of the cell it is deposited in.  Position
+ * gradients (g_x, g_y, g_z: all NULL or all non-NULL, particle type) are
+ * overwritten, or added to when accumulate != 0, so that the deposit's and
+ * the interpolation's contributions can be summed in place. */
+
+/* VJP of sc_deposit: g_q[p] = sum_8 w g_rho (g_rho: nx*ny*nz of the mesh
+ * type) and g_pos[p] = q[p] * d/dpos sum_8 w g_rho.  g_q may be NULL. */
+int sc_deposit_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                       const void *q, int64_t n, const double min_bounds[3], const double delta[3],
+                       const void *g_rho, void *g_q, void *g_x, void *g_y, void *g_z,
+                       int accumulate);
+
+/* VJP of sc_solve: g_rho = S^T g_efield, S the field solve's linear map from
+ * rho to efield (with the cathode image when at_cathode != 0).  g_efield:
+ * 3*nx*ny*nz, g_rho: nx*ny*nz, both of the mesh type, caller-owned.  Uses
+ * the mesh's solver workspace (not rho or efield). */
+int sc_solve_adjoint(sc_mesh *mesh, const double min_bounds[3], const double max_bounds[3],
+                     const double delta[3], double gamma, int at_cathode, const void *g_efield,
+                     void *g_rho);
+
+/* VJP of sc_interpolate at the mesh's current efield: g_efield (3*nx*ny*nz
+ * of the mesh type, overwritten; may be NULL) = the deposit of (g_Ex, g_Ey,
+ * g_Ez) with the same weights, and g_pos[p] = sum_c g_Ec[p] * d/dpos
+ * sum_8 w efield_c. */
+int sc_interpolate_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                           int64_t n, const double min_bounds[3], const double delta[3],
+                           const void *g_Ex, const void *g_Ey, const void *g_Ez, void *g_efield,
+                           void *g_x, void *g_y, void *g_z, int accumulate);
+
 /* ---- many bunches per call (no reference counterpart: the reference runs
  * one mesh per call, e.g. benchmark/full_pipeline_benchmark.jl:25-47; these
  * serve the batched configuration of many independent bunches) -------------

@@ -58,6 +58,8 @@ def _load():
     lib.sco_solve_potential_extended.argtypes = [pll, pd, d, vp, vp, i]
     lib.sco_potential_green_function_f64.argtypes = [d, d, d]
     lib.sco_potential_green_function_f64.restype = d
+    lib.sco_solve_transpose.argtypes = [i, pll, pd, pd, pd, d, i, vp, vp, i]
+    lib.sco_solve_transpose_extended.argtypes = [pll, pd, pd, pd, d, i, vp, vp, i]
     _lib = lib
     return lib
 
@@ -227,3 +229,98 @@ def solve_potential_extended(rho, delta, gamma=1.0, nthreads=0):
 def potential_green_function(x, y, z):
     """src/green_functions.jl:13-22 in Float64."""
     return _load().sco_potential_green_function_f64(x, y, z)
+
+
+# ---------------------------------------------------------------- adjoints
+# The reference has no derivatives; these are the vector-Jacobian products of
+# its deposit, solve and interpolation, used to check the library's.
+
+
+def solve_transpose(g_efield, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0):
+    """S^T g for the field solve S (rho -> efield): g_efield is a Fortran
+    (nx, ny, nz, 3) array; returns the (nx, ny, nz) cotangent of rho, in
+    g_efield's dtype (see solve_transpose in sc_oracle_body.h)."""
+    lib = _load()
+    g = np.asfortranarray(g_efield)
+    out = np.zeros(g.shape[:3], dtype=g.dtype, order="F")
+    rc = lib.sco_solve_transpose(_code(g.dtype), _ll3(g.shape[:3]), _d3(lo), _d3(hi), _d3(delta),
+                                 float(gamma), int(bool(at_cathode)), _ptr(g), _ptr(out), nthreads)
+    if rc != 0:
+        raise OracleError(f"solve_transpose failed ({rc})")
+    return out
+
+
+def solve_transpose_extended(g_efield, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0):
+    """solve_transpose in x87 extended precision (Float64 in and out)."""
+    lib = _load()
+    g = np.asfortranarray(g_efield, dtype=np.float64)
+    out = np.zeros(g.shape[:3], dtype=np.float64, order="F")
+    rc = lib.sco_solve_transpose_extended(_ll3(g.shape[:3]), _d3(lo), _d3(hi), _d3(delta),
+                                          float(gamma), int(bool(at_cathode)), _ptr(g), _ptr(out),
+                                          nthreads)
+    if rc != 0:
+        raise OracleError(f"solve_transpose_extended failed ({rc})")
+    return out
+
+
+def _cic_frame(x, y, z, grid, lo, delta, dtype=np.float64):
+    """Per particle: lower cell index (clamped to [0, n-2], src/deposition.jl:
+    43-48 with out-of-mesh particles clamped) and fractions.  The normalised
+    coordinate (p - lo)/delta is formed in ``dtype`` (the wider of the
+    particle and mesh types, as the deposit does), the rest in Float64."""
+    idx, frac = [], []
+    for p, l, d, n in zip((x, y, z), lo, delta, grid):
+        t = ((np.asarray(p, dtype=dtype) - dtype(l)) / dtype(d)).astype(np.float64)
+        i = np.clip(np.floor(t), 0, n - 2)
+        i = np.where(np.isnan(i), 0, i).astype(np.int64)
+        idx.append(i)
+        frac.append(t - i)
+    return idx, frac
+
+
+def _trilinear_grad(field, idx, frac, delta):
+    """Interpolant of a Fortran (nx, ny, nz) node field at the particles and
+    its x, y, z derivatives, cell index held fixed."""
+    (i, j, k), (dx, dy, dz) = idx, frac
+    w = [(1 - dx, dx), (1 - dy, dy), (1 - dz, dz)]
+    dw = [(-1.0 / delta[0], 1.0 / delta[0]), (-1.0 / delta[1], 1.0 / delta[1]),
+          (-1.0 / delta[2], 1.0 / delta[2])]
+    val = np.zeros_like(dx)
+    gx, gy, gz = np.zeros_like(dx), np.zeros_like(dx), np.zeros_like(dx)
+    for a in (0, 1):
+        for b in (0, 1):
+            for c in (0, 1):
+                f = field[i + a, j + b, k + c]
+                val += f * w[0][a] * w[1][b] * w[2][c]
+                gx += f * dw[0][a] * w[1][b] * w[2][c]
+                gy += f * w[0][a] * dw[1][b] * w[2][c]
+                gz += f * w[0][a] * w[1][b] * dw[2][c]
+    return val, gx, gy, gz
+
+
+def deposit_vjp(x, y, z, q, g_rho, lo, delta, frame_dtype=np.float64):
+    """VJP of deposit at cotangent g_rho (Fortran (nx, ny, nz)): (g_q, g_x,
+    g_y, g_z), Float64."""
+    idx, frac = _cic_frame(x, y, z, g_rho.shape, lo, delta, frame_dtype)
+    val, gx, gy, gz = _trilinear_grad(np.asarray(g_rho, dtype=np.float64), idx, frac, delta)
+    q = np.asarray(q, dtype=np.float64)
+    return val, q * gx, q * gy, q * gz
+
+
+def interpolate_vjp(efield, x, y, z, gE, lo, delta, frame_dtype=np.float64):
+    """VJP of interpolate at the field efield (Fortran (nx, ny, nz, 3)) and
+    cotangents gE = (g_Ex, g_Ey, g_Ez): (g_efield, g_x, g_y, g_z), Float64.
+    g_efield is the deposit of each g_Ec (the gather's transpose)."""
+    grid = efield.shape[:3]
+    gef = np.zeros(tuple(grid) + (3,), order="F")
+    for c in range(3):
+        gef[..., c] = deposit(np.asarray(x, np.float64), np.asarray(y, np.float64),
+                              np.asarray(z, np.float64), np.asarray(gE[c], np.float64), grid, lo,
+                              delta)
+    idx, frac = _cic_frame(x, y, z, grid, lo, delta, frame_dtype)
+    gx, gy, gz = 0.0, 0.0, 0.0
+    for c in range(3):
+        _, ax, ay, az = _trilinear_grad(np.asarray(efield[..., c], np.float64), idx, frac, delta)
+        w = np.asarray(gE[c], dtype=np.float64)
+        gx, gy, gz = gx + w * ax, gy + w * ay, gz + w * az
+    return gef, gx, gy, gz

@@ -382,6 +382,45 @@ int sco_solve_potential_extended(const long long grid[3], const double delta[3],
     return rc;
 }
 
+/* Transpose of the field solve (no reference counterpart): rho_g = S^T g,
+ * g of 3M elements, rho_g of M. */
+int sco_solve_transpose(int mdt, const long long grid[3], const double lo[3], const double hi[3],
+                        const double delta[3], double gamma, int at_cathode, const void *g,
+                        void *rho_g, int nthreads)
+{
+    const int gr[3] = {(int)grid[0], (int)grid[1], (int)grid[2]};
+    int nth = sco_threads(nthreads);
+    if (mdt == SCO_F64) return solve_transpose_f64(g, rho_g, gr, lo, hi, delta, gamma, at_cathode, nth);
+    if (mdt == SCO_F32) {
+        const float l[3] = {(float)lo[0], (float)lo[1], (float)lo[2]};
+        const float h[3] = {(float)hi[0], (float)hi[1], (float)hi[2]};
+        const float d[3] = {(float)delta[0], (float)delta[1], (float)delta[2]};
+        return solve_transpose_f32(g, rho_g, gr, l, h, d, (float)gamma, at_cathode, nth);
+    }
+    return -4;
+}
+
+int sco_solve_transpose_extended(const long long grid[3], const double lo[3], const double hi[3],
+                                 const double delta[3], double gamma, int at_cathode,
+                                 const double *g, double *rho_g, int nthreads)
+{
+    const int gr[3] = {(int)grid[0], (int)grid[1], (int)grid[2]};
+    const long long M = grid[0] * grid[1] * grid[2];
+    long double *a = (long double *)malloc(sizeof(long double) * 3 * (size_t)M);
+    long double *r = (long double *)malloc(sizeof(long double) * (size_t)M);
+    if (!a || !r) { free(a); free(r); return -1; }
+    for (long long i = 0; i < 3 * M; ++i) a[i] = g[i];
+    const long double l[3] = {lo[0], lo[1], lo[2]};
+    const long double h[3] = {hi[0], hi[1], hi[2]};
+    const long double d[3] = {delta[0], delta[1], delta[2]};
+    int rc = solve_transpose_f80(a, r, gr, l, h, d, (long double)gamma, at_cathode,
+                                 sco_threads(nthreads));
+    for (long long i = 0; i < M; ++i) rho_g[i] = (double)r[i];
+    free(a);
+    free(r);
+    return rc;
+}
+
 /* ---- internals exposed for the oracle's own tests ---- */
 
 double sco_potential_green_function_f64(double x, double y, double z)

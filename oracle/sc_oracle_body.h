@@ -414,3 +414,56 @@ static int SFX(solve_potential)(const R *rho, R *phi, const int grid[3], const R
     free(temp);
     return 0;
 }
+
+/* Transpose of SFX(solve), rho_g = S^T g for g = (g_x, g_y, g_z) (no solver
+ * in the reference; this is the vector-Jacobian product of solve!).
+ * S_c convolves rho with the cell integral G_c of src/green_functions.jl,
+ * which is odd along axis c and even along the other two, so the transpose
+ * of the free-space part is S_c^T g_c = -(S_c g_c).  The cathode image term
+ * convolves -flip_z(rho) with G_c(r + off), off = (0, 0, zmin + zmax)
+ * (src/solvers/free_space.jl:34-45); G_c(r + off) = -G_c(-r - off), so its
+ * transpose is minus the convolution with the Green function at offset -off,
+ * and -flip_z is its own transpose:
+ *   rho_g = -sum_c S_c g_c + flip_z(sum_c S_c^(-off) g_c). */
+static int SFX(solve_transpose)(const R *g, R *out, const int grid[3], const R lo[3],
+                                const R hi[3], const R delta[3], R gamma, int at_cathode,
+                                int nthreads)
+{
+    const long long nx = grid[0], ny = grid[1], nz = grid[2];
+    const long long M = nx * ny * nz, P = 8 * M;
+    R *crho = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    R *cgrn = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    R *temp = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    R *ef = (R *)malloc(sizeof(R) * (size_t)M);
+    R *acc = (R *)malloc(sizeof(R) * (size_t)M);
+    if (!crho || !cgrn || !temp || !ef || !acc) {
+        free(crho); free(cgrn); free(temp); free(ef); free(acc);
+        return -1;
+    }
+    const R zero3[3] = {0, 0, 0};
+    for (long long e = 0; e < M; ++e) out[e] = (R)0;
+    for (int c = 0; c < 3; ++c) {
+        SFX(solve_freespace)(g + (size_t)c * M, ef, grid, delta, gamma, zero3, crho, cgrn, temp,
+                             nthreads, c + 1, c + 1);
+        for (long long e = 0; e < M; ++e) out[e] = out[e] - ef[e];
+    }
+    if (at_cathode) {
+        const R off[3] = {0, 0, -((R)2 * lo[2] + (hi[2] - lo[2]))};
+        for (long long e = 0; e < M; ++e) acc[e] = (R)0;
+        for (int c = 0; c < 3; ++c) {
+            SFX(solve_freespace)(g + (size_t)c * M, ef, grid, delta, gamma, off, crho, cgrn, temp,
+                                 nthreads, c + 1, c + 1);
+            for (long long e = 0; e < M; ++e) acc[e] = acc[e] + ef[e];
+        }
+        for (long long k = 0; k < nz; ++k)
+            for (long long j = 0; j < ny; ++j)
+                for (long long i = 0; i < nx; ++i)
+                    out[i + nx * (j + ny * k)] += acc[i + nx * (j + ny * (nz - 1 - k))];
+    }
+    free(crho);
+    free(cgrn);
+    free(temp);
+    free(ef);
+    free(acc);
+    return 0;
+}

@@ -20,7 +20,7 @@ const char *const kKernelNames[K_COUNT] = {
     "green_x",      "green_y",     "green_z",     "rho_fwd_x",      "rho_fwd_y",
     "zconv",        "inv_y",       "inv_x",       "bin_count",      "bin_scan",
     "bin_scatter",  "deposit_tiles", "interleave",  "gather_tiles",   "green_f",
-    "bin_rowscan",  "tile_reduce",
+    "bin_rowscan",  "tile_reduce", "deposit_vjp", "interp_vjp",
 };
 
 namespace {
@@ -458,6 +458,73 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
     if (int rc = check_geom(c, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
     return launch_interp(&M, pdtype, x, y, z, n, lo, dl, Ex, Ey, Ez);
+}
+
+// ---- vector-Jacobian products (no reference counterpart) ----
+static int check_pos_outputs(Ctx *c, const void *gx, const void *gy, const void *gz)
+{
+    if ((gx == nullptr) != (gy == nullptr) || (gx == nullptr) != (gz == nullptr))
+        return set_err(c, SC_ERR_INVALID, "g_x, g_y, g_z must be all NULL or all non-NULL");
+    return SC_OK;
+}
+
+int sc_deposit_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                       const void *q, int64_t n, const double lo[3], const double dl[3],
+                       const void *g_rho, void *g_q, void *g_x, void *g_y, void *g_z, int accumulate)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+    if (int rc = check_pos_outputs(c, g_x, g_y, g_z)) return rc;
+    if (n > 0 && (g_q || g_x)) {
+        if (!x || !y || !z) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
+        if (!g_rho) return set_err(c, SC_ERR_INVALID, "NULL g_rho");
+        if (g_x && !q) return set_err(c, SC_ERR_INVALID, "position gradients need q");
+    }
+    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_dev(c)) return rc;
+    return launch_deposit_vjp(&M, pdtype, x, y, z, q, n, lo, dl, g_rho, g_q, g_x, g_y, g_z,
+                              accumulate);
+}
+
+int sc_solve_adjoint(sc_mesh *mesh, const double lo[3], const double hi[3], const double dl[3],
+                     double gamma, int at_cathode, const void *g_efield, void *g_rho)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (!hi) return set_err(c, SC_ERR_INVALID, "NULL max_bounds");
+    if (!std::isfinite(gamma) || !(gamma > 0.0))
+        return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
+    if (at_cathode && !std::isfinite(hi[2]))
+        return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
+    if (!g_efield || !g_rho) return set_err(c, SC_ERR_INVALID, "NULL g_efield or g_rho");
+    if (int rc = check_dev(c)) return rc;
+    return solve_adjoint_mesh(&M, lo, hi, dl, gamma, at_cathode, g_efield, g_rho);
+}
+
+int sc_interpolate_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                           int64_t n, const double lo[3], const double dl[3], const void *g_Ex,
+                           const void *g_Ey, const void *g_Ez, void *g_efield, void *g_x, void *g_y,
+                           void *g_z, int accumulate)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+    if (int rc = check_pos_outputs(c, g_x, g_y, g_z)) return rc;
+    if (n > 0 && (g_efield || g_x)) {
+        if (!x || !y || !z) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
+        if (!g_Ex || !g_Ey || !g_Ez) return set_err(c, SC_ERR_INVALID, "NULL field cotangent");
+    }
+    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_dev(c)) return rc;
+    return launch_interp_vjp(&M, pdtype, x, y, z, n, lo, dl, g_Ex, g_Ey, g_Ez, g_efield, g_x, g_y,
+                             g_z, accumulate);
 }
 
 // ---- many bunches per call ----

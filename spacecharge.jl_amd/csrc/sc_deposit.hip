@@ -581,7 +581,7 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
 
 template <typename PT, typename MT>
 int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, int64_t n,
-           const Geo &g, int clear)
+           const Geo &g, int clear, void *out)
 {
     Ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
@@ -626,7 +626,7 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, 
     launch_end(ctx, K_TILE_REDUCE);
     launch_begin(ctx, K_GATHER_TILES);
     hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
-                       (const double *)B.partials, g, static_cast<MT *>(m->rho), clear ? 0 : 1);
+                       (const double *)B.partials, g, static_cast<MT *>(out), clear ? 0 : 1);
     launch_end(ctx, K_GATHER_TILES);
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
@@ -645,12 +645,13 @@ void free_bins(Mesh *m)
     B = DepositBins();
 }
 
-// Binned deposit; returns SC_ERR_UNSUPPORTED (without launching anything)
-// when the grid has more tiles than the LDS histogram holds.  Writes every
-// node of rho (clear != 0) or adds to it (clear == 0).
+// Binned deposit into `out` (nx*ny*nz of the mesh type); returns
+// SC_ERR_UNSUPPORTED (without launching anything) when the grid has more
+// tiles than the LDS histogram holds.  Writes every node of out
+// (clear != 0) or adds to it (clear == 0).
 int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const void *z,
                           const void *q, int64_t n, const double lo[3], const double dl[3],
-                          int clear)
+                          int clear, void *out)
 {
     Geo g;
     g.nx = (int)m->n[0];
@@ -666,10 +667,11 @@ int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const 
         g.dl[a] = dl[a];
     }
     if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz)) return rc;
-    if (pdt == SC_F32 && m->dtype == SC_F32) return binned<float, float>(m, x, y, z, q, n, g, clear);
-    if (pdt == SC_F64 && m->dtype == SC_F64) return binned<double, double>(m, x, y, z, q, n, g, clear);
-    if (pdt == SC_F64 && m->dtype == SC_F32) return binned<double, float>(m, x, y, z, q, n, g, clear);
-    return binned<float, double>(m, x, y, z, q, n, g, clear);
+    if (pdt == SC_F32 && m->dtype == SC_F32) return binned<float, float>(m, x, y, z, q, n, g, clear, out);
+    if (pdt == SC_F64 && m->dtype == SC_F64)
+        return binned<double, double>(m, x, y, z, q, n, g, clear, out);
+    if (pdt == SC_F64 && m->dtype == SC_F32) return binned<double, float>(m, x, y, z, q, n, g, clear, out);
+    return binned<float, double>(m, x, y, z, q, n, g, clear, out);
 }
 
 }  // namespace sc

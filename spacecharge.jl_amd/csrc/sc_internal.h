@@ -40,6 +40,8 @@ enum KernelId : int {
     K_GREEN_F,
     K_BIN_ROWSCAN,
     K_TILE_REDUCE,
+    K_DEPOSIT_VJP,
+    K_INTERP_VJP,
     K_COUNT
 };
 extern const char *const kKernelNames[K_COUNT];
@@ -186,9 +188,23 @@ void launch_end(Ctx *ctx, int kid);
 int launch_clear(Ctx *ctx, void *p, int64_t bytes);
 int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
                    int64_t n, const double lo[3], const double dl[3], int clear);
+// The deposit of launch_deposit into `out` (nx*ny*nz of the mesh type)
+// instead of m->rho.
+int launch_deposit_to(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
+                      int64_t n, const double lo[3], const double dl[3], int clear, void *out);
 int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const void *z,
                           const void *q, int64_t n, const double lo[3], const double dl[3],
-                          int clear);
+                          int clear, void *out);
+// Vector-Jacobian products of deposit (g_rho -> g_q, g_x, g_y, g_z) and of
+// interpolation (g_E -> g_efield, g_x, g_y, g_z) at the mesh's current
+// efield; NULL outputs are skipped; position gradients are added to the
+// outputs when accumulate != 0.
+int launch_deposit_vjp(Mesh *m, int pdt, const void *x, const void *y, const void *z,
+                       const void *q, int64_t n, const double lo[3], const double dl[3],
+                       const void *g_rho, void *gq, void *gx, void *gy, void *gz, int accumulate);
+int launch_interp_vjp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                      const double lo[3], const double dl[3], const void *gEx, const void *gEy,
+                      const void *gEz, void *g_efield, void *gx, void *gy, void *gz, int accumulate);
 void free_bins(Mesh *m);
 int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
                   const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez);
@@ -199,6 +215,10 @@ int solve_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[
 // Scalar potential (free space) of m->rho into phi (nx*ny*nz of the mesh type).
 int solve_potential_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3],
                          double gamma, void *phi);
+// Transpose of the field solve: g_rho (nx*ny*nz) = S^T g_efield (3 nx*ny*nz),
+// both of the mesh type.
+int solve_adjoint_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3],
+                       double gamma, int at_cathode, const void *g_efield, void *g_rho);
 int ensure_workspace(Mesh *m);
 int ensure_side_stream(Mesh *m);
 void free_workspace(Mesh *m);

@@ -4,6 +4,9 @@
 //  * interpolate: trilinear gather (src/interpolation.jl:17-86).
 //  * bounds: min/max reduction for the particle-extrema mesh constructor
 //    (src/mesh.jl:120-122).
+//  * vector-Jacobian products of deposit and interpolation (no reference
+//    counterpart): gathers of the node cotangent / field with the CIC
+//    weights and their position derivatives.
 //
 // Per-particle arithmetic keeps the reference's evaluation order without
 // FMA contraction, so a particle's eight deposit contributions and its
@@ -131,6 +134,134 @@ k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ p
 #undef SC_ITP
 }
 
+// ---- vector-Jacobian products of the CIC weights ----
+// At a particle with cell (i, j, k) and fractions (dx, dy, dz) the weight of
+// node (i+a, j+b, k+c) is Wa(dx) Wb(dy) Wc(dz), W0(t) = 1 - t, W1(t) = t,
+// and dt/dx = 1/delta_x with the cell index held fixed: the weights are
+// linear in the position inside a cell, and a particle on a cell face takes
+// the derivative of the cell it is assigned to (as its weights do).  For
+// node values f[a + 2b + 4c] this returns the interpolant (in the order of
+// k_interp) and its x, y, z derivatives times the spacings.
+template <typename C>
+__device__ __forceinline__ void trilinear_grad(const C f[8], C dx, C dy, C dz, C &val, C &gx, C &gy,
+                                               C &gz)
+{
+#pragma clang fp contract(off)
+    const C one = (C)1;
+    const C ex = one - dx, ey = one - dy, ez = one - dz;
+    val = ((((((f[0] * ((ex * ey) * ez) + f[1] * ((dx * ey) * ez)) + f[2] * ((ex * dy) * ez)) +
+               f[3] * ((dx * dy) * ez)) +
+              f[4] * ((ex * ey) * dz)) +
+             f[5] * ((dx * ey) * dz)) +
+            f[6] * ((ex * dy) * dz)) +
+          f[7] * ((dx * dy) * dz);
+    gx = (((f[1] - f[0]) * (ey * ez) + (f[3] - f[2]) * (dy * ez)) + (f[5] - f[4]) * (ey * dz)) +
+         (f[7] - f[6]) * (dy * dz);
+    gy = (((f[2] - f[0]) * (ex * ez) + (f[3] - f[1]) * (dx * ez)) + (f[6] - f[4]) * (ex * dz)) +
+         (f[7] - f[5]) * (dx * dz);
+    gz = (((f[4] - f[0]) * (ex * ey) + (f[5] - f[1]) * (dx * ey)) + (f[6] - f[2]) * (ex * dy)) +
+         (f[7] - f[3]) * (dx * dy);
+}
+
+// VJP of deposit at cotangent g (one value per node): gq = sum_8 w g and
+// g_pos = q * d/dpos (sum_8 w g).  Null outputs are skipped.
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(256)
+k_deposit_vjp(const MT *__restrict__ g, const PT *__restrict__ px, const PT *__restrict__ py,
+              const PT *__restrict__ pz, const PT *__restrict__ pq, int64_t n, int nx, int ny, int nz,
+              double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
+              PT *__restrict__ gq, PT *__restrict__ gx, PT *__restrict__ gy, PT *__restrict__ gz,
+              int accumulate)
+{
+#pragma clang fp contract(off)
+    using C = wider_t<PT, MT>;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
+    const C nxn = ((C)px[p] - (C)(MT)lx_) / ddx;
+    const C nyn = ((C)py[p] - (C)(MT)ly_) / ddy;
+    const C nzn = ((C)pz[p] - (C)(MT)lz_) / ddz;
+    const int ix = cell_index(nxn, nx), iy = cell_index(nyn, ny), iz = cell_index(nzn, nz);
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const MT *b = g + (ix + sy * iy + sz * iz);
+    const C f[8] = {(C)b[0],  (C)b[1],      (C)b[sy],      (C)b[sy + 1],
+                    (C)b[sz], (C)b[sz + 1], (C)b[sz + sy], (C)b[sz + sy + 1]};
+    C val, Gx, Gy, Gz;
+    trilinear_grad(f, nxn - (C)ix, nyn - (C)iy, nzn - (C)iz, val, Gx, Gy, Gz);
+    if (gq) gq[p] = (PT)val;
+    if (gx) {
+        const C q = (C)pq[p];
+        const C tx = q * (Gx / ddx), ty = q * (Gy / ddy), tz = q * (Gz / ddz);
+        if (accumulate) {
+            gx[p] = (PT)((C)gx[p] + tx);
+            gy[p] = (PT)((C)gy[p] + ty);
+            gz[p] = (PT)((C)gz[p] + tz);
+        } else {
+            gx[p] = (PT)tx;
+            gy[p] = (PT)ty;
+            gz[p] = (PT)tz;
+        }
+    }
+}
+
+// Position part of the VJP of interpolation at cotangents (gEx, gEy, gEz):
+// g_pos = sum_c gE_c * d/dpos (sum_8 w E_c), from the interleaved field.
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(256)
+k_interp_vjp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
+             const PT *__restrict__ py, const PT *__restrict__ pz, const PT *__restrict__ gEx,
+             const PT *__restrict__ gEy, const PT *__restrict__ gEz, int64_t n, int nx, int ny,
+             int nz, double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
+             PT *__restrict__ gx, PT *__restrict__ gy, PT *__restrict__ gz, int accumulate)
+{
+#pragma clang fp contract(off)
+    using C = wider_t<PT, MT>;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
+    const C nxn = ((C)px[p] - (C)(MT)lx_) / ddx;
+    const C nyn = ((C)py[p] - (C)(MT)ly_) / ddy;
+    const C nzn = ((C)pz[p] - (C)(MT)lz_) / ddz;
+    const int ix = cell_index(nxn, nx), iy = cell_index(nyn, ny), iz = cell_index(nzn, nz);
+    const C dx = nxn - (C)ix, dy = nyn - (C)iy, dz = nzn - (C)iz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const typename V4<MT>::type *e = efi + (ix + sy * iy + sz * iz);
+    const typename V4<MT>::type nd[8] = {e[0],  e[1],      e[sy],      e[sy + 1],
+                                         e[sz], e[sz + 1], e[sz + sy], e[sz + sy + 1]};
+    C f[8], val, Gx, Gy, Gz;
+    C sx = (C)0, sy_ = (C)0, sz_ = (C)0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = (C)nd[k].x;
+    trilinear_grad(f, dx, dy, dz, val, Gx, Gy, Gz);
+    const C wx = (C)gEx[p];
+    sx = wx * Gx;
+    sy_ = wx * Gy;
+    sz_ = wx * Gz;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = (C)nd[k].y;
+    trilinear_grad(f, dx, dy, dz, val, Gx, Gy, Gz);
+    const C wy = (C)gEy[p];
+    sx = sx + wy * Gx;
+    sy_ = sy_ + wy * Gy;
+    sz_ = sz_ + wy * Gz;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) f[k] = (C)nd[k].z;
+    trilinear_grad(f, dx, dy, dz, val, Gx, Gy, Gz);
+    const C wz = (C)gEz[p];
+    sx = (sx + wz * Gx) / ddx;
+    sy_ = (sy_ + wz * Gy) / ddy;
+    sz_ = (sz_ + wz * Gz) / ddz;
+    if (accumulate) {
+        gx[p] = (PT)((C)gx[p] + sx);
+        gy[p] = (PT)((C)gy[p] + sy_);
+        gz[p] = (PT)((C)gz[p] + sz_);
+    } else {
+        gx[p] = (PT)sx;
+        gy[p] = (PT)sy_;
+        gz[p] = (PT)sz_;
+    }
+}
+
 // ---- bounds: grid-stride min/max per block, then one block folds them ----
 constexpr int kRedBlocks = 1024;
 
@@ -214,11 +345,11 @@ int64_t blocks_for(int64_t n, int bs) { return (n + bs - 1) / bs; }
 
 template <typename PT, typename MT>
 void dep(Mesh *m, const void *x, const void *y, const void *z, const void *q, int64_t n,
-         const double lo[3], const double dl[3])
+         const double lo[3], const double dl[3], void *out)
 {
     hipLaunchKernelGGL((k_deposit<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                        m->ctx->stream, (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q, n,
-                       (MT *)m->rho, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1], lo[2],
+                       (MT *)out, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1], lo[2],
                        dl[0], dl[1], dl[2]);
 }
 
@@ -244,6 +375,43 @@ void itp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, const 
 
 }  // namespace
 
+namespace {
+
+template <typename PT, typename MT>
+void dep_vjp(Mesh *m, const void *x, const void *y, const void *z, const void *q, int64_t n,
+             const double lo[3], const double dl[3], const void *g, void *gq, void *gx, void *gy,
+             void *gz, int accumulate)
+{
+    hipLaunchKernelGGL((k_deposit_vjp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+                       m->ctx->stream, (const MT *)g, (const PT *)x, (const PT *)y, (const PT *)z,
+                       (const PT *)q, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1],
+                       lo[2], dl[0], dl[1], dl[2], (PT *)gq, (PT *)gx, (PT *)gy, (PT *)gz, accumulate);
+}
+
+template <typename PT, typename MT>
+void itp_vjp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, const double lo[3],
+             const double dl[3], const void *gEx, const void *gEy, const void *gEz, void *gx,
+             void *gy, void *gz, int accumulate)
+{
+    hipLaunchKernelGGL((k_interp_vjp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+                       m->ctx->stream, static_cast<const typename V4<MT>::type *>(m->efi),
+                       (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)gEx, (const PT *)gEy,
+                       (const PT *)gEz, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1],
+                       lo[2], dl[0], dl[1], dl[2], (PT *)gx, (PT *)gy, (PT *)gz, accumulate);
+}
+
+int ensure_efi(Mesh *m)
+{
+    if (!m->efi) {
+        const int64_t M = m->n[0] * m->n[1] * m->n[2];
+        if (hipMalloc(&m->efi, (size_t)M * 4 * m->dtype) != hipSuccess)
+            return set_err(m->ctx, SC_ERR_NOMEM, "device allocation of the interleaved field failed");
+    }
+    return SC_OK;
+}
+
+}  // namespace
+
 int launch_clear(Ctx *ctx, void *p, int64_t bytes)
 {
     if (bytes <= 0) return SC_OK;
@@ -258,6 +426,29 @@ int launch_clear(Ctx *ctx, void *p, int64_t bytes)
     return SC_OK;
 }
 
+int launch_deposit_to(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
+                      int64_t n, const double lo[3], const double dl[3], int clear, void *out)
+{
+    const int64_t rho_bytes = m->n[0] * m->n[1] * m->n[2] * m->dtype;
+    if (n <= 0) return clear ? launch_clear(m->ctx, out, rho_bytes) : SC_OK;
+    const int rc = launch_deposit_binned(m, pdt, x, y, z, q, n, lo, dl, clear, out);
+    if (rc != SC_ERR_UNSUPPORTED) return rc;
+    // more tiles than the binned path holds: one atomic per node contribution
+    if (clear) {
+        if (int rc2 = launch_clear(m->ctx, out, rho_bytes)) return rc2;
+    }
+    if ((n + 255) / 256 > 0x7fffffffLL)
+        return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
+    launch_begin(m->ctx, K_DEPOSIT);
+    if (pdt == SC_F32 && m->dtype == SC_F32) dep<float, float>(m, x, y, z, q, n, lo, dl, out);
+    else if (pdt == SC_F64 && m->dtype == SC_F64) dep<double, double>(m, x, y, z, q, n, lo, dl, out);
+    else if (pdt == SC_F64 && m->dtype == SC_F32) dep<double, float>(m, x, y, z, q, n, lo, dl, out);
+    else dep<float, double>(m, x, y, z, q, n, lo, dl, out);
+    launch_end(m->ctx, K_DEPOSIT);
+    SC_HIP(m->ctx, hipGetLastError());
+    return SC_OK;
+}
+
 int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
                    int64_t n, const double lo[3], const double dl[3], int clear)
 {
@@ -267,24 +458,7 @@ int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z
         SC_HIP(m->ctx, hipEventRecord(m->ws.deposit_start, m->ctx->stream));
         m->ws.deposit_start_recorded = true;
     }
-    const int64_t rho_bytes = m->n[0] * m->n[1] * m->n[2] * m->dtype;
-    if (n <= 0) return clear ? launch_clear(m->ctx, m->rho, rho_bytes) : SC_OK;
-    const int rc = launch_deposit_binned(m, pdt, x, y, z, q, n, lo, dl, clear);
-    if (rc != SC_ERR_UNSUPPORTED) return rc;
-    // more tiles than the binned path holds: one atomic per node contribution
-    if (clear) {
-        if (int rc2 = launch_clear(m->ctx, m->rho, rho_bytes)) return rc2;
-    }
-    if ((n + 255) / 256 > 0x7fffffffLL)
-        return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
-    launch_begin(m->ctx, K_DEPOSIT);
-    if (pdt == SC_F32 && m->dtype == SC_F32) dep<float, float>(m, x, y, z, q, n, lo, dl);
-    else if (pdt == SC_F64 && m->dtype == SC_F64) dep<double, double>(m, x, y, z, q, n, lo, dl);
-    else if (pdt == SC_F64 && m->dtype == SC_F32) dep<double, float>(m, x, y, z, q, n, lo, dl);
-    else dep<float, double>(m, x, y, z, q, n, lo, dl);
-    launch_end(m->ctx, K_DEPOSIT);
-    SC_HIP(m->ctx, hipGetLastError());
-    return SC_OK;
+    return launch_deposit_to(m, pdt, x, y, z, q, n, lo, dl, clear, m->rho);
 }
 
 int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
@@ -293,11 +467,7 @@ int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z,
     if (n <= 0) return SC_OK;
     if ((n + 255) / 256 > 0x7fffffffLL)
         return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
-    const int64_t M = m->n[0] * m->n[1] * m->n[2];
-    if (!m->efi) {
-        if (hipMalloc(&m->efi, (size_t)M * 4 * m->dtype) != hipSuccess)
-            return set_err(m->ctx, SC_ERR_NOMEM, "device allocation of the interleaved field failed");
-    }
+    if (int rc = ensure_efi(m)) return rc;
     // efield may have been written by the caller since the last solve, so the
     // interleaved copy is refreshed on every call
     launch_begin(m->ctx, K_INTERLEAVE);
@@ -310,6 +480,63 @@ int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z,
     else if (pdt == SC_F64 && m->dtype == SC_F32) itp<double, float>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
     else itp<float, double>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
     launch_end(m->ctx, K_INTERP);
+    SC_HIP(m->ctx, hipGetLastError());
+    return SC_OK;
+}
+
+
+int launch_deposit_vjp(Mesh *m, int pdt, const void *x, const void *y, const void *z,
+                       const void *q, int64_t n, const double lo[3], const double dl[3],
+                       const void *g_rho, void *gq, void *gx, void *gy, void *gz, int accumulate)
+{
+    if (n <= 0 || (!gq && !gx)) return SC_OK;
+    if ((n + 255) / 256 > 0x7fffffffLL)
+        return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
+    launch_begin(m->ctx, K_DEPOSIT_VJP);
+    if (pdt == SC_F32 && m->dtype == SC_F32)
+        dep_vjp<float, float>(m, x, y, z, q, n, lo, dl, g_rho, gq, gx, gy, gz, accumulate);
+    else if (pdt == SC_F64 && m->dtype == SC_F64)
+        dep_vjp<double, double>(m, x, y, z, q, n, lo, dl, g_rho, gq, gx, gy, gz, accumulate);
+    else if (pdt == SC_F64 && m->dtype == SC_F32)
+        dep_vjp<double, float>(m, x, y, z, q, n, lo, dl, g_rho, gq, gx, gy, gz, accumulate);
+    else
+        dep_vjp<float, double>(m, x, y, z, q, n, lo, dl, g_rho, gq, gx, gy, gz, accumulate);
+    launch_end(m->ctx, K_DEPOSIT_VJP);
+    SC_HIP(m->ctx, hipGetLastError());
+    return SC_OK;
+}
+
+int launch_interp_vjp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                      const double lo[3], const double dl[3], const void *gEx, const void *gEy,
+                      const void *gEz, void *g_efield, void *gx, void *gy, void *gz, int accumulate)
+{
+    const int64_t M = m->n[0] * m->n[1] * m->n[2];
+    if (g_efield) {
+        // the transpose of the gather is the deposit of the cotangents
+        const void *gE[3] = {gEx, gEy, gEz};
+        for (int c = 0; c < 3; ++c) {
+            void *out = static_cast<char *>(g_efield) + (size_t)c * M * m->dtype;
+            if (int rc = launch_deposit_to(m, pdt, x, y, z, gE[c], n, lo, dl, 1, out)) return rc;
+        }
+    }
+    if (n <= 0 || !gx) return SC_OK;
+    if ((n + 255) / 256 > 0x7fffffffLL)
+        return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
+    if (int rc = ensure_efi(m)) return rc;
+    launch_begin(m->ctx, K_INTERLEAVE);
+    if (m->dtype == SC_F32) interleave<float>(m);
+    else interleave<double>(m);
+    launch_end(m->ctx, K_INTERLEAVE);
+    launch_begin(m->ctx, K_INTERP_VJP);
+    if (pdt == SC_F32 && m->dtype == SC_F32)
+        itp_vjp<float, float>(m, x, y, z, n, lo, dl, gEx, gEy, gEz, gx, gy, gz, accumulate);
+    else if (pdt == SC_F64 && m->dtype == SC_F64)
+        itp_vjp<double, double>(m, x, y, z, n, lo, dl, gEx, gEy, gEz, gx, gy, gz, accumulate);
+    else if (pdt == SC_F64 && m->dtype == SC_F32)
+        itp_vjp<double, float>(m, x, y, z, n, lo, dl, gEx, gEy, gEz, gx, gy, gz, accumulate);
+    else
+        itp_vjp<float, double>(m, x, y, z, n, lo, dl, gEx, gEy, gEz, gx, gy, gz, accumulate);
+    launch_end(m->ctx, K_INTERP_VJP);
     SC_HIP(m->ctx, hipGetLastError());
     return SC_OK;
 }

@@ -22,6 +22,14 @@
 // Only rows that can be non-zero are read, only outputs that are used are
 // written.  Workspace: 4 spectra of nz*2ny*(nx+1) complex T, plus the Green
 // spectra and their build scratch.
+//
+// The same kernels run the scalar potential (one Green spectrum, real) and
+// the transpose of the field solve, g_rho = S^T g_E (the vector-Jacobian
+// product of solve!): S_c is a real convolution, so S_c^T is the correlation
+// with G_c, i.e. the product with conj(G_c^): the three components of g_E
+// go through the forward x and y passes, the z convolution sums their
+// products into one spectrum, and one inverse y, x pass gives g_rho.  The
+// image term rho_img = -flip_z(rho) transposes to -flip_z of its correlation.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,16 +44,19 @@ namespace sc {
 namespace {
 
 // ---------------------------------------------------------------------------
-// rho x pass: real rows (y, z), zero-padded to 2nx, kx in [0, nx] to A.
+// x pass of the real input: rows (y, z) of component blockIdx.y, zero-padded
+// to 2nx, kx in [0, nx] to that component's spectrum.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
-k_rho_fwd_x(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg,
+k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__restrict__ twg,
             FftPlan pl, Dims d, int nb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + d.Lx;
     const int Lx = d.Lx, nx = d.nx, nrows = d.ny * d.nz;
+    const T *rho = in + (size_t)blockIdx.y * nx * nrows;
+    cpx<T> *A = out + (size_t)blockIdx.y * d.nz * d.Ly * d.Kp;
     load_twiddles(tw, twg, Lx);
     const int r0 = blockIdx.x * nb;
     for (int idx = threadIdx.x; idx < nb * Lx; idx += blockDim.x) {
@@ -67,7 +78,8 @@ k_rho_fwd_x(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *__r
     }
 }
 
-// rho y pass: pencils (kx, z < nz) along y, ny inputs zero-padded to 2ny.
+// y pass: pencils (kx, z < nz) of component blockIdx.z along y, ny inputs
+// zero-padded to 2ny.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
 k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb)
@@ -78,7 +90,7 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
     const int Ly = d.Ly, Kx = d.Kx, Kp = d.Kp;
     load_twiddles(tw, twg, Ly);
     const int kx0 = blockIdx.x * nb, z = blockIdx.y;
-    cpx<T> *Ap = A + (size_t)z * Ly * Kp;
+    cpx<T> *Ap = A + ((size_t)blockIdx.z * d.nz + z) * Ly * Kp;
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
@@ -174,6 +186,82 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
     }
 }
 
+// Transposed z convolution.  Per (kx, ky): forward z transform of each of
+// the three g_E spectra in Gs, product with conj(G_c^) summed over c (and,
+// with the cathode, with conj(Gi_c^) into a second sum), backward z
+// transforms, and for z < nz: out[z] = W[z] - Wi[nz-1-z].
+template <typename T>
+__global__ void __launch_bounds__(kFftThreads)
+k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__restrict__ Gf,
+            const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d,
+            int nb, int cathode)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
+    cpx<T> *Q = tw + d.Lz;
+    cpx<T> *W = Q + nb * d.Lz;
+    cpx<T> *W1 = W + nb * d.Lz;  // only with the cathode
+    const int Lz = d.Lz, nz = d.nz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
+    const int ny1 = ny + 1;
+    load_twiddles(tw, twg, Lz);
+    const int kx0 = blockIdx.x * nb, ky = blockIdx.y;
+    const size_t zs = (size_t)Ly * Kp;
+    const bool kyneg = ky > ny;
+    const int kyp = kyneg ? Ly - ky : ky;
+    const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;
+    const size_t MI = (size_t)Lz * ny1 * Kp;
+    for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
+        W[idx] = cpx<T>{(T)0, (T)0};
+        if (cathode) W1[idx] = cpx<T>{(T)0, (T)0};
+    }
+    for (int c = 0; c < 3; ++c) {
+        const cpx<T> *Gc = Gs + (size_t)c * nz * zs;
+        for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
+            const int e = idx / nb, b = idx - e * nb;
+            const int kx = kx0 + b;
+            cpx<T> v{(T)0, (T)0};
+            if (kx < Kx && e < nz) v = Gc[(size_t)e * zs + (size_t)ky * Kp + kx];
+            Q[idx] = v;
+        }
+        __syncthreads();
+        lds_fft(Q, tw, pl, nb, nb, 1, true, -1);
+        const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
+        for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
+            const int kz = idx / nb, b = idx - kz * nb;
+            const int kx = kx0 + b;
+            if (kx < Kx) {
+                const bool kzneg = kz > nz;
+                const int kzp = kzneg ? Lz - kz : kz;
+                const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
+                const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kx];
+                const cpx<T> p = Q[idx];
+                // p * conj(i g) = p * (-i g)
+                W[idx] = c_add(W[idx], cpx<T>{p.y * g, -p.x * g});
+                if (cathode) {
+                    cpx<T> q = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kx];
+                    q.x *= sy;
+                    q.y *= sy;
+                    if (c < 2) q = cpx<T>{-q.y, q.x};
+                    W1[idx] = c_add(W1[idx], c_mul(p, cpx<T>{q.x, -q.y}));
+                }
+            }
+        }
+        __syncthreads();
+    }
+    lds_fft(W, tw, pl, nb, nb, 1, true, +1);
+    if (cathode) lds_fft(W1, tw, pl, nb, nb, 1, true, +1);
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nb * nz; idx += blockDim.x) {
+        const int z = idx / nb, b = idx - z * nb;
+        const int kx = kx0 + b;
+        if (kx < Kx) {
+            cpx<T> v = W[idx];
+            if (cathode) v = c_sub(v, W1[(nz - 1 - z) * nb + b]);
+            out[(size_t)z * zs + (size_t)ky * Kp + kx] = v;
+        }
+    }
+}
+
 // Backward y pass: pencils (kx, z < nz) of each field spectrum, y < ny kept.
 template <typename T>
 __global__ void __launch_bounds__(kFftThreads)
@@ -259,10 +347,11 @@ int upload_twiddles(Ctx *ctx, int n, void **dst)
 
 template <typename T>
 int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
-              int mode, void *out)
+              const SolveIO &io)
 {
-    const int cath = mode == kFieldCathode;
-    const int ncomp = mode == kPotential ? 1 : 3;
+    const int mode = io.mode;
+    const int gmode = green_mode(mode);
+    const int cath = mode_cathode(mode);
     Ctx *ctx = m->ctx;
     Workspace &ws = m->ws;
     hipStream_t st = ctx->stream;
@@ -273,8 +362,8 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
 
     // ---- Green spectra ----
     const double offz = lo[2] + hi[2];  // 2*zmin + (zmax - zmin), free_space.jl:39
-    const double key[6] = {dl[0], dl[1], dl[2], gamma, cath ? offz : 0.0, (double)mode};
-    const bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == mode &&
+    const double key[6] = {dl[0], dl[1], dl[2], gamma, cath ? offz : 0.0, (double)gmode};
+    const bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == gmode &&
                        std::memcmp(key, ws.g_key, sizeof(key)) == 0;
     // The Green spectra depend on the spacing, gamma and the cathode offset
     // only, not on rho: built on the context's side stream they overlap the
@@ -290,40 +379,40 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
             if (ws.deposit_start_recorded)
                 SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.deposit_start, 0));
             ctx->stream = ctx->side;  // the build launches (and times) on ctx->stream
-            const int rc = build_green<T>(m, dl, gamma, offz, mode);
+            const int rc = build_green<T>(m, dl, gamma, offz, gmode);
             ctx->stream = st;
             if (rc) return rc;
             SC_HIP(ctx, hipEventRecord(ws.green_done, ctx->side));
         } else {
-            if (int rc = build_green<T>(m, dl, gamma, offz, mode)) return rc;
+            if (int rc = build_green<T>(m, dl, gamma, offz, gmode)) return rc;
         }
         ws.g_valid = true;
-        ws.g_cath = mode;
+        ws.g_cath = gmode;
         std::memcpy(ws.g_key, key, sizeof(key));
     }
 
     // Each transform kernel runs on the register-resident engine when its
     // length is a supported power of two, else on the generic LDS engine.
     const bool fast_ok = !m->generic_transforms;
-    auto fast = [&](int which) { return fast_ok && launch_fast<T>(m, which, mode, out); };
+    auto fast = [&](int which) { return fast_ok && launch_fast<T>(m, which, io); };
 
-    // ---- rho forward x, y ----
-    cpx<T> *A = static_cast<cpx<T> *>(ws.A);
-    cpx<T> *B = static_cast<cpx<T> *>(ws.B);
+    // ---- forward x, y ----
+    cpx<T> *F = static_cast<cpx<T> *>(io.fwd);
+    cpx<T> *B = static_cast<cpx<T> *>(io.inv);
     if (!fast(0)) {
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
         launch_begin(ctx, K_RHO_FWD_X);
-        hipLaunchKernelGGL(k_rho_fwd_x<T>, dim3(div_up(d.ny * d.nz, nb)), dim3(kFftThreads), sh, st,
-                           static_cast<const T *>(m->rho), A, twx, ws.plan[0], d, nb);
+        hipLaunchKernelGGL(k_rho_fwd_x<T>, dim3(div_up(d.ny * d.nz, nb), io.nin), dim3(kFftThreads),
+                           sh, st, static_cast<const T *>(io.in), F, twx, ws.plan[0], d, nb);
         launch_end(ctx, K_RHO_FWD_X);
     }
     if (!fast(1)) {
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
         launch_begin(ctx, K_RHO_FWD_Y);
-        hipLaunchKernelGGL(k_rho_fwd_y<T>, dim3(div_up(d.Kx, nb), d.nz), dim3(kFftThreads), sh, st,
-                           A, twy, ws.plan[1], d, nb);
+        hipLaunchKernelGGL(k_rho_fwd_y<T>, dim3(div_up(d.Kx, nb), d.nz, io.nin), dim3(kFftThreads),
+                           sh, st, F, twy, ws.plan[1], d, nb);
         launch_end(ctx, K_RHO_FWD_Y);
     }
     // ---- fused z convolution ----
@@ -334,9 +423,14 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         while (nb > 1 && sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz) > 65536) nb /= 2;
         const size_t sh = sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz);
         launch_begin(ctx, K_ZCONV);
-        hipLaunchKernelGGL(k_zconv<T>, dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh, st,
-                           (const cpx<T> *)A, B, static_cast<const T *>(ws.Gf),
-                           static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, mode);
+        if (mode_adjoint(mode))
+            hipLaunchKernelGGL(k_zconv_adj<T>, dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh, st,
+                               (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
+                               static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, cath);
+        else
+            hipLaunchKernelGGL(k_zconv<T>, dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh, st,
+                               (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
+                               static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, mode);
         launch_end(ctx, K_ZCONV);
     }
     if (ws.green_done) {  // the side stream exists: the next build waits for this point
@@ -348,20 +442,35 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
         launch_begin(ctx, K_INV_Y);
-        hipLaunchKernelGGL(k_inv_y<T>, dim3(div_up(d.Kx, nb), d.nz, ncomp), dim3(kFftThreads), sh, st, B,
-                           twy, ws.plan[1], d, nb);
+        hipLaunchKernelGGL(k_inv_y<T>, dim3(div_up(d.Kx, nb), d.nz, io.nout), dim3(kFftThreads), sh,
+                           st, B, twy, ws.plan[1], d, nb);
         launch_end(ctx, K_INV_Y);
     }
     if (!fast(4)) {
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
         launch_begin(ctx, K_INV_X);
-        hipLaunchKernelGGL(k_inv_x<T>, dim3(div_up(d.ny * d.nz, nb), ncomp), dim3(kFftThreads), sh,
-                           st, (const cpx<T> *)B, static_cast<T *>(out), twx, ws.plan[0], d, nb);
+        hipLaunchKernelGGL(k_inv_x<T>, dim3(div_up(d.ny * d.nz, nb), io.nout), dim3(kFftThreads), sh,
+                           st, (const cpx<T> *)B, static_cast<T *>(io.out), twx, ws.plan[0], d, nb);
         launch_end(ctx, K_INV_X);
     }
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
+}
+
+// Runs the solve described by `io` (mode, components, real input and
+// output); the spectra buffers come from the mesh's workspace: A holds the
+// single-component spectrum and B the three-component ones.
+int run_solve_io(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
+                 SolveIO io)
+{
+    int rc = ensure_workspace(m);
+    if (rc != SC_OK) return rc;
+    const bool adj = mode_adjoint(io.mode);
+    io.fwd = adj ? m->ws.B : m->ws.A;
+    io.inv = adj ? m->ws.A : m->ws.B;
+    if (m->dtype == SC_F32) return run_solve<float>(m, lo, hi, dl, gamma, io);
+    return run_solve<double>(m, lo, hi, dl, gamma, io);
 }
 
 }  // namespace
@@ -474,20 +583,22 @@ int ensure_workspace(Mesh *m)
 int solve_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
                int at_cathode)
 {
-    int rc = ensure_workspace(m);
-    if (rc != SC_OK) return rc;
-    const int mode = at_cathode ? kFieldCathode : kFieldFree;
-    if (m->dtype == SC_F32) return run_solve<float>(m, lo, hi, dl, gamma, mode, m->efield);
-    return run_solve<double>(m, lo, hi, dl, gamma, mode, m->efield);
+    SolveIO io{at_cathode ? kFieldCathode : kFieldFree, 1, 3, m->rho, m->efield, nullptr, nullptr};
+    return run_solve_io(m, lo, hi, dl, gamma, io);
 }
 
 int solve_potential_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3],
                          double gamma, void *phi)
 {
-    int rc = ensure_workspace(m);
-    if (rc != SC_OK) return rc;
-    if (m->dtype == SC_F32) return run_solve<float>(m, lo, hi, dl, gamma, kPotential, phi);
-    return run_solve<double>(m, lo, hi, dl, gamma, kPotential, phi);
+    SolveIO io{kPotential, 1, 1, m->rho, phi, nullptr, nullptr};
+    return run_solve_io(m, lo, hi, dl, gamma, io);
+}
+
+int solve_adjoint_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3],
+                       double gamma, int at_cathode, const void *g_efield, void *g_rho)
+{
+    SolveIO io{at_cathode ? kAdjointCathode : kAdjointFree, 3, 1, g_efield, g_rho, nullptr, nullptr};
+    return run_solve_io(m, lo, hi, dl, gamma, io);
 }
 
 }  // namespace sc

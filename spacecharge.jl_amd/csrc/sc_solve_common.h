@@ -45,20 +45,49 @@ inline int pick_nb(int n, int cap, int nbmax)
 }
 
 // What one solve computes: the three field components in free space or with
-// the cathode image charge, or the scalar potential in free space.
-enum SolveMode { kFieldFree = 0, kFieldCathode = 1, kPotential = 2 };
+// the cathode image charge, the scalar potential in free space, or the
+// transpose of the field solve (free space or cathode): g_rho = S^T g_E.
+enum SolveMode {
+    kFieldFree = 0,
+    kFieldCathode = 1,
+    kPotential = 2,
+    kAdjointFree = 3,
+    kAdjointCathode = 4
+};
 
-// Builds the transformed Green functions of one solve into ws.Gf (and
-// ws.Gi for kFieldCathode): three field components, or the potential's one
-// (stored in the first component's slot).
+inline bool mode_cathode(int mode) { return mode == kFieldCathode || mode == kAdjointCathode; }
+inline bool mode_adjoint(int mode) { return mode == kAdjointFree || mode == kAdjointCathode; }
+// The Green spectra a mode needs (the transpose uses the field's).
+inline int green_mode(int mode)
+{
+    return mode == kPotential ? kPotential : (mode_cathode(mode) ? kFieldCathode : kFieldFree);
+}
+
+// Buffers of one solve: the real input `in` (nin components of nx*ny*nz) is
+// transformed along x and y into the spectra `fwd` (nin of them), the z
+// convolution writes nout spectra to `inv`, and the inverse y and x passes
+// write the real output `out` (nout components).  Spectra are
+// [nz][2ny][Kp] complex each, consecutive.
+struct SolveIO {
+    int mode;
+    int nin, nout;
+    const void *in;
+    void *out;
+    void *fwd;
+    void *inv;
+};
+
+// Builds the transformed Green functions for green_mode `mode` into ws.Gf
+// (and ws.Gi for kFieldCathode): three field components, or the
+// potential's one (stored in the first component's slot).
 template <typename T>
 int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode);
 
-// Solve kernel `which` (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
-// 4 inv_x) on the register-resident engine; false when its transform length
-// is not one of the supported powers of two (nothing is launched then).
-// inv_x writes the solve's components to `out` (ncomp * nx*ny*nz elements).
+// Solve kernel `which` (0 forward x, 1 forward y, 2 z convolution, 3 inverse
+// y, 4 inverse x) of `io` on the register-resident engine; false when its
+// transform length is not one of the supported powers of two (nothing is
+// launched then).
 template <typename T>
-bool launch_fast(Mesh *m, int which, int mode, void *out);
+bool launch_fast(Mesh *m, int which, const SolveIO &io);
 
 }  // namespace sc

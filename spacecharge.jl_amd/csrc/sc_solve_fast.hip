@@ -36,10 +36,11 @@ constexpr int zconv_threads()
     return kThreads;
 }
 
-// rho x pass: real rows (y, z), zero-padded to 2nx, kx in [0, nx] to A.
+// x pass of the real input: rows (y, z) of component blockIdx.y, zero-padded
+// to 2nx, kx in [0, nx] to that component's spectrum.
 template <typename T, int L>
 __global__ void __launch_bounds__(kThreads)
-k_rho_fwd_x_r(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg,
+k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> *__restrict__ twg,
               Dims d)
 {
     using S = Shape<T, L>;
@@ -50,6 +51,8 @@ k_rho_fwd_x_r(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *_
     const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
     const int r = blockIdx.x * NB + b;
     const int nrows = d.ny * d.nz, nx = d.nx;
+    const T *rho = in + (size_t)blockIdx.y * nx * nrows;
+    cpx<T> *A = spec + (size_t)blockIdx.y * d.nz * d.Ly * d.Kp;
     const bool live = r < nrows;
     cpx<T> v[E];
 #pragma unroll
@@ -69,7 +72,8 @@ k_rho_fwd_x_r(const T *__restrict__ rho, cpx<T> *__restrict__ A, const cpx<T> *_
     }
 }
 
-// rho y pass: pencils (kx, z < nz) along y, ny inputs zero-padded to 2ny.
+// y pass: pencils (kx, z < nz) of component blockIdx.z along y, ny inputs
+// zero-padded to 2ny.
 template <typename T, int L>
 __global__ void __launch_bounds__(kThreads)
 k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
@@ -83,7 +87,7 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
     const int kx = blockIdx.x * NB + b, z = blockIdx.y;
     const int Kx = d.Kx, Kp = d.Kp;
     const bool live = kx < Kx;
-    cpx<T> *Ap = A + (size_t)z * d.Ly * Kp + kx;
+    cpx<T> *Ap = A + ((size_t)blockIdx.z * d.nz + z) * d.Ly * Kp + kx;
     cpx<T> v[E];
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -172,6 +176,82 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
     }
 }
 
+// Transposed z convolution (see k_zconv_adj in sc_solve.hip): the three g_E
+// spectra in Gs are transformed one at a time and their products with
+// conj(G_c^) summed in registers; the image sum is reversed in z through
+// LDS before it is subtracted.
+template <typename T, int L, bool CATH, int THREADS>
+__global__ void __launch_bounds__(THREADS)
+k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__restrict__ Gf,
+              const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d)
+{
+    using S = Shape<T, L, THREADS>;
+    constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
+    __shared__ cpx<T> tw[L];
+    __shared__ cpx<T> lds[NB * L];
+    const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
+    for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
+    const int b = threadIdx.x % NB, j = threadIdx.x / NB;
+    const int kx = blockIdx.x * NB + b, ky = blockIdx.y;
+    const bool live = kx < d.Kx;
+    const size_t zs = (size_t)Ly * Kp;
+    const bool kyneg = ky > ny;
+    const int kyp = kyneg ? Ly - ky : ky;
+    const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;
+    const size_t MI = (size_t)L * ny1 * Kp;
+    const int kxc = live ? kx : 0;  // keep dead lanes' loads in bounds
+    cpx<T> w[E], wi[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+        w[q] = cpx<T>{(T)0, (T)0};
+        wi[q] = cpx<T>{(T)0, (T)0};
+    }
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        const cpx<T> *Gp = Gs + (size_t)c * nz * zs + (size_t)ky * Kp + kxc;
+        cpx<T> p[E];
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            p[q] = cpx<T>{(T)0, (T)0};
+            if (q < E / 2 && live) p[q] = Gp[(size_t)(j + q * TPP) * zs];
+        }
+        reg::fft<T, L, E, -1>(p, j, lds + b, NB, tw);
+        const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int kz = j + q * TPP;
+            const bool kzneg = kz > nz;
+            const int kzp = kzneg ? L - kz : kz;
+            const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
+            const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kxc];
+            w[q] = c_add(w[q], cpx<T>{p[q].y * g, -p[q].x * g});  // p * conj(i g)
+            if constexpr (CATH) {
+                cpx<T> gi = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kxc];
+                gi.x *= sy;
+                gi.y *= sy;
+                if (c < 2) gi = cpx<T>{-gi.y, gi.x};
+                wi[q] = c_add(wi[q], c_mul(p[q], cpx<T>{gi.x, -gi.y}));
+            }
+        }
+    }
+    reg::fft<T, L, E, +1>(w, j, lds + b, NB, tw);
+    if constexpr (CATH) {
+        reg::fft<T, L, E, +1>(wi, j, lds + b, NB, tw);
+        // out[z] = w[z] - wi[nz-1-z]: the reversed image sum lives in other lanes
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E; ++q) lds[b + (j + q * TPP) * NB] = wi[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q) w[q] = c_sub(w[q], lds[b + (nz - 1 - (j + q * TPP)) * NB]);
+    }
+    if (live) {
+        cpx<T> *o = out + (size_t)ky * Kp + kx;
+#pragma unroll
+        for (int q = 0; q < E / 2; ++q) o[(size_t)(j + q * TPP) * zs] = w[q];
+    }
+}
+
 // Backward y pass: pencils (kx, z < nz) of each field spectrum, y < ny kept.
 template <typename T, int L>
 __global__ void __launch_bounds__(kThreads)
@@ -237,9 +317,8 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
 }
 
 template <typename T, int L>
-void launch_all(Mesh *m, const Dims &d, int axis_kernel, int mode, void *out)
+void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
 {
-    const int ncomp = mode == kPotential ? 1 : 3;
     Workspace &ws = m->ws;
     Ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
@@ -247,19 +326,19 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, int mode, void *out)
     const cpx<T> *tw = static_cast<const cpx<T> *>(ws.twT[axis_kernel == 0 || axis_kernel == 4 ? 0
                                                            : axis_kernel == 2         ? 2
                                                                                       : 1]);
-    cpx<T> *A = static_cast<cpx<T> *>(ws.A);
-    cpx<T> *B = static_cast<cpx<T> *>(ws.B);
+    cpx<T> *F = static_cast<cpx<T> *>(io.fwd);
+    cpx<T> *B = static_cast<cpx<T> *>(io.inv);
     switch (axis_kernel) {
     case 0:
         launch_begin(ctx, K_RHO_FWD_X);
-        hipLaunchKernelGGL((k_rho_fwd_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB)), dim3(kThreads),
-                           0, st, static_cast<const T *>(m->rho), A, tw, d);
+        hipLaunchKernelGGL((k_rho_fwd_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), io.nin),
+                           dim3(kThreads), 0, st, static_cast<const T *>(io.in), F, tw, d);
         launch_end(ctx, K_RHO_FWD_X);
         break;
     case 1:
         launch_begin(ctx, K_RHO_FWD_Y);
-        hipLaunchKernelGGL((k_rho_fwd_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz), dim3(kThreads),
-                           0, st, A, tw, d);
+        hipLaunchKernelGGL((k_rho_fwd_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, io.nin),
+                           dim3(kThreads), 0, st, F, tw, d);
         launch_end(ctx, K_RHO_FWD_Y);
         break;
     case 2: {
@@ -267,30 +346,44 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, int mode, void *out)
         using SZ = Shape<T, L, ZT>;
         launch_begin(ctx, K_ZCONV);
         const dim3 zgrid(div_up(d.Kx, SZ::NB), d.Ly);
+        const cpx<T> *Fc = F;
         const T *Gf = static_cast<const T *>(ws.Gf);
         const cpx<T> *Gi = static_cast<const cpx<T> *>(ws.Gi);
-        if (mode == kFieldCathode)
-            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldCathode, ZT>), zgrid, dim3(ZT), 0, st,
-                               (const cpx<T> *)A, B, Gf, Gi, tw, d);
-        else if (mode == kPotential)
-            hipLaunchKernelGGL((k_zconv_r<T, L, kPotential, ZT>), zgrid, dim3(ZT), 0, st,
-                               (const cpx<T> *)A, B, Gf, Gi, tw, d);
-        else
-            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldFree, ZT>), zgrid, dim3(ZT), 0, st,
-                               (const cpx<T> *)A, B, Gf, Gi, tw, d);
+        switch (io.mode) {
+        case kFieldCathode:
+            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldCathode, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf,
+                               Gi, tw, d);
+            break;
+        case kPotential:
+            hipLaunchKernelGGL((k_zconv_r<T, L, kPotential, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
+                               tw, d);
+            break;
+        case kAdjointFree:
+            hipLaunchKernelGGL((k_zconv_adj_r<T, L, false, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
+                               tw, d);
+            break;
+        case kAdjointCathode:
+            hipLaunchKernelGGL((k_zconv_adj_r<T, L, true, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
+                               tw, d);
+            break;
+        default:
+            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldFree, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
+                               tw, d);
+            break;
+        }
         launch_end(ctx, K_ZCONV);
         break;
     }
     case 3:
         launch_begin(ctx, K_INV_Y);
-        hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, ncomp), dim3(kThreads),
-                           0, st, B, tw, d);
+        hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, io.nout),
+                           dim3(kThreads), 0, st, B, tw, d);
         launch_end(ctx, K_INV_Y);
         break;
     case 4:
         launch_begin(ctx, K_INV_X);
-        hipLaunchKernelGGL((k_inv_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), ncomp), dim3(kThreads),
-                           0, st, (const cpx<T> *)B, static_cast<T *>(out), tw, d);
+        hipLaunchKernelGGL((k_inv_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), io.nout),
+                           dim3(kThreads), 0, st, (const cpx<T> *)B, static_cast<T *>(io.out), tw, d);
         launch_end(ctx, K_INV_X);
         break;
     }
@@ -302,21 +395,21 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, int mode, void *out)
 // 4 inv_x) with the register engine if its transform length is a supported
 // power of two; returns false (nothing launched) otherwise.
 template <typename T>
-bool launch_fast(Mesh *m, int which, int mode, void *out)
+bool launch_fast(Mesh *m, int which, const SolveIO &io)
 {
     const Dims d = make_dims(m);
     const int L = (which == 0 || which == 4) ? d.Lx : (which == 2 ? d.Lz : d.Ly);
     switch (L) {
-    case 32: launch_all<T, 32>(m, d, which, mode, out); return true;
-    case 64: launch_all<T, 64>(m, d, which, mode, out); return true;
-    case 128: launch_all<T, 128>(m, d, which, mode, out); return true;
-    case 256: launch_all<T, 256>(m, d, which, mode, out); return true;
-    case 512: launch_all<T, 512>(m, d, which, mode, out); return true;
+    case 32: launch_all<T, 32>(m, d, which, io); return true;
+    case 64: launch_all<T, 64>(m, d, which, io); return true;
+    case 128: launch_all<T, 128>(m, d, which, io); return true;
+    case 256: launch_all<T, 256>(m, d, which, io); return true;
+    case 512: launch_all<T, 512>(m, d, which, io); return true;
     default: return false;
     }
 }
 
-template bool launch_fast<float>(Mesh *, int, int, void *);
-template bool launch_fast<double>(Mesh *, int, int, void *);
+template bool launch_fast<float>(Mesh *, int, const SolveIO &);
+template bool launch_fast<double>(Mesh *, int, const SolveIO &);
 
 }  // namespace sc

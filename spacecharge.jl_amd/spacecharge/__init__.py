@@ -44,6 +44,9 @@ __all__ = [
     "solve",
     "solve_potential",
     "interpolate_field",
+    "deposit_adjoint",
+    "solve_adjoint",
+    "interpolate_adjoint",
     "SpaceChargeError",
     "CLIGHT",
     "FPEI",
@@ -301,6 +304,86 @@ def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
                                         Ez.data_ptr())
     _check(rc, ctx)
     return Ex, Ey, Ez
+
+
+# ---- vector-Jacobian products (no reference counterpart) ----
+
+def _fortran_flat(mesh: Mesh3D, t, shape, name):
+    """Flat tensor of the mesh's dtype and device holding a (nx, ny, nz[, 3])
+    array in the reference's column-major order (a view when ``t`` already
+    has those strides)."""
+    t = torch.as_tensor(t, device=mesh.device)
+    if tuple(t.shape) != tuple(shape):
+        raise SpaceChargeError(f"{name} must have shape {tuple(shape)}")
+    t = t.to(mesh.rho.dtype)
+    return t.permute(*reversed(range(t.dim()))).contiguous().view(-1)
+
+
+def _mesh_views(flat, mesh: Mesh3D, ncomp):
+    nx, ny, nz = mesh.grid_size
+    if ncomp == 1:
+        return flat.as_strided((nx, ny, nz), (1, nx, nx * ny))
+    return flat.as_strided((nx, ny, nz, 3), (1, nx, nx * ny, nx * ny * nz))
+
+
+def deposit_adjoint(mesh: Mesh3D, particles_x, particles_y, particles_z, particles_q, g_rho, *,
+                    positions: bool = True):
+    """Vector-Jacobian product of deposit at the cotangent ``g_rho`` (shape
+    (nx, ny, nz)): returns (g_q, g_x, g_y, g_z) in the particle dtype, with
+    g_q = sum_8 w g_rho and g_pos = q * d/dpos sum_8 w g_rho (the mesh
+    geometry held fixed); the position terms are None unless ``positions``."""
+    x, y, z, q = _particles(mesh, (particles_x, particles_y, particles_z, particles_q), "xyzq")
+    g = _fortran_flat(mesh, g_rho, mesh.grid_size, "g_rho")
+    gq = torch.empty_like(x)
+    gx, gy, gz = (torch.empty_like(x) for _ in range(3)) if positions else (None, None, None)
+    ptr = (lambda t: 0 if t is None else t.data_ptr())
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_deposit_adjoint(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
+                                            z.data_ptr(), q.data_ptr(), x.numel(),
+                                            _lib.d3(mesh.min_bounds), _lib.d3(mesh.delta),
+                                            g.data_ptr(), gq.data_ptr(), ptr(gx), ptr(gy), ptr(gz), 0)
+    _check(rc, ctx)
+    return gq, gx, gy, gz
+
+
+def solve_adjoint(mesh: Mesh3D, g_efield, *, at_cathode: bool = False):
+    """Vector-Jacobian product of solve: g_rho = S^T g_efield for the cotangent
+    ``g_efield`` (shape (nx, ny, nz, 3)); returns a (nx, ny, nz) tensor with
+    rho's strides.  Uses the mesh's solver workspace, not rho or efield."""
+    g = _fortran_flat(mesh, g_efield, tuple(mesh.grid_size) + (3,), "g_efield")
+    out = torch.empty(math.prod(mesh.grid_size), dtype=mesh.rho.dtype, device=mesh.device)
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_solve_adjoint(mesh._h, _lib.d3(mesh.min_bounds), _lib.d3(mesh.max_bounds),
+                                          _lib.d3(mesh.delta), float(mesh.gamma),
+                                          int(bool(at_cathode)), g.data_ptr(), out.data_ptr())
+    _check(rc, ctx)
+    return _mesh_views(out, mesh, 1)
+
+
+def interpolate_adjoint(mesh: Mesh3D, particles_x, particles_y, particles_z, g_Ex, g_Ey, g_Ez, *,
+                        positions: bool = True):
+    """Vector-Jacobian product of interpolate_field at the mesh's current
+    efield: returns (g_efield, g_x, g_y, g_z), g_efield a (nx, ny, nz, 3)
+    tensor with efield's strides (the deposit of the cotangents) and the
+    position terms (None unless ``positions``) in the particle dtype."""
+    x, y, z = _particles(mesh, (particles_x, particles_y, particles_z), "xyz")
+    gE = [_as_device(a, mesh.device, x.dtype) for a in (g_Ex, g_Ey, g_Ez)]
+    if not all(t.numel() == x.numel() for t in gE):
+        raise SpaceChargeError("Field cotangents must have one value per particle.")
+    gef = torch.empty(3 * math.prod(mesh.grid_size), dtype=mesh.rho.dtype, device=mesh.device)
+    gx, gy, gz = (torch.empty_like(x) for _ in range(3)) if positions else (None, None, None)
+    ptr = (lambda t: 0 if t is None else t.data_ptr())
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_interpolate_adjoint(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
+                                                z.data_ptr(), x.numel(), _lib.d3(mesh.min_bounds),
+                                                _lib.d3(mesh.delta), gE[0].data_ptr(),
+                                                gE[1].data_ptr(), gE[2].data_ptr(), gef.data_ptr(),
+                                                ptr(gx), ptr(gy), ptr(gz), 0)
+    _check(rc, ctx)
+    return _mesh_views(gef, mesh, 3), gx, gy, gz
 
 
 # ---- many bunches per call (no reference counterpart) ----

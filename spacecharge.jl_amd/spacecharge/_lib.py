@@ -50,6 +50,11 @@ PROTOTYPES = {
     "sc_solve": (_i, [_vp, _pd, _pd, _pd, _d, _i]),
     "sc_solve_potential": (_i, [_vp, _pd, _d, _vp]),
     "sc_interpolate": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pd, _pd, _vp, _vp, _vp]),
+    "sc_deposit_adjoint": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64, _pd, _pd, _vp, _vp, _vp, _vp,
+                                _vp, _i]),
+    "sc_solve_adjoint": (_i, [_vp, _pd, _pd, _pd, _d, _i, _vp, _vp]),
+    "sc_interpolate_adjoint": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pd, _pd, _vp, _vp, _vp, _vp, _vp,
+                                    _vp, _vp, _i]),
     "sc_ctx_set_pool_size": (_i, [_vp, _i]),
     "sc_deposit_batched": (_i, [_pvp, _i, _i, _pvp, _pvp, _pvp, _pvp, _pi64, _pd, _pd, _i]),
     "sc_solve_batched": (_i, [_pvp, _i, _pd, _pd, _pd, _pd, _pi]),

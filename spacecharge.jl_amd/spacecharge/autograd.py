@@ -1,0 +1,112 @@
+"""Differentiable forms of deposit, solve and interpolate_field for torch.autograd.
+
+The reference has no derivatives; these wrap the library's vector-Jacobian
+products (sc_deposit_adjoint, sc_solve_adjoint, sc_interpolate_adjoint) as
+torch.autograd.Function objects, so that a tracking loop written in torch
+(drift-kick steps through the space-charge field) can be differentiated with
+respect to the particles' positions and charges:
+
+    import spacecharge.autograd as scad
+    rho = scad.deposit(mesh, x, y, z, q)
+    efield = scad.solve(mesh, rho)
+    Ex, Ey, Ez = scad.interpolate(mesh, efield, x, y, z)
+    # or: Ex, Ey, Ez = scad.step(mesh, x, y, z, q)
+
+The mesh geometry (bounds, spacing, gamma) is a constant of the
+differentiation: derivatives are taken with it held fixed, and each backward
+pass checks that it has not changed since the forward pass.  The functions
+use the mesh's rho and efield as scratch: they return copies, and the
+backward of ``interpolate`` writes the saved field back into mesh.efield
+before it runs.
+"""
+from __future__ import annotations
+
+import torch
+
+from . import (SpaceChargeError, deposit as _deposit, deposit_adjoint, interpolate_adjoint,
+               interpolate_field, solve as _solve, solve_adjoint)
+
+__all__ = ["deposit", "solve", "interpolate", "step"]
+
+
+def _geometry(mesh):
+    return (tuple(mesh.min_bounds), tuple(mesh.max_bounds), tuple(mesh.delta), float(mesh.gamma))
+
+
+def _check_geometry(ctx):
+    if _geometry(ctx.mesh) != ctx.geometry:
+        raise SpaceChargeError("the mesh geometry changed between the forward and backward pass")
+
+
+class _Deposit(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mesh, x, y, z, q):
+        _deposit(mesh, x, y, z, q)
+        ctx.mesh, ctx.geometry = mesh, _geometry(mesh)
+        ctx.save_for_backward(x, y, z, q)
+        return mesh.rho.clone()
+
+    @staticmethod
+    def backward(ctx, g_rho):
+        _check_geometry(ctx)
+        x, y, z, q = ctx.saved_tensors
+        need_pos = any(ctx.needs_input_grad[1:4])
+        gq, gx, gy, gz = deposit_adjoint(ctx.mesh, x, y, z, q, g_rho, positions=need_pos)
+        return None, gx, gy, gz, gq
+
+
+class _Solve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mesh, rho, at_cathode):
+        mesh.rho.copy_(rho)
+        _solve(mesh, at_cathode=at_cathode)
+        ctx.mesh, ctx.geometry, ctx.at_cathode = mesh, _geometry(mesh), at_cathode
+        return mesh.efield.clone()
+
+    @staticmethod
+    def backward(ctx, g_efield):
+        _check_geometry(ctx)
+        return None, solve_adjoint(ctx.mesh, g_efield, at_cathode=ctx.at_cathode), None
+
+
+class _Interpolate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, mesh, efield, x, y, z):
+        mesh.efield.copy_(efield)
+        Ex, Ey, Ez = interpolate_field(mesh, x, y, z)
+        ctx.mesh, ctx.geometry = mesh, _geometry(mesh)
+        ctx.save_for_backward(efield, x, y, z)
+        return Ex, Ey, Ez
+
+    @staticmethod
+    def backward(ctx, gEx, gEy, gEz):
+        _check_geometry(ctx)
+        efield, x, y, z = ctx.saved_tensors
+        gE = [torch.zeros_like(x) if g is None else g for g in (gEx, gEy, gEz)]
+        mesh = ctx.mesh
+        mesh.efield.copy_(efield)  # the position terms are taken at the forward field
+        need_pos = any(ctx.needs_input_grad[2:5])
+        g_ef, gx, gy, gz = interpolate_adjoint(mesh, x, y, z, *gE, positions=need_pos)
+        return None, g_ef, gx, gy, gz
+
+
+def deposit(mesh, x, y, z, q):
+    """rho (nx, ny, nz) of the CIC deposit of (x, y, z, q), differentiable in x, y, z, q."""
+    return _Deposit.apply(mesh, x, y, z, q)
+
+
+def solve(mesh, rho, at_cathode: bool = False):
+    """efield (nx, ny, nz, 3) of rho on the mesh, differentiable in rho."""
+    return _Solve.apply(mesh, rho, bool(at_cathode))
+
+
+def interpolate(mesh, efield, x, y, z):
+    """(Ex, Ey, Ez) of efield at (x, y, z), differentiable in efield, x, y, z."""
+    return _Interpolate.apply(mesh, efield, x, y, z)
+
+
+def step(mesh, x, y, z, q, at_cathode: bool = False):
+    """The space-charge step deposit -> solve -> interpolate, differentiable."""
+    rho = deposit(mesh, x, y, z, q)
+    efield = solve(mesh, rho, at_cathode)
+    return interpolate(mesh, efield, x, y, z)

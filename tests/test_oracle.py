@@ -17,6 +17,11 @@ code or formula:
                            the adjoint identity <interp(E), q> = <E, deposit(q)>.
   * bounds                 the closed form of src/mesh.jl:95-174 and
                            test/test_mesh.jl's assertions.
+  * adjoints (VJPs)         solve_transpose = the transpose of the solve's
+                           matrix built column by column from unit charges
+                           (free space and cathode); deposit / interpolation
+                           VJPs = central differences of the forward oracle,
+                           and the dot-product identities.
   * potential              40-digit mpmath values of the antiderivative of 1/r
                            (tests/golden/pot_golden.json, whose generator checks
                            d3P/dxdydz = 1/r) and cell integrals of 1/r computed
@@ -509,3 +514,90 @@ def test_potential_gradient_is_field():
         e = ef[..., ax]
         err = rel_l2(g[s, s, s], e[s, s, s])
         assert err < 0.015, (ax, err)
+
+
+# ---------------------------------------------------------------- adjoints
+@pytest.mark.parametrize("at_cathode", [False, True])
+def test_solve_transpose_is_matrix_transpose(at_cathode):
+    """S built column by column (solve of each unit charge), then S^T g."""
+    grid = (3, 4, 5)
+    d = (0.5, 0.25, 0.4)
+    gamma = 1.7
+    lo = (0.0, 0.0, 2 * d[2] if at_cathode else -0.3)
+    hi = tuple(l + (g - 1) * dd for l, g, dd in zip(lo, grid, d))
+    M = int(np.prod(grid))
+    A = np.zeros((3 * M, M))
+    for n in range(M):
+        e = np.zeros(M)
+        e[n] = 1.0
+        A[:, n] = orc.solve(np.asfortranarray(e.reshape(grid, order="F")), lo, hi, d, gamma,
+                            at_cathode).ravel(order="F")
+    rng = np.random.default_rng(21)
+    g = np.asfortranarray(rng.standard_normal(grid + (3,)))
+    ref = (A.T @ g.ravel(order="F")).reshape(grid, order="F")
+    got = orc.solve_transpose(g, lo, hi, d, gamma, at_cathode)
+    assert rel_l2(got, ref) < 1e-12
+    ext = orc.solve_transpose_extended(g, lo, hi, d, gamma, at_cathode)
+    assert rel_l2(ext, ref) < 1e-12
+    # Float32: the reference's Float32 Green function is ~1e-4 off (differenced in Float32)
+    g32 = orc.solve_transpose(g.astype(np.float32), lo, hi, d, gamma, at_cathode)
+    assert g32.dtype == np.float32 and rel_l2(g32, ref) < 1e-3
+    # the cathode image's transpose is not the image solve's negative
+    if at_cathode:
+        free = orc.solve_transpose(g, lo, hi, d, gamma, False)
+        assert rel_l2(free, ref) > 1e-3
+
+
+def _particles_in(grid, lo, d, n, seed):
+    rng = np.random.default_rng(seed)
+    pts = [lo[a] + rng.uniform(0.02, grid[a] - 1.02, n) * d[a] for a in range(3)]
+    return pts[0], pts[1], pts[2], rng.uniform(0.5, 1.5, n) * 1e-12
+
+
+def test_deposit_vjp_matches_central_differences():
+    grid, lo, d = (6, 5, 7), (-0.1, 0.2, 0.0), (0.3, 0.25, 0.2)
+    x, y, z, q = _particles_in(grid, lo, d, 40, 22)
+    rng = np.random.default_rng(23)
+    g = np.asfortranarray(rng.standard_normal(grid))
+    gq, gx, gy, gz = orc.deposit_vjp(x, y, z, q, g, lo, d)
+
+    def loss(px, py, pz, qq):
+        return float(np.sum(orc.deposit(px, py, pz, qq, grid, lo, d) * g))
+
+    # g_q: L is linear in q, so <g_q, dq> = L(dq) exactly
+    dq = rng.standard_normal(x.size)
+    assert abs(np.dot(gq, dq) - loss(x, y, z, dq)) <= 1e-12 * np.abs(gq).sum() * np.abs(dq).max()
+    for p in range(0, 40, 5):
+        for a, gref in enumerate((gx, gy, gz)):
+            h = 1e-6 * d[a]
+            pos = [x.copy(), y.copy(), z.copy()]
+            pos[a][p] += h
+            up = loss(pos[0][p:p + 1], pos[1][p:p + 1], pos[2][p:p + 1], q[p:p + 1])
+            pos[a][p] -= 2 * h
+            dn = loss(pos[0][p:p + 1], pos[1][p:p + 1], pos[2][p:p + 1], q[p:p + 1])
+            fd = (up - dn) / (2 * h)
+            assert abs(fd - gref[p]) <= 1e-7 * max(abs(gref[p]), 1e-30) + 1e-22, (p, a, fd, gref[p])
+
+
+def test_interpolate_vjp_matches_central_differences():
+    grid, lo, d = (6, 5, 7), (-0.1, 0.2, 0.0), (0.3, 0.25, 0.2)
+    x, y, z, _ = _particles_in(grid, lo, d, 60, 24)
+    rng = np.random.default_rng(25)
+    ef = np.asfortranarray(rng.standard_normal(grid + (3,)))
+    gE = [rng.standard_normal(x.size) for _ in range(3)]
+    gef, gx, gy, gz = orc.interpolate_vjp(ef, x, y, z, gE, lo, d)
+    # g_efield: <interp(E'), gE> = <E', g_efield> for any field E'
+    e2 = np.asfortranarray(rng.standard_normal(grid + (3,)))
+    lhs = sum(np.dot(a, b) for a, b in zip(orc.interpolate(e2, x, y, z, lo, d), gE))
+    assert abs(lhs - np.sum(e2 * gef)) <= 1e-12 * np.abs(e2).sum() * np.abs(gef).max()
+    # positions: each particle's terms depend on its own position only, so all
+    # particles are displaced at once and differenced per particle
+    for a, gref in enumerate((gx, gy, gz)):
+        h = 1e-6 * d[a]
+        pos = [x.copy(), y.copy(), z.copy()]
+        pos[a] += h
+        up = sum(e * w for e, w in zip(orc.interpolate(ef, *pos, lo, d), gE))
+        pos[a] -= 2 * h
+        dn = sum(e * w for e, w in zip(orc.interpolate(ef, *pos, lo, d), gE))
+        fd = (up - dn) / (2 * h)
+        assert np.abs(fd - gref).max() <= 1e-7 * np.abs(gref).max(), a
