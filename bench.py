@@ -18,6 +18,13 @@ at_cathode=true)).
 meshes from 16^3 to 128^3) through the batched calls and reports, beside it,
 the same bunches through a loop of single-bunch calls.
 
+--workload adjoint runs the differentiable configuration: 1,000,000 Gaussian
+particles (sigma = 1 mm), 64^3 mesh with fixed bounds (+-8 sigma); it times
+one forward step and one vector-Jacobian product of the step (interpolation,
+solve and deposit adjoints), and a 100-step drift-kick tracking run through
+torch.autograd whose loss (the final rms size) is differentiated with respect
+to the initial positions.
+
 Prints one JSON line.  `value` is particles/s of the whole job (one GPU).
 The `roofline` object is for the kernel with the largest share of the step
 time, measured with the library's HIP-event timers on the launching stream
@@ -162,6 +169,112 @@ def bench_batched(args, sc, torch, dev, T):
     }
 
 
+def bench_adjoint(args, sc, torch, dev, T):
+    """Forward step, its VJP, and a differentiated drift-kick tracking run."""
+    import spacecharge.autograd as scad
+    n = args.particles if args.particles != 4_000_000 else 1_000_000
+    g = args.grid if args.grid != 128 else 64
+    grid = (g, g, g)
+    sig = 1e-3
+    gen = torch.Generator(device=dev).manual_seed(17)
+    p = torch.randn((n, 3), generator=gen, dtype=torch.float64, device=dev) * sig
+    x, y, z = (p[:, i].contiguous().to(T) for i in range(3))
+    q = torch.full((n,), 1e-9 / n, dtype=T, device=dev)
+    mesh = sc.Mesh3D(grid, (-8 * sig,) * 3, (8 * sig,) * 3, T=T, gamma=args.gamma)
+    gE = [torch.randn(n, generator=gen, dtype=torch.float64, device=dev).to(T) for _ in range(3)]
+
+    def fwd():
+        sc.deposit(mesh, x, y, z, q)
+        sc.solve(mesh)
+        return sc.interpolate_field(mesh, x, y, z)
+
+    def vjp():
+        gef, gx, gy, gz = sc.interpolate_adjoint(mesh, x, y, z, *gE)
+        g_rho = sc.solve_adjoint(mesh, gef)
+        return sc.deposit_adjoint(mesh, x, y, z, q, g_rho)
+
+    def timed(fn, steps):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    ms_fwd = timed(fwd, args.steps)
+    fwd()
+    ms_vjp = timed(vjp, args.steps)
+
+    # drift-kick tracking: x += dt p, p += k E (normalised units), loss = rms size
+    nsteps = 100
+    dt, kick = 1.0, 1e-3 * sig / 1e7  # a kick of ~1e-3 sigma per step at 1e7 V/m
+
+    def track():
+        xs, ys, zs = (t.clone().requires_grad_(True) for t in (x, y, z))
+        px = torch.zeros_like(xs)
+        py = torch.zeros_like(xs)
+        pz = torch.zeros_like(xs)
+        cx, cy, cz = xs, ys, zs
+        for _ in range(nsteps):
+            Ex, Ey, Ez = scad.step(mesh, cx, cy, cz, q)
+            px, py, pz = px + kick * Ex, py + kick * Ey, pz + kick * Ez
+            cx, cy, cz = cx + dt * px, cy + dt * py, cz + dt * pz
+        loss = (cx.double().square() + cy.double().square() + cz.double().square()).mean()
+        loss.backward()
+        return xs.grad
+
+    track()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    grad = track()
+    torch.cuda.synchronize()
+    ms_track = (time.perf_counter() - t0) * 1e3
+    with torch.no_grad():
+        t2 = time.perf_counter()
+        cx, cy, cz = x.clone(), y.clone(), z.clone()
+        px, py, pz = (torch.zeros_like(x) for _ in range(3))
+        for _ in range(nsteps):
+            Ex, Ey, Ez = scad.step(mesh, cx, cy, cz, q)
+            px, py, pz = px + kick * Ex, py + kick * Ey, pz + kick * Ez
+            cx, cy, cz = cx + dt * px, cy + dt * py, cz + dt * pz
+        torch.cuda.synchronize()
+        ms_track_fwd = (time.perf_counter() - t2) * 1e3
+    return {
+        "metric": "particles/s per differentiable space-charge step (forward + vector-Jacobian product)",
+        "value": n / ((ms_fwd + ms_vjp) * 1e-3),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_fwd + ms_vjp,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic (seeded Gaussian bunch generated on the GPU)",
+        "config": {
+            "workload": (f"differentiable step, {n} particles, {grid[0]}^3 mesh with fixed bounds, "
+                         f"{nsteps}-step drift-kick tracking through torch.autograd"),
+            "particles": n,
+            "grid": list(grid),
+            "gamma": args.gamma,
+            "parallelism": "single GPU",
+        },
+        "forward_ms": ms_fwd,
+        "vjp_ms": ms_vjp,
+        "vjp_over_forward": ms_vjp / ms_fwd,
+        "tracking": {
+            "steps": nsteps,
+            "forward_only_ms": ms_track_fwd,
+            "forward_and_backward_ms": ms_track,
+            "grad_over_forward": ms_track / ms_track_fwd,
+            "grad_finite": bool(torch.isfinite(grad).all()),
+        },
+    }
+
+
 def kernel_bytes(name, n, grid, s, ps):
     """Algorithmic HBM bytes of one launch of a step kernel (compulsory I/O)."""
     nx, ny, nz = grid
@@ -271,11 +384,14 @@ def main():
     ap.add_argument("--particles", type=int, default=4_000_000)
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cathode", action="store_true")
-    ap.add_argument("--workload", choices=["bunch", "cathode", "batched"], default="bunch",
+    ap.add_argument("--workload", choices=["bunch", "cathode", "batched", "adjoint"],
+                    default="bunch",
                     help="bunch: Gaussian + uniform ellipsoid (defaults 4M particles, 128^3); "
                          "cathode: uniform disk with exponential z emitted from a cathode, "
                          "2M particles, 64x64x128, image charge on; batched: 32 bunches of "
-                         "uneven size and mesh through the batched calls")
+                         "uneven size and mesh through the batched calls; adjoint: forward "
+                         "step, its VJP and a differentiated 100-step tracking run, 1M "
+                         "particles, 64^3")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the CPU-oracle sample (0 disables it)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -296,6 +412,9 @@ def main():
     T = torch.float32 if args.dtype == "f32" else torch.float64
     if args.workload == "batched":
         print(json.dumps(bench_batched(args, sc, torch, dev, T)))
+        return
+    if args.workload == "adjoint":
+        print(json.dumps(bench_adjoint(args, sc, torch, dev, T)))
         return
     if args.workload == "cathode":
         n = args.particles if args.particles != 4_000_000 else 2_000_000

@@ -256,12 +256,17 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
 // tile_of as k_bin_count make every run exactly as long as counted.  Also
 // flags the tiles that hold a particle with weights outside [0, 1] and
 // takes max|q| (both steer the deposit's accumulator).
-template <typename PT, typename MT>
+// With NCH = 3 the charges q1, q2 of two more channels are written in the
+// same order to q1s, q2s and their per-slice max|q| after channel 0's
+// (kBinMaxSlices apart).
+template <typename PT, typename MT, int NCH>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
               const PT *__restrict__ pq, Split sp, Geo g, const unsigned *__restrict__ cmat,
               const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted,
-              unsigned *__restrict__ ext_flags, unsigned long long *__restrict__ slice_qmax)
+              unsigned *__restrict__ ext_flags, unsigned long long *__restrict__ slice_qmax,
+              const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s,
+              PT *__restrict__ q2s)
 {
     __shared__ unsigned cur[kMaxTiles];
     __shared__ unsigned char ext_lds[kMaxTiles];
@@ -271,11 +276,12 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
         ext_lds[t] = 0;
     }
     __syncthreads();
-    double qm = 0.0;
+    double qm = 0.0, qm1 = 0.0, qm2 = 0.0;
     const int64_t first = (int64_t)blockIdx.x * sp.slice;
     const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
     for (int64_t base = first; base < end; base += kBatch * kBinThreads) {
         Rec<PT> r[kBatch];
+        PT c1[kBatch], c2[kBatch];
         unsigned slot[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
@@ -285,6 +291,10 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
             r[s].y = py[p];
             r[s].z = pz[p];
             r[s].q = pq[p];
+            if constexpr (NCH == 3) {
+                c1[s] = q1[p];
+                c2[s] = q2[p];
+            }
         }
         // tiles first, then the slot atomics back to back, then the flags
         // and stores (an LDS store between the atomics would order them)
@@ -306,27 +316,48 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
                 const double aq = fabs((double)r[s].q);
                 qm = (aq > qm || aq != aq) ? aq : qm;  // NaN sticks
                 sorted[slot[s]] = r[s];
+                if constexpr (NCH == 3) {
+                    const double a1 = fabs((double)c1[s]), a2 = fabs((double)c2[s]);
+                    qm1 = (a1 > qm1 || a1 != a1) ? a1 : qm1;
+                    qm2 = (a2 > qm2 || a2 != a2) ? a2 : qm2;
+                    q1s[slot[s]] = c1[s];
+                    q2s[slot[s]] = c2[s];
+                }
             }
         }
     }
     // |q| >= 0, so its bit pattern orders like its value (NaN above +inf);
     // one plain store per slice (the deposit reduces them)
-    __shared__ unsigned long long wmax[kBinThreads / 64];
-    unsigned long long bits = (unsigned long long)__double_as_longlong(qm);
+    __shared__ unsigned long long wmax[NCH][kBinThreads / 64];
+    const double qms[3] = {qm, qm1, qm2};
 #pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        const unsigned long long o = __shfl_xor(bits, d, 64);
-        bits = o > bits ? o : bits;
+    for (int c = 0; c < NCH; ++c) {
+        unsigned long long bits = (unsigned long long)__double_as_longlong(qms[c]);
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) {
+            const unsigned long long o = __shfl_xor(bits, d, 64);
+            bits = o > bits ? o : bits;
+        }
+        if ((threadIdx.x & 63) == 0) wmax[c][threadIdx.x >> 6] = bits;
     }
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = bits;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long m = wmax[0];
-        for (int k = 1; k < kBinThreads / 64; ++k) m = wmax[k] > m ? wmax[k] : m;
-        slice_qmax[blockIdx.x] = m;
+    if (threadIdx.x < NCH) {
+        const int c = threadIdx.x;
+        unsigned long long m = wmax[c][0];
+        for (int k = 1; k < kBinThreads / 64; ++k) m = wmax[c][k] > m ? wmax[c][k] : m;
+        slice_qmax[(size_t)c * kBinMaxSlices + blockIdx.x] = m;
     }
     for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
         if (ext_lds[t]) atomicOr(&ext_flags[t], 1u);  // rare: particles outside the mesh
+}
+
+// Charges of another channel into the records (same tile order).
+template <typename PT>
+__global__ void __launch_bounds__(256)
+k_set_q(Rec<PT> *__restrict__ sorted, const PT *__restrict__ qs, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sorted[i].q = qs[i];
 }
 
 // CIC contributions of one record, in the reference's arithmetic
@@ -563,8 +594,8 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
     }
     if (!B.counts) {
         // totals | tile start | first extra sub-grid slot | extrapolation flags | nchunks, pad |
-        // per-slice max|q| (u64 bit patterns)
-        const size_t bytes = sizeof(unsigned) * (4 * kMaxTiles + 4 + 2 * kBinMaxSlices);
+        // per-slice max|q| of up to three channels (u64 bit patterns)
+        const size_t bytes = sizeof(unsigned) * (4 * kMaxTiles + 4 + 3 * 2 * kBinMaxSlices);
         if (hipMalloc((void **)&B.counts, bytes) != hipSuccess) return fail();
     }
     const int64_t need_cmat = (int64_t)ntiles * split_of(n).G;
@@ -580,8 +611,8 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
 }
 
 template <typename PT, typename MT>
-int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, int64_t n,
-           const Geo &g, int clear, void *out)
+int binned(Mesh *m, const void *x, const void *y, const void *z, const void *const *q, int nchan,
+           int64_t n, const Geo &g, int clear, void *const *out)
 {
     Ctx *ctx = m->ctx;
     hipStream_t st = ctx->stream;
@@ -605,29 +636,46 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *q, 
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
                        cbase, chunks, nch, ext, ntiles);
     launch_end(ctx, K_BIN_SCAN);
+    Rec<PT> *sorted = static_cast<Rec<PT> *>(B.sorted);
+    PT *q1s = static_cast<PT *>(B.qs), *q2s = q1s ? q1s + n : nullptr;
     launch_begin(ctx, K_BIN_SCATTER);
-    hipLaunchKernelGGL((k_bin_scatter<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st,
-                       (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q, sp, g,
-                       (const unsigned *)B.cmat, (const unsigned *)cursor,
-                       static_cast<Rec<PT> *>(B.sorted), ext, slice_qmax);
+    if (nchan == 3)
+        hipLaunchKernelGGL((k_bin_scatter<PT, MT, 3>), dim3(sp.G), dim3(kBinThreads), 0, st,
+                           (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q[0], sp, g,
+                           (const unsigned *)B.cmat, (const unsigned *)cursor, sorted, ext,
+                           slice_qmax, (const PT *)q[1], (const PT *)q[2], q1s, q2s);
+    else
+        hipLaunchKernelGGL((k_bin_scatter<PT, MT, 1>), dim3(sp.G), dim3(kBinThreads), 0, st,
+                           (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q[0], sp, g,
+                           (const unsigned *)B.cmat, (const unsigned *)cursor, sorted, ext,
+                           slice_qmax, (const PT *)nullptr, (const PT *)nullptr, (PT *)nullptr,
+                           (PT *)nullptr);
     launch_end(ctx, K_BIN_SCATTER);
     const unsigned maxchunks = (unsigned)(ntiles + n / kChunk + 1);
-    launch_begin(ctx, K_DEPOSIT_TILES);
-    hipLaunchKernelGGL((k_deposit_tiles<PT, MT>), dim3(maxchunks), dim3(kDepThreads), 0, st,
-                       (const Rec<PT> *)B.sorted, (const int4 *)chunks, (const unsigned *)nch, g,
-                       static_cast<double *>(B.partials), (const unsigned *)ext,
-                       (const unsigned long long *)slice_qmax, sp.G);
-    launch_end(ctx, K_DEPOSIT_TILES);
     const int64_t M = (int64_t)g.nx * g.ny * g.nz;
-    launch_begin(ctx, K_TILE_REDUCE);
-    hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((kSubVol + 255) / 256), (unsigned)ntiles), dim3(256),
-                       0, st, static_cast<double *>(B.partials), (const unsigned *)counts,
-                       (const unsigned *)cbase);
-    launch_end(ctx, K_TILE_REDUCE);
-    launch_begin(ctx, K_GATHER_TILES);
-    hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
-                       (const double *)B.partials, g, static_cast<MT *>(out), clear ? 0 : 1);
-    launch_end(ctx, K_GATHER_TILES);
+    for (int c = 0; c < nchan; ++c) {
+        if (c > 0) {
+            // the next channel's charges into the records, same tile order
+            hipLaunchKernelGGL((k_set_q<PT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                               sorted, (const PT *)(c == 1 ? q1s : q2s), n);
+        }
+        launch_begin(ctx, K_DEPOSIT_TILES);
+        hipLaunchKernelGGL((k_deposit_tiles<PT, MT>), dim3(maxchunks), dim3(kDepThreads), 0, st,
+                           (const Rec<PT> *)sorted, (const int4 *)chunks, (const unsigned *)nch, g,
+                           static_cast<double *>(B.partials), (const unsigned *)ext,
+                           (const unsigned long long *)(slice_qmax + (size_t)c * kBinMaxSlices),
+                           sp.G);
+        launch_end(ctx, K_DEPOSIT_TILES);
+        launch_begin(ctx, K_TILE_REDUCE);
+        hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((kSubVol + 255) / 256), (unsigned)ntiles),
+                           dim3(256), 0, st, static_cast<double *>(B.partials),
+                           (const unsigned *)counts, (const unsigned *)cbase);
+        launch_end(ctx, K_TILE_REDUCE);
+        launch_begin(ctx, K_GATHER_TILES);
+        hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
+                           (const double *)B.partials, g, static_cast<MT *>(out[c]), clear ? 0 : 1);
+        launch_end(ctx, K_GATHER_TILES);
+    }
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
 }
@@ -642,17 +690,15 @@ void free_bins(Mesh *m)
     if (B.partials) (void)hipFree(B.partials);
     if (B.counts) (void)hipFree(B.counts);
     if (B.cmat) (void)hipFree(B.cmat);
+    if (B.qs) (void)hipFree(B.qs);
     B = DepositBins();
 }
 
-// Binned deposit into `out` (nx*ny*nz of the mesh type); returns
-// SC_ERR_UNSUPPORTED (without launching anything) when the grid has more
-// tiles than the LDS histogram holds.  Writes every node of out
-// (clear != 0) or adds to it (clear == 0).
-int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const void *z,
-                          const void *q, int64_t n, const double lo[3], const double dl[3],
-                          int clear, void *out)
+int launch_deposit_binned_multi(Mesh *m, int pdt, const void *x, const void *y, const void *z,
+                                const void *const *q, int nchan, int64_t n, const double lo[3],
+                                const double dl[3], int clear, void *const *out)
 {
+    if (nchan != 1 && nchan != 3) return SC_ERR_UNSUPPORTED;
     Geo g;
     g.nx = (int)m->n[0];
     g.ny = (int)m->n[1];
@@ -667,11 +713,36 @@ int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const 
         g.dl[a] = dl[a];
     }
     if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz)) return rc;
-    if (pdt == SC_F32 && m->dtype == SC_F32) return binned<float, float>(m, x, y, z, q, n, g, clear, out);
+    if (nchan > 1) {
+        DepositBins &B = m->bins;
+        const size_t need = (size_t)2 * n * pdt;
+        if (B.qs_bytes < need) {
+            if (B.qs) (void)hipFree(B.qs);
+            B.qs = nullptr;
+            B.qs_bytes = 0;
+            if (hipMalloc(&B.qs, need) != hipSuccess)
+                return set_err(m->ctx, SC_ERR_NOMEM, "device allocation of the deposit bins failed");
+            B.qs_bytes = need;
+        }
+    }
+    if (pdt == SC_F32 && m->dtype == SC_F32)
+        return binned<float, float>(m, x, y, z, q, nchan, n, g, clear, out);
     if (pdt == SC_F64 && m->dtype == SC_F64)
-        return binned<double, double>(m, x, y, z, q, n, g, clear, out);
-    if (pdt == SC_F64 && m->dtype == SC_F32) return binned<double, float>(m, x, y, z, q, n, g, clear, out);
-    return binned<float, double>(m, x, y, z, q, n, g, clear, out);
+        return binned<double, double>(m, x, y, z, q, nchan, n, g, clear, out);
+    if (pdt == SC_F64 && m->dtype == SC_F32)
+        return binned<double, float>(m, x, y, z, q, nchan, n, g, clear, out);
+    return binned<float, double>(m, x, y, z, q, nchan, n, g, clear, out);
+}
+
+// Binned deposit into `out` (nx*ny*nz of the mesh type); returns
+// SC_ERR_UNSUPPORTED (without launching anything) when the grid has more
+// tiles than the LDS histogram holds.  Writes every node of out
+// (clear != 0) or adds to it (clear == 0).
+int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const void *z,
+                          const void *q, int64_t n, const double lo[3], const double dl[3],
+                          int clear, void *out)
+{
+    return launch_deposit_binned_multi(m, pdt, x, y, z, &q, 1, n, lo, dl, clear, &out);
 }
 
 }  // namespace sc

@@ -153,6 +153,8 @@ struct DepositBins {
     unsigned *counts = nullptr;   // per-tile totals | tile start | chunk_base | nchunks
     unsigned *cmat = nullptr;     // [tile][binning workgroup] counts, then offsets
     int64_t cmat_cap = 0;         // entries
+    void *qs = nullptr;           // charges of channels 1, 2 in tile order (multi-channel deposit)
+    size_t qs_bytes = 0;
 };
 
 struct Mesh {
@@ -195,6 +197,12 @@ int launch_deposit_to(Mesh *m, int pdt, const void *x, const void *y, const void
 int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const void *z,
                           const void *q, int64_t n, const double lo[3], const double dl[3],
                           int clear, void *out);
+// nch (1..3) deposits of the same particles with charges q[c] into out[c],
+// binned once; SC_ERR_UNSUPPORTED (nothing launched) where the binned path
+// does not apply.
+int launch_deposit_binned_multi(Mesh *m, int pdt, const void *x, const void *y, const void *z,
+                                const void *const *q, int nch, int64_t n, const double lo[3],
+                                const double dl[3], int clear, void *const *out);
 // Vector-Jacobian products of deposit (g_rho -> g_q, g_x, g_y, g_z) and of
 // interpolation (g_E -> g_efield, g_x, g_y, g_z) at the mesh's current
 // efield; NULL outputs are skipped; position gradients are added to the

@@ -512,11 +512,18 @@ int launch_interp_vjp(Mesh *m, int pdt, const void *x, const void *y, const void
 {
     const int64_t M = m->n[0] * m->n[1] * m->n[2];
     if (g_efield) {
-        // the transpose of the gather is the deposit of the cotangents
+        // the transpose of the gather is the deposit of the cotangents: the
+        // three channels share one binning pass where the binned path applies
         const void *gE[3] = {gEx, gEy, gEz};
-        for (int c = 0; c < 3; ++c) {
-            void *out = static_cast<char *>(g_efield) + (size_t)c * M * m->dtype;
-            if (int rc = launch_deposit_to(m, pdt, x, y, z, gE[c], n, lo, dl, 1, out)) return rc;
+        void *outs[3];
+        for (int c = 0; c < 3; ++c) outs[c] = static_cast<char *>(g_efield) + (size_t)c * M * m->dtype;
+        int rc = n > 0 ? launch_deposit_binned_multi(m, pdt, x, y, z, gE, 3, n, lo, dl, 1, outs)
+                       : SC_ERR_UNSUPPORTED;
+        if (rc == SC_ERR_UNSUPPORTED) {
+            for (int c = 0; c < 3; ++c)
+                if ((rc = launch_deposit_to(m, pdt, x, y, z, gE[c], n, lo, dl, 1, outs[c]))) return rc;
+        } else if (rc) {
+            return rc;
         }
     }
     if (n <= 0 || !gx) return SC_OK;
