@@ -116,6 +116,22 @@ int sc_bounds_from_particles(sc_ctx *ctx, int pdtype, const void *x, const void 
                              int64_t n, const int64_t grid[3], int dtype, double min_bounds[3],
                              double max_bounds[3], double delta[3]);
 
+/* ---- device-resident bounds (no reference counterpart; serve tracking
+ * loops that re-mesh every step without a host round trip) ----------------
+ * sc_mesh_auto_bounds computes the bounds of Mesh3D(grid, x, y, z; T)
+ * (src/mesh.jl:95-174, the same rules and bits as sc_bounds_from_particles)
+ * on the device into the mesh's own geometry block, stream-ordered, without
+ * synchronising.  Every step function then accepts NULL bounds (min_bounds
+ * and delta NULL; for sc_solve and sc_solve_adjoint also max_bounds) and
+ * reads that block instead, so the whole step, bounds included, can be
+ * captured into a graph.  Without host spacings the solve rebuilds its
+ * Green function every call (sc_mesh_set_green_reuse has no key to compare).
+ * sc_mesh_read_bounds copies the block to the host (synchronises). */
+int sc_mesh_auto_bounds(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                        int64_t n);
+int sc_mesh_read_bounds(sc_mesh *mesh, double min_bounds[3], double max_bounds[3],
+                        double delta[3]);
+
 /* Replaces clear_mesh! (src/deposition.jl:10-12): rho = 0. */
 int sc_clear_mesh(sc_mesh *mesh);
 

@@ -312,6 +312,7 @@ int sc_mesh_destroy(sc_mesh *mesh)
     free_workspace(&M);
     free_bins(&M);
     if (M.efi) (void)hipFree(M.efi);
+    if (M.geom) (void)hipFree(M.geom);
     if (M.owns) {
         (void)hipFree(M.rho);
         (void)hipFree(M.efield);
@@ -393,6 +394,64 @@ static int check_geom(Ctx *c, const double lo[3], const double dl[3])
     return SC_OK;
 }
 
+// Host bounds, or none (lo and dl NULL) when the mesh holds device-resident ones.
+static int check_geom_m(Ctx *c, const Mesh &M, const double lo[3], const double dl[3])
+{
+    if (!lo && !dl) {
+        if (!M.geom_set)
+            return set_err(c, SC_ERR_INVALID,
+                           "NULL bounds: the mesh has no device-resident bounds (sc_mesh_auto_bounds)");
+        return SC_OK;
+    }
+    return check_geom(c, lo, dl);
+}
+
+// The solve's bounds: all three given (and checked), or all NULL for the
+// mesh's device-resident ones.
+static int check_solve_geom(Ctx *c, const Mesh &M, const double lo[3], const double hi[3],
+                            const double dl[3], int at_cathode)
+{
+    if (!lo && !hi && !dl) return check_geom_m(c, M, lo, dl);
+    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (!hi) return set_err(c, SC_ERR_INVALID, "NULL max_bounds");
+    if (at_cathode && !std::isfinite(hi[2]))
+        return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
+    return SC_OK;
+}
+
+int sc_mesh_auto_bounds(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                        int64_t n)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n <= 0) return set_err(c, SC_ERR_INVALID, "Particle arrays cannot be empty.");
+    if (!x || !y || !z) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
+    if (int rc = check_dev(c)) return rc;
+    return launch_bounds_geom(&M, pdtype, x, y, z, n);
+}
+
+int sc_mesh_read_bounds(sc_mesh *mesh, double lo[3], double hi[3], double delta[3])
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!lo || !hi || !delta) return set_err(c, SC_ERR_INVALID, "NULL argument");
+    if (!M.geom_set)
+        return set_err(c, SC_ERR_INVALID, "the mesh has no device-resident bounds (sc_mesh_auto_bounds)");
+    if (int rc = check_dev(c)) return rc;
+    double h[9];
+    SC_HIP(c, hipMemcpyAsync(h, M.geom, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    SC_HIP(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = h[a];
+        hi[a] = h[3 + a];
+        delta[a] = h[6 + a];
+    }
+    return SC_OK;
+}
+
 int sc_clear_mesh(sc_mesh *mesh)
 {
     if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
@@ -410,7 +469,7 @@ int sc_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y, const vo
     if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
     if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
     if (n > 0 && (!x || !y || !z || !q)) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
-    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_geom_m(c, M, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
     return launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl, clear);
 }
@@ -421,12 +480,9 @@ int sc_solve(sc_mesh *mesh, const double lo[3], const double hi[3], const double
     if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
     Mesh &M = mesh->m;
     Ctx *c = M.ctx;
-    if (int rc = check_geom(c, lo, dl)) return rc;
-    if (!hi) return set_err(c, SC_ERR_INVALID, "NULL max_bounds");
+    if (int rc = check_solve_geom(c, M, lo, hi, dl, at_cathode)) return rc;
     if (!std::isfinite(gamma) || !(gamma > 0.0))
         return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
-    if (at_cathode && !std::isfinite(hi[2]))
-        return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
     if (int rc = check_dev(c)) return rc;
     return solve_mesh(&M, lo, hi, dl, gamma, at_cathode);
 }
@@ -437,12 +493,12 @@ int sc_solve_potential(sc_mesh *mesh, const double dl[3], double gamma, void *ph
     Mesh &M = mesh->m;
     Ctx *c = M.ctx;
     static const double origin[3] = {0.0, 0.0, 0.0};  // free space: no bounds enter
-    if (int rc = check_geom(c, origin, dl)) return rc;
+    if (int rc = dl ? check_geom(c, origin, dl) : check_geom_m(c, M, nullptr, nullptr)) return rc;
     if (!std::isfinite(gamma) || !(gamma > 0.0))
         return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
     if (!phi) return set_err(c, SC_ERR_INVALID, "NULL phi");
     if (int rc = check_dev(c)) return rc;
-    return solve_potential_mesh(&M, origin, origin, dl, gamma, phi);
+    return solve_potential_mesh(&M, dl ? origin : nullptr, dl ? origin : nullptr, dl, gamma, phi);
 }
 
 int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
@@ -455,7 +511,7 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
     if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
     if (n > 0 && (!x || !y || !z || !Ex || !Ey || !Ez))
         return set_err(c, SC_ERR_INVALID, "NULL particle or output pointer");
-    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_geom_m(c, M, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
     return launch_interp(&M, pdtype, x, y, z, n, lo, dl, Ex, Ey, Ez);
 }
@@ -483,7 +539,7 @@ int sc_deposit_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void *y, 
         if (!g_rho) return set_err(c, SC_ERR_INVALID, "NULL g_rho");
         if (g_x && !q) return set_err(c, SC_ERR_INVALID, "position gradients need q");
     }
-    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_geom_m(c, M, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
     return launch_deposit_vjp(&M, pdtype, x, y, z, q, n, lo, dl, g_rho, g_q, g_x, g_y, g_z,
                               accumulate);
@@ -495,12 +551,9 @@ int sc_solve_adjoint(sc_mesh *mesh, const double lo[3], const double hi[3], cons
     if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
     Mesh &M = mesh->m;
     Ctx *c = M.ctx;
-    if (int rc = check_geom(c, lo, dl)) return rc;
-    if (!hi) return set_err(c, SC_ERR_INVALID, "NULL max_bounds");
+    if (int rc = check_solve_geom(c, M, lo, hi, dl, at_cathode)) return rc;
     if (!std::isfinite(gamma) || !(gamma > 0.0))
         return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
-    if (at_cathode && !std::isfinite(hi[2]))
-        return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
     if (!g_efield || !g_rho) return set_err(c, SC_ERR_INVALID, "NULL g_efield or g_rho");
     if (int rc = check_dev(c)) return rc;
     return solve_adjoint_mesh(&M, lo, hi, dl, gamma, at_cathode, g_efield, g_rho);
@@ -521,7 +574,7 @@ int sc_interpolate_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void 
         if (!x || !y || !z) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
         if (!g_Ex || !g_Ey || !g_Ez) return set_err(c, SC_ERR_INVALID, "NULL field cotangent");
     }
-    if (int rc = check_geom(c, lo, dl)) return rc;
+    if (int rc = check_geom_m(c, M, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
     return launch_interp_vjp(&M, pdtype, x, y, z, n, lo, dl, g_Ex, g_Ey, g_Ez, g_efield, g_x, g_y,
                              g_z, accumulate);

@@ -68,7 +68,21 @@ struct Geo {
     int nx, ny, nz;
     int tx, ty, tz;  // tiles per axis
     double lo[3], dl[3];
+    const double *dev;  // device-resident geometry (lo at 0, delta at 6) when set
 };
+
+// Takes the lower bounds and spacing from device memory when the mesh
+// geometry lives there.
+__device__ __forceinline__ void resolve_geo(Geo &g)
+{
+    if (g.dev) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            g.lo[a] = g.dev[a];
+            g.dl[a] = g.dev[6 + a];
+        }
+    }
+}
 
 template <typename C>
 __device__ __forceinline__ int cell_of(C norm, int n)
@@ -118,6 +132,7 @@ __global__ void __launch_bounds__(kBinThreads)
 k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
             Split sp, Geo g, unsigned *__restrict__ cmat)
 {
+    resolve_geo(g);
     __shared__ unsigned hist[kMaxTiles];
     const int ntiles = g.tx * g.ty * g.tz;
     for (int t = threadIdx.x; t < ntiles; t += kBinThreads) hist[t] = 0u;
@@ -268,6 +283,7 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
               const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s,
               PT *__restrict__ q2s)
 {
+    resolve_geo(g);
     __shared__ unsigned cur[kMaxTiles];
     __shared__ unsigned char ext_lds[kMaxTiles];
     const int ntiles = g.tx * g.ty * g.tz;
@@ -436,6 +452,7 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chu
                 const unsigned *__restrict__ ext_flags,
                 const unsigned long long *__restrict__ slice_qmax, int nslices)
 {
+    resolve_geo(g);
     __shared__ double sub[kSubVol];
     __shared__ unsigned long long wmax[kDepThreads / 64];
     if (blockIdx.x >= *nchunks) return;
@@ -708,9 +725,11 @@ int launch_deposit_binned_multi(Mesh *m, int pdt, const void *x, const void *y, 
     g.tz = (g.nz - 1 + kTile - 1) / kTile;
     if ((int64_t)g.tx * g.ty * g.tz > kMaxTiles) return SC_ERR_UNSUPPORTED;
     if (n > 0x7fffffffLL) return SC_ERR_UNSUPPORTED;
+    // without host bounds the kernels read the mesh's device-resident ones
+    g.dev = lo ? nullptr : m->geom;
     for (int a = 0; a < 3; ++a) {
-        g.lo[a] = lo[a];
-        g.dl[a] = dl[a];
+        g.lo[a] = lo ? lo[a] : 0.0;
+        g.dl[a] = lo ? dl[a] : 1.0;
     }
     if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz)) return rc;
     if (nchan > 1) {

@@ -79,6 +79,11 @@ struct GreenArgs {
     double offzg;       // image z offset times gamma (:84)
     double scale[4];    // FPEI * factor_c / (8 nx ny nz); [3] for the potential
     int pot;            // 1: one component, the scalar potential (c = 3)
+    // device-resident mesh geometry (lo at 0, hi at 3, delta at 6): when set,
+    // the spacings, the image offset and the scales above are formed from it
+    // in the kernels, with the same operations as build_green
+    const double *geom;
+    double gamma;
     int64_t f_free, f_img;  // F elements per component
     int64_t r_free, r_img;  // R elements per component
 };
@@ -105,6 +110,34 @@ __device__ __forceinline__ void unpack_pair(cpx<T> X, bool odd, T &ra, T &rb)
     else { ra = X.x; rb = X.y; }
 }
 
+// Spacings (dz stretched by gamma) and the image offset times gamma.
+__device__ __forceinline__ void green_geometry(const GreenArgs &a, double &dx, double &dy, double &dz,
+                                               double &offzg)
+{
+    if (a.geom) {
+        dx = a.geom[6];
+        dy = a.geom[7];
+        dz = a.geom[8] * a.gamma;
+        offzg = (a.geom[2] + a.geom[5]) * a.gamma;
+    } else {
+        dx = a.dx;
+        dy = a.dy;
+        dz = a.dz;
+        offzg = a.offzg;
+    }
+}
+
+// FPEI * factor_c / (8 nx ny nz) of component c (3: the potential).
+__device__ __forceinline__ double green_scale(const GreenArgs &a, int c)
+{
+    if (!a.geom) return a.scale[c];
+    double dx, dy, dz, offzg;
+    green_geometry(a, dx, dy, dz, offzg);
+    const double vol = (dx * dy) * dz;
+    const double P = (double)a.d.Lx * (double)a.d.Ly * (double)a.d.Lz;
+    return kFPEI * ((c != 2 ? a.gamma : 1.0) / vol) / P;
+}
+
 // The function computed by component set q: field component q % 3 (image
 // set q / 3), or the potential (c = 3, odd in no axis).
 __device__ __forceinline__ int comp_of(const GreenArgs &a, int q) { return a.pot ? 3 : q % 3; }
@@ -122,10 +155,11 @@ __global__ void __launch_bounds__(256) k_green_f(double *__restrict__ F, GreenAr
     const int in = pidx % nx1;
     const int jn = pidx / nx1;
     const int64_t idx = (int64_t)kn * nx1 * ny1 + pidx;
-    const double u = ((double)in - 0.5) * a.dx;
-    const double v = ((double)jn - 0.5) * a.dy;
-    const double w = image ? ((double)(kn - a.d.nz) + 0.5) * a.dz + a.offzg
-                           : ((double)kn - 0.5) * a.dz;
+    double dx, dy, dz, offzg;
+    green_geometry(a, dx, dy, dz, offzg);
+    const double u = ((double)in - 0.5) * dx;
+    const double v = ((double)jn - 0.5) * dy;
+    const double w = image ? ((double)(kn - a.d.nz) + 0.5) * dz + offzg : ((double)kn - 0.5) * dz;
     F[f_off(a, q) + idx] = field_green_comp(c, u, v, w);
 }
 
@@ -305,7 +339,7 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
         __syncthreads();
     }
     lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
-    const T scale = (T)a.scale[c];
+    const T scale = (T)green_scale(a, c);
     if (!image) {
         T *G = Gf + (size_t)(q % 3) * (nz + 1) * ny1 * Kp;
         for (int idx = threadIdx.x; idx < nb * (nz + 1); idx += blockDim.x) {
@@ -525,7 +559,7 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
         v[r].y = (kxb < Kx && on[r]) ? sc[r] * rb[r] : (T)0;
     }
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
-    const T scale = (T)a.scale[c];
+    const T scale = (T)green_scale(a, c);
     if (!image) {
         T *G = Gf + (size_t)(q % 3) * (nz + 1) * ny1 * Kp;
 #pragma unroll
@@ -607,16 +641,21 @@ template <typename T>
 int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode)
 {
     const int cath = mode == kFieldCathode;
+    // without host spacings the kernels read the mesh's device-resident geometry
+    static const double unit[3] = {1.0, 1.0, 1.0};
+    const double *dlh = dl ? dl : unit;
     Ctx *ctx = m->ctx;
     Workspace &ws = m->ws;
     hipStream_t st = ctx->stream;
     GreenArgs a;
     a.d = make_dims(m);
     const Dims &d = a.d;
-    a.dx = dl[0];
-    a.dy = dl[1];
-    a.dz = dl[2] * gamma;
+    a.dx = dlh[0];
+    a.dy = dlh[1];
+    a.dz = dlh[2] * gamma;
     a.offzg = offz * gamma;
+    a.geom = dl ? nullptr : m->geom;
+    a.gamma = gamma;
     const double vol = (a.dx * a.dy) * a.dz;
     const double P = (double)d.Lx * (double)d.Ly * (double)d.Lz;
     for (int c = 0; c < 3; ++c) a.scale[c] = kFPEI * ((c < 2 ? gamma : 1.0) / vol) / P;

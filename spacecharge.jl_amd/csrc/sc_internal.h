@@ -169,6 +169,11 @@ struct Mesh {
     Workspace ws;
     DepositBins bins;
     void *efi = nullptr;  // node-interleaved field for the interpolation gather
+    // device-resident geometry (lo[3], hi[3], delta[3] as doubles holding
+    // mesh-type values), written by sc_mesh_auto_bounds on the stream; the
+    // step functions read it when called without bounds
+    double *geom = nullptr;
+    bool geom_set = false;
 };
 
 // ---------------------------------------------------------------------------
@@ -218,6 +223,9 @@ int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z,
                   const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez);
 int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,
                   double mn[3], double mx[3]);
+// The particle-extrema bounds of src/mesh.jl:95-174 computed on the device
+// into m->geom, stream-ordered, without a host round trip.
+int launch_bounds_geom(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n);
 int solve_mesh(Mesh *m, const double lo[3], const double hi[3], const double dl[3], double gamma,
                int at_cathode);
 // Scalar potential (free space) of m->rho into phi (nx*ny*nz of the mesh type).

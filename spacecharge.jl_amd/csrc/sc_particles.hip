@@ -35,16 +35,33 @@ __device__ __forceinline__ int cell_index(C norm, int n)
     return (int)f;
 }
 
+// Lower bounds and spacing from the mesh's device-resident geometry (lo at
+// 0, delta at 6) when the call came without host bounds.
+__device__ __forceinline__ void device_geometry(const double *geom, double &lx, double &ly,
+                                                double &lz, double &dx, double &dy, double &dz)
+{
+    if (geom) {
+        lx = geom[0];
+        ly = geom[1];
+        lz = geom[2];
+        dx = geom[6];
+        dy = geom[7];
+        dz = geom[8];
+    }
+}
+
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(256)
 k_deposit(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
           const PT *__restrict__ pq, int64_t n, MT *__restrict__ rho, int nx, int ny, int nz,
-          double lx_, double ly_, double lz_, double dx_, double dy_, double dz_)
+          double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
+          const double *__restrict__ geom)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    device_geometry(geom, lx_, ly_, lz_, dx_, dy_, dz_);
     const C lx = (C)(MT)lx_, ly = (C)(MT)ly_, lz = (C)(MT)lz_;
     const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
     const C nxn = ((C)px[p] - lx) / ddx;
@@ -98,12 +115,14 @@ __global__ void __launch_bounds__(256)
 k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
          const PT *__restrict__ py, const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz,
          double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
-         PT *__restrict__ Ex, PT *__restrict__ Ey, PT *__restrict__ Ez)
+         const double *__restrict__ geom, PT *__restrict__ Ex, PT *__restrict__ Ey,
+         PT *__restrict__ Ez)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    device_geometry(geom, lx_, ly_, lz_, dx_, dy_, dz_);
     const C lx = (C)(MT)lx_, ly = (C)(MT)ly_, lz = (C)(MT)lz_;
     const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
     const C nxn = ((C)px[p] - lx) / ddx;
@@ -170,13 +189,14 @@ __global__ void __launch_bounds__(256)
 k_deposit_vjp(const MT *__restrict__ g, const PT *__restrict__ px, const PT *__restrict__ py,
               const PT *__restrict__ pz, const PT *__restrict__ pq, int64_t n, int nx, int ny, int nz,
               double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
-              PT *__restrict__ gq, PT *__restrict__ gx, PT *__restrict__ gy, PT *__restrict__ gz,
-              int accumulate)
+              const double *__restrict__ geom, PT *__restrict__ gq, PT *__restrict__ gx,
+              PT *__restrict__ gy, PT *__restrict__ gz, int accumulate)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    device_geometry(geom, lx_, ly_, lz_, dx_, dy_, dz_);
     const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
     const C nxn = ((C)px[p] - (C)(MT)lx_) / ddx;
     const C nyn = ((C)py[p] - (C)(MT)ly_) / ddy;
@@ -212,12 +232,14 @@ k_interp_vjp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict
              const PT *__restrict__ py, const PT *__restrict__ pz, const PT *__restrict__ gEx,
              const PT *__restrict__ gEy, const PT *__restrict__ gEz, int64_t n, int nx, int ny,
              int nz, double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
-             PT *__restrict__ gx, PT *__restrict__ gy, PT *__restrict__ gz, int accumulate)
+             const double *__restrict__ geom, PT *__restrict__ gx, PT *__restrict__ gy,
+             PT *__restrict__ gz, int accumulate)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    device_geometry(geom, lx_, ly_, lz_, dx_, dy_, dz_);
     const C ddx = (C)(MT)dx_, ddy = (C)(MT)dy_, ddz = (C)(MT)dz_;
     const C nxn = ((C)px[p] - (C)(MT)lx_) / ddx;
     const C nyn = ((C)py[p] - (C)(MT)ly_) / ddy;
@@ -334,6 +356,39 @@ __global__ void __launch_bounds__(256) k_bounds_final(double *__restrict__ part,
     if (threadIdx.x < 6) part[(size_t)nblk * 6 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
+// Padded bounds from the reduced extrema (fin: min x, y, z, max x, y, z):
+// the Mesh3D particle-extrema rules of src/mesh.jl:120-156 -- first spacing
+// in the particle type, Float64 padding by 1e-6 of it, spacing again, the
+// zero patch, conversion to the mesh type -- with the same operations in the
+// same order as the host path (no contraction), so both give the same bits.
+// One lane per axis.
+__global__ void __launch_bounds__(64)
+k_bounds_geom(const double *__restrict__ fin, double *__restrict__ geom, int pdt, int mdt,
+              long long g0, long long g1, long long g2)
+{
+#pragma clang fp contract(off)
+    const int a = threadIdx.x;
+    if (a >= 3) return;
+    const double mn = fin[a], mx = fin[3 + a];
+    const double nm1 = (double)((a == 0 ? g0 : a == 1 ? g1 : g2) - 1);
+    double d0;
+    if (pdt == SC_F32) d0 = (double)(((float)mx - (float)mn) / (float)nm1);
+    else d0 = (mx - mn) / nm1;
+    const double l = mn - 1e-6 * d0;
+    const double h = mx + 1e-6 * d0;
+    double d = (h - l) / nm1;
+    if (d == 0.0) d = 1e-6;
+    if (mdt == SC_F32) {
+        geom[a] = (double)(float)l;
+        geom[3 + a] = (double)(float)h;
+        geom[6 + a] = (double)(float)d;
+    } else {
+        geom[a] = l;
+        geom[3 + a] = h;
+        geom[6 + a] = d;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_clear(uint32_t *__restrict__ p, int64_t nwords)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -343,14 +398,33 @@ __global__ void __launch_bounds__(256) k_clear(uint32_t *__restrict__ p, int64_t
 
 int64_t blocks_for(int64_t n, int bs) { return (n + bs - 1) / bs; }
 
+// Bounds for a launch: the host values, or (lo == NULL) the mesh's
+// device-resident geometry, which the kernel reads itself.
+struct LaunchGeo {
+    double l[3], d[3];
+    const double *dev;
+};
+
+LaunchGeo launch_geo(const Mesh *m, const double *lo, const double *dl)
+{
+    LaunchGeo h;
+    h.dev = lo ? nullptr : m->geom;
+    for (int a = 0; a < 3; ++a) {
+        h.l[a] = lo ? lo[a] : 0.0;
+        h.d[a] = lo ? dl[a] : 1.0;
+    }
+    return h;
+}
+
 template <typename PT, typename MT>
 void dep(Mesh *m, const void *x, const void *y, const void *z, const void *q, int64_t n,
          const double lo[3], const double dl[3], void *out)
 {
+    const LaunchGeo h = launch_geo(m, lo, dl);
     hipLaunchKernelGGL((k_deposit<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                        m->ctx->stream, (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q, n,
-                       (MT *)out, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1], lo[2],
-                       dl[0], dl[1], dl[2]);
+                       (MT *)out, (int)m->n[0], (int)m->n[1], (int)m->n[2], h.l[0], h.l[1], h.l[2],
+                       h.d[0], h.d[1], h.d[2], h.dev);
 }
 
 template <typename MT>
@@ -366,11 +440,12 @@ template <typename PT, typename MT>
 void itp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, const double lo[3],
          const double dl[3], void *Ex, void *Ey, void *Ez)
 {
+    const LaunchGeo h = launch_geo(m, lo, dl);
     hipLaunchKernelGGL((k_interp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                        m->ctx->stream, static_cast<const typename V4<MT>::type *>(m->efi),
                        (const PT *)x, (const PT *)y, (const PT *)z, n, (int)m->n[0], (int)m->n[1],
-                       (int)m->n[2], lo[0], lo[1], lo[2], dl[0], dl[1], dl[2], (PT *)Ex, (PT *)Ey,
-                       (PT *)Ez);
+                       (int)m->n[2], h.l[0], h.l[1], h.l[2], h.d[0], h.d[1], h.d[2], h.dev,
+                       (PT *)Ex, (PT *)Ey, (PT *)Ez);
 }
 
 }  // namespace
@@ -382,10 +457,12 @@ void dep_vjp(Mesh *m, const void *x, const void *y, const void *z, const void *q
              const double lo[3], const double dl[3], const void *g, void *gq, void *gx, void *gy,
              void *gz, int accumulate)
 {
+    const LaunchGeo h = launch_geo(m, lo, dl);
     hipLaunchKernelGGL((k_deposit_vjp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                        m->ctx->stream, (const MT *)g, (const PT *)x, (const PT *)y, (const PT *)z,
-                       (const PT *)q, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1],
-                       lo[2], dl[0], dl[1], dl[2], (PT *)gq, (PT *)gx, (PT *)gy, (PT *)gz, accumulate);
+                       (const PT *)q, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], h.l[0], h.l[1],
+                       h.l[2], h.d[0], h.d[1], h.d[2], h.dev, (PT *)gq, (PT *)gx, (PT *)gy, (PT *)gz,
+                       accumulate);
 }
 
 template <typename PT, typename MT>
@@ -393,11 +470,13 @@ void itp_vjp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, co
              const double dl[3], const void *gEx, const void *gEy, const void *gEz, void *gx,
              void *gy, void *gz, int accumulate)
 {
+    const LaunchGeo h = launch_geo(m, lo, dl);
     hipLaunchKernelGGL((k_interp_vjp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                        m->ctx->stream, static_cast<const typename V4<MT>::type *>(m->efi),
                        (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)gEx, (const PT *)gEy,
-                       (const PT *)gEz, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], lo[0], lo[1],
-                       lo[2], dl[0], dl[1], dl[2], (PT *)gx, (PT *)gy, (PT *)gz, accumulate);
+                       (const PT *)gEz, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], h.l[0], h.l[1],
+                       h.l[2], h.d[0], h.d[1], h.d[2], h.dev, (PT *)gx, (PT *)gy, (PT *)gz,
+                       accumulate);
 }
 
 int ensure_efi(Mesh *m)
@@ -548,8 +627,12 @@ int launch_interp_vjp(Mesh *m, int pdt, const void *x, const void *y, const void
     return SC_OK;
 }
 
-int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,
-                  double mn[3], double mx[3])
+namespace {
+
+// Extrema of the particle coordinates into ctx->red[6*nblk .. 6*nblk+5]
+// (min x, y, z, max x, y, z), stream-ordered; returns nblk through *nb.
+int reduce_extrema(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                   int64_t *nb)
 {
     int64_t nblk = blocks_for(n, 256);
     if (nblk > kRedBlocks) nblk = kRedBlocks;
@@ -569,6 +652,37 @@ int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z
     launch_begin(ctx, K_BOUNDS_FINAL);
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, ctx->stream, part, (int)nblk);
     launch_end(ctx, K_BOUNDS_FINAL);
+    *nb = nblk;
+    return SC_OK;
+}
+
+}  // namespace
+
+int launch_bounds_geom(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n)
+{
+    Ctx *ctx = m->ctx;
+    if (!m->geom && hipMalloc((void **)&m->geom, sizeof(double) * 12) != hipSuccess) {
+        m->geom = nullptr;
+        return set_err(ctx, SC_ERR_NOMEM, "device allocation of the mesh geometry failed");
+    }
+    int64_t nblk = 0;
+    if (int rc = reduce_extrema(ctx, pdt, x, y, z, n, &nblk)) return rc;
+    launch_begin(ctx, K_BOUNDS_FINAL);
+    hipLaunchKernelGGL(k_bounds_geom, dim3(1), dim3(64), 0, ctx->stream,
+                       (const double *)(ctx->red + 6 * nblk), m->geom, pdt, m->dtype,
+                       (long long)m->n[0], (long long)m->n[1], (long long)m->n[2]);
+    launch_end(ctx, K_BOUNDS_FINAL);
+    SC_HIP(ctx, hipGetLastError());
+    m->geom_set = true;
+    return SC_OK;
+}
+
+int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                  double mn[3], double mx[3])
+{
+    int64_t nblk = 0;
+    if (int rc = reduce_extrema(ctx, pdt, x, y, z, n, &nblk)) return rc;
+    double *part = ctx->red;
     double h[6];
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)

@@ -361,9 +361,16 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     const cpx<T> *twz = static_cast<const cpx<T> *>(ws.twT[2]);
 
     // ---- Green spectra ----
-    const double offz = lo[2] + hi[2];  // 2*zmin + (zmax - zmin), free_space.jl:39
-    const double key[6] = {dl[0], dl[1], dl[2], gamma, cath ? offz : 0.0, (double)gmode};
-    const bool reuse = m->green_reuse && ws.g_valid && ws.g_cath == gmode &&
+    // dl == NULL: the geometry lives on the device (sc_mesh_auto_bounds) and
+    // the Green kernels read it there; the host cannot key a cache on it
+    const bool devgeom = dl == nullptr;
+    const double offz = devgeom ? 0.0 : lo[2] + hi[2];  // 2*zmin + (zmax - zmin), free_space.jl:39
+    double key[6] = {0.0, 0.0, 0.0, gamma, 0.0, (double)gmode};
+    if (!devgeom) {
+        for (int a = 0; a < 3; ++a) key[a] = dl[a];
+        key[4] = cath ? offz : 0.0;
+    }
+    const bool reuse = !devgeom && m->green_reuse && ws.g_valid && ws.g_cath == gmode &&
                        std::memcmp(key, ws.g_key, sizeof(key)) == 0;
     // The Green spectra depend on the spacing, gamma and the cathode offset
     // only, not on rho: built on the context's side stream they overlap the
@@ -386,7 +393,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         } else {
             if (int rc = build_green<T>(m, dl, gamma, offz, gmode)) return rc;
         }
-        ws.g_valid = true;
+        ws.g_valid = !devgeom;
         ws.g_cath = gmode;
         std::memcpy(ws.g_key, key, sizeof(key));
     }

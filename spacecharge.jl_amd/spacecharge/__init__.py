@@ -197,6 +197,8 @@ class Mesh3D:
                 ctx, _DT[T], nx, ny, nz, self._rho.data_ptr(), self._ef.data_ptr(), ctypes.byref(h))
         _check(rc, ctx)
         self._h = h
+        self._device_bounds = False
+        self._geom_version = 0
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -223,6 +225,51 @@ class Mesh3D:
     def force_generic_transforms(self, enable: bool = True) -> None:
         """Diagnostic: run every transform on the generic mixed-radix engine."""
         _check(_lib.load().sc_mesh_force_generic_transforms(self._h, int(bool(enable))), None)
+
+    def update_bounds(self, particles_x, particles_y, particles_z) -> None:
+        """Recompute the bounds from the particle extrema with the rules of
+        Mesh3D(grid, x, y, z) (src/mesh.jl:95-174) on the device, without a
+        host round trip (no reference counterpart).  From then on the step
+        functions read the device-resident bounds; ``min_bounds``,
+        ``max_bounds`` and ``delta`` are refreshed only by fetch_bounds()."""
+        lens = [len(a) for a in (particles_x, particles_y, particles_z)]
+        if min(lens) == 0:
+            raise SpaceChargeError("Particle arrays cannot be empty.")
+        if not (lens[0] == lens[1] == lens[2]):
+            raise SpaceChargeError("Particle coordinate arrays must have the same length.")
+        x, y, z = _particles(self, (particles_x, particles_y, particles_z), "xyz")
+        ctx = _ctx(self.device)
+        with torch.cuda.device(self.device):
+            rc = _lib.load().sc_mesh_auto_bounds(self._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
+                                                 z.data_ptr(), x.numel())
+        _check(rc, ctx)
+        self._device_bounds = True
+        self._geom_version += 1
+
+    def fetch_bounds(self):
+        """Copy the device-resident bounds into min_bounds, max_bounds and
+        delta (synchronises) and return them."""
+        lo, hi, dl = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+        ctx = _ctx(self.device)
+        with torch.cuda.device(self.device):
+            rc = _lib.load().sc_mesh_read_bounds(self._h, lo, hi, dl)
+        _check(rc, ctx)
+        self.min_bounds, self.max_bounds, self.delta = tuple(lo), tuple(hi), tuple(dl)
+        return self.min_bounds, self.max_bounds, self.delta
+
+
+def _bounds_ld(mesh: Mesh3D):
+    """(min_bounds, delta) arguments: NULL when the mesh's bounds live on the device."""
+    if mesh._device_bounds:
+        return None, None
+    return _lib.d3(mesh.min_bounds), _lib.d3(mesh.delta)
+
+
+def _bounds_lhd(mesh: Mesh3D):
+    """(min_bounds, max_bounds, delta) arguments, NULL for device-resident bounds."""
+    if mesh._device_bounds:
+        return None, None, None
+    return _lib.d3(mesh.min_bounds), _lib.d3(mesh.max_bounds), _lib.d3(mesh.delta)
 
 
 def _particles(mesh: Mesh3D, arrays, names) -> list:
@@ -254,7 +301,7 @@ def deposit(mesh: Mesh3D, particles_x, particles_y, particles_z, particles_q, *,
     with torch.cuda.device(mesh.device):
         rc = _lib.load().sc_deposit(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                     z.data_ptr(), q.data_ptr(), x.numel(),
-                                    _lib.d3(mesh.min_bounds), _lib.d3(mesh.delta), int(bool(clear)))
+                                    *_bounds_ld(mesh), int(bool(clear)))
     _check(rc, ctx)
 
 
@@ -262,8 +309,8 @@ def solve(mesh: Mesh3D, *, at_cathode: bool = False) -> None:
     """solve! (src/solvers/free_space.jl:14-47): mesh.efield from mesh.rho."""
     ctx = _ctx(mesh.device)
     with torch.cuda.device(mesh.device):
-        rc = _lib.load().sc_solve(mesh._h, _lib.d3(mesh.min_bounds), _lib.d3(mesh.max_bounds),
-                                  _lib.d3(mesh.delta), float(mesh.gamma), int(bool(at_cathode)))
+        rc = _lib.load().sc_solve(mesh._h, *_bounds_lhd(mesh), float(mesh.gamma),
+                                  int(bool(at_cathode)))
     _check(rc, ctx)
 
 
@@ -283,7 +330,7 @@ def solve_potential(mesh: Mesh3D, out=None):
         raise SpaceChargeError("out must be a contiguous tensor of the mesh's dtype, device and size")
     ctx = _ctx(mesh.device)
     with torch.cuda.device(mesh.device):
-        rc = _lib.load().sc_solve_potential(mesh._h, _lib.d3(mesh.delta), float(mesh.gamma),
+        rc = _lib.load().sc_solve_potential(mesh._h, _bounds_ld(mesh)[1], float(mesh.gamma),
                                             out.data_ptr())
     _check(rc, ctx)
     return out.as_strided((nx, ny, nz), (1, nx, nx * ny))
@@ -299,9 +346,8 @@ def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
     ctx = _ctx(mesh.device)
     with torch.cuda.device(mesh.device):
         rc = _lib.load().sc_interpolate(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
-                                        z.data_ptr(), x.numel(), _lib.d3(mesh.min_bounds),
-                                        _lib.d3(mesh.delta), Ex.data_ptr(), Ey.data_ptr(),
-                                        Ez.data_ptr())
+                                        z.data_ptr(), x.numel(), *_bounds_ld(mesh),
+                                        Ex.data_ptr(), Ey.data_ptr(), Ez.data_ptr())
     _check(rc, ctx)
     return Ex, Ey, Ez
 
@@ -341,7 +387,7 @@ def deposit_adjoint(mesh: Mesh3D, particles_x, particles_y, particles_z, particl
     with torch.cuda.device(mesh.device):
         rc = _lib.load().sc_deposit_adjoint(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                             z.data_ptr(), q.data_ptr(), x.numel(),
-                                            _lib.d3(mesh.min_bounds), _lib.d3(mesh.delta),
+                                            *_bounds_ld(mesh),
                                             g.data_ptr(), gq.data_ptr(), ptr(gx), ptr(gy), ptr(gz), 0)
     _check(rc, ctx)
     return gq, gx, gy, gz
@@ -355,8 +401,7 @@ def solve_adjoint(mesh: Mesh3D, g_efield, *, at_cathode: bool = False):
     out = torch.empty(math.prod(mesh.grid_size), dtype=mesh.rho.dtype, device=mesh.device)
     ctx = _ctx(mesh.device)
     with torch.cuda.device(mesh.device):
-        rc = _lib.load().sc_solve_adjoint(mesh._h, _lib.d3(mesh.min_bounds), _lib.d3(mesh.max_bounds),
-                                          _lib.d3(mesh.delta), float(mesh.gamma),
+        rc = _lib.load().sc_solve_adjoint(mesh._h, *_bounds_lhd(mesh), float(mesh.gamma),
                                           int(bool(at_cathode)), g.data_ptr(), out.data_ptr())
     _check(rc, ctx)
     return _mesh_views(out, mesh, 1)
@@ -378,8 +423,8 @@ def interpolate_adjoint(mesh: Mesh3D, particles_x, particles_y, particles_z, g_E
     ctx = _ctx(mesh.device)
     with torch.cuda.device(mesh.device):
         rc = _lib.load().sc_interpolate_adjoint(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
-                                                z.data_ptr(), x.numel(), _lib.d3(mesh.min_bounds),
-                                                _lib.d3(mesh.delta), gE[0].data_ptr(),
+                                                z.data_ptr(), x.numel(), *_bounds_ld(mesh),
+                                                gE[0].data_ptr(),
                                                 gE[1].data_ptr(), gE[2].data_ptr(), gef.data_ptr(),
                                                 ptr(gx), ptr(gy), ptr(gz), 0)
     _check(rc, ctx)
@@ -396,6 +441,8 @@ def _batch_device(meshes) -> torch.device:
         raise SpaceChargeError("all meshes of a batch must be on one device")
     if len({id(m) for m in meshes}) != len(meshes):
         raise SpaceChargeError("a mesh appears twice in one batch")
+    if any(m._device_bounds for m in meshes):
+        raise SpaceChargeError("the batched calls take host bounds (call fetch_bounds() first)")
     return dev
 
 

@@ -45,6 +45,8 @@ PROTOTYPES = {
     "sc_mesh_set_green_reuse": (_i, [_vp, _i]),
     "sc_mesh_force_generic_transforms": (_i, [_vp, _i]),
     "sc_bounds_from_particles": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pi64, _i, _pd, _pd, _pd]),
+    "sc_mesh_auto_bounds": (_i, [_vp, _i, _vp, _vp, _vp, _i64]),
+    "sc_mesh_read_bounds": (_i, [_vp, _pd, _pd, _pd]),
     "sc_clear_mesh": (_i, [_vp]),
     "sc_deposit": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64, _pd, _pd, _i]),
     "sc_solve": (_i, [_vp, _pd, _pd, _pd, _d, _i]),

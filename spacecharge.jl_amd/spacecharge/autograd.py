@@ -30,7 +30,9 @@ __all__ = ["deposit", "solve", "interpolate", "step"]
 
 
 def _geometry(mesh):
-    return (tuple(mesh.min_bounds), tuple(mesh.max_bounds), tuple(mesh.delta), float(mesh.gamma))
+    # device-resident bounds (Mesh3D.update_bounds) are identified by their version
+    return (tuple(mesh.min_bounds), tuple(mesh.max_bounds), tuple(mesh.delta), float(mesh.gamma),
+            mesh._device_bounds, mesh._geom_version)
 
 
 def _check_geometry(ctx):
