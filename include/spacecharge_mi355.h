@@ -176,6 +176,20 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
                    int64_t n, const double min_bounds[3], const double delta[3], void *Ex,
                    void *Ey, void *Ez);
 
+/* Tile order of the particles (no reference counterpart): perm[i] is the
+ * index of the particle that goes to position i when the particles are
+ * grouped by the 16^3-cell tile of the deposit that holds them (tiles in
+ * x-fastest order; the order inside a tile is unspecified).  A tracking
+ * code applies it to all its per-particle arrays every few steps: deposit
+ * and interpolation on tile-ordered particles touch the mesh in local
+ * order, which makes the interpolation's gathers and the deposit's scatter
+ * cheaper.  The step's results do not depend on the particle order
+ * (Float32 meshes bit for bit).  Bounds as for sc_deposit (NULL for the
+ * mesh's device-resident ones).  perm: n int32 on the device. */
+int sc_sort_permutation(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                        int64_t n, const double min_bounds[3], const double delta[3],
+                        int32_t *perm);
+
 /* ---- vector-Jacobian products (no reference counterpart: the reference has
  * no derivatives; these serve differentiable tracking, e.g. a ChainRulesCore
  * rrule per function) -----------------------------------------------------

@@ -516,6 +516,21 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
     return launch_interp(&M, pdtype, x, y, z, n, lo, dl, Ex, Ey, Ez);
 }
 
+int sc_sort_permutation(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,
+                        int64_t n, const double lo[3], const double dl[3], int32_t *perm)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+    if (n > 0 && (!x || !y || !z || !perm)) return set_err(c, SC_ERR_INVALID, "NULL pointer");
+    if (int rc = check_geom_m(c, M, lo, dl)) return rc;
+    if (int rc = check_dev(c)) return rc;
+    if (n == 0) return SC_OK;
+    return launch_sort_perm(&M, pdtype, x, y, z, n, lo, dl, perm);
+}
+
 // ---- vector-Jacobian products (no reference counterpart) ----
 static int check_pos_outputs(Ctx *c, const void *gx, const void *gy, const void *gz)
 {

@@ -367,6 +367,41 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
         if (ext_lds[t]) atomicOr(&ext_flags[t], 1u);  // rare: particles outside the mesh
 }
 
+// The particle order of the binning as a permutation: perm[slot] = particle
+// index, slots in tile order (the same cursors as k_bin_scatter).
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(kBinThreads)
+k_bin_perm(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz, Split sp,
+           Geo g, const unsigned *__restrict__ cmat, const unsigned *__restrict__ start,
+           int *__restrict__ perm)
+{
+    resolve_geo(g);
+    __shared__ unsigned cur[kMaxTiles];
+    const int ntiles = g.tx * g.ty * g.tz;
+    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
+        cur[t] = start[t] + cmat[(size_t)t * sp.G + blockIdx.x];
+    __syncthreads();
+    const int64_t first = (int64_t)blockIdx.x * sp.slice;
+    const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
+    for (int64_t base = first; base < end; base += kBatch * kBinThreads) {
+        int tl[kBatch];
+        bool ok[kBatch];
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
+            int64_t p = base + s * kBinThreads + threadIdx.x;
+            ok[s] = p < end;
+            p = ok[s] ? p : end - 1;
+            tl[s] = tile_of<PT, MT>(px[p], py[p], pz[p], g);
+        }
+        unsigned slot[kBatch];
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) slot[s] = atomicAdd(&cur[tl[s]], ok[s] ? 1u : 0u);
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s)
+            if (ok[s]) perm[slot[s]] = (int)(base + s * kBinThreads + threadIdx.x);
+    }
+}
+
 // Charges of another channel into the records (same tile order).
 template <typename PT>
 __global__ void __launch_bounds__(256)
@@ -709,6 +744,70 @@ void free_bins(Mesh *m)
     if (B.cmat) (void)hipFree(B.cmat);
     if (B.qs) (void)hipFree(B.qs);
     B = DepositBins();
+}
+
+namespace {
+
+template <typename PT, typename MT>
+int perm_binned(Mesh *m, const void *x, const void *y, const void *z, int64_t n, const Geo &g,
+                int *perm)
+{
+    Ctx *ctx = m->ctx;
+    hipStream_t st = ctx->stream;
+    DepositBins &B = m->bins;
+    const int ntiles = g.tx * g.ty * g.tz;
+    unsigned *counts = B.counts, *cursor = B.counts + kMaxTiles;
+    unsigned *cbase = B.counts + 2 * kMaxTiles, *ext = B.counts + 3 * kMaxTiles;
+    unsigned *nch = B.counts + 4 * kMaxTiles;
+    int4 *chunks = static_cast<int4 *>(B.chunks);
+    const Split sp = split_of(n);
+    launch_begin(ctx, K_BIN_COUNT);
+    hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
+                       (const PT *)y, (const PT *)z, sp, g, B.cmat);
+    launch_end(ctx, K_BIN_COUNT);
+    launch_begin(ctx, K_BIN_ROWSCAN);
+    hipLaunchKernelGGL(k_bin_rowscan, dim3((unsigned)div_up4(ntiles)), dim3(256), 0, st, B.cmat,
+                       sp.G, ntiles, counts);
+    launch_end(ctx, K_BIN_ROWSCAN);
+    launch_begin(ctx, K_BIN_SCAN);
+    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
+                       cbase, chunks, nch, ext, ntiles);
+    launch_end(ctx, K_BIN_SCAN);
+    launch_begin(ctx, K_BIN_SCATTER);
+    hipLaunchKernelGGL((k_bin_perm<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
+                       (const PT *)y, (const PT *)z, sp, g, (const unsigned *)B.cmat,
+                       (const unsigned *)cursor, perm);
+    launch_end(ctx, K_BIN_SCATTER);
+    SC_HIP(ctx, hipGetLastError());
+    return SC_OK;
+}
+
+}  // namespace
+
+int launch_sort_perm(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                     const double lo[3], const double dl[3], int *perm)
+{
+    Geo g;
+    g.nx = (int)m->n[0];
+    g.ny = (int)m->n[1];
+    g.nz = (int)m->n[2];
+    g.tx = (g.nx - 1 + kTile - 1) / kTile;
+    g.ty = (g.ny - 1 + kTile - 1) / kTile;
+    g.tz = (g.nz - 1 + kTile - 1) / kTile;
+    if ((int64_t)g.tx * g.ty * g.tz > kMaxTiles)
+        return set_err(m->ctx, SC_ERR_UNSUPPORTED, "the mesh has too many tiles to sort by");
+    if (n > 0x7fffffffLL) return set_err(m->ctx, SC_ERR_INVALID, "too many particles to sort");
+    g.dev = lo ? nullptr : m->geom;
+    for (int a = 0; a < 3; ++a) {
+        g.lo[a] = lo ? lo[a] : 0.0;
+        g.dl[a] = lo ? dl[a] : 1.0;
+    }
+    if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz)) return rc;
+    if (pdt == SC_F32 && m->dtype == SC_F32) return perm_binned<float, float>(m, x, y, z, n, g, perm);
+    if (pdt == SC_F64 && m->dtype == SC_F64)
+        return perm_binned<double, double>(m, x, y, z, n, g, perm);
+    if (pdt == SC_F64 && m->dtype == SC_F32) return perm_binned<double, float>(m, x, y, z, n, g, perm);
+    return perm_binned<float, double>(m, x, y, z, n, g, perm);
 }
 
 int launch_deposit_binned_multi(Mesh *m, int pdt, const void *x, const void *y, const void *z,

@@ -208,6 +208,9 @@ int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const 
 int launch_deposit_binned_multi(Mesh *m, int pdt, const void *x, const void *y, const void *z,
                                 const void *const *q, int nch, int64_t n, const double lo[3],
                                 const double dl[3], int clear, void *const *out);
+// Tile-order permutation of the particles (perm[slot] = particle index).
+int launch_sort_perm(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                     const double lo[3], const double dl[3], int *perm);
 // Vector-Jacobian products of deposit (g_rho -> g_q, g_x, g_y, g_z) and of
 // interpolation (g_E -> g_efield, g_x, g_y, g_z) at the mesh's current
 // efield; NULL outputs are skipped; position gradients are added to the

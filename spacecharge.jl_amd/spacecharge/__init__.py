@@ -44,6 +44,7 @@ __all__ = [
     "solve",
     "solve_potential",
     "interpolate_field",
+    "sort_permutation",
     "deposit_adjoint",
     "solve_adjoint",
     "interpolate_adjoint",
@@ -350,6 +351,23 @@ def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
                                         Ex.data_ptr(), Ey.data_ptr(), Ez.data_ptr())
     _check(rc, ctx)
     return Ex, Ey, Ez
+
+
+def sort_permutation(mesh: Mesh3D, particles_x, particles_y, particles_z) -> torch.Tensor:
+    """Tile order of the particles on this mesh (no reference counterpart):
+    an int32 tensor ``perm`` such that ``x[perm]`` groups the particles by the
+    deposit's 16^3-cell tiles.  Applied to every per-particle array every
+    few tracking steps, it makes the deposit and interpolation touch the mesh
+    in local order; the step's results do not depend on the order."""
+    x, y, z = _particles(mesh, (particles_x, particles_y, particles_z), "xyz")
+    perm = torch.empty(x.numel(), dtype=torch.int32, device=mesh.device)
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_sort_permutation(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
+                                             z.data_ptr(), x.numel(), *_bounds_ld(mesh),
+                                             perm.data_ptr())
+    _check(rc, ctx)
+    return perm
 
 
 # ---- vector-Jacobian products (no reference counterpart) ----
