@@ -142,6 +142,18 @@ def bench_batched(args, sc, torch, dev, T):
 
     ms = timed(step_batched)
     ms_loop = timed(step_loop)
+    # the same batched step captured once into a graph and replayed (the
+    # workspace, bins and scratch exist after the warmup above)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step_batched()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step_batched()
+    ms_graph = timed(graph.replay)
     return {
         "metric": "particles/s per full space-charge step (deposit+solve+interpolate), many bunches",
         "value": ntot / (ms * 1e-3),
@@ -166,6 +178,8 @@ def bench_batched(args, sc, torch, dev, T):
         },
         "sequential_ms_per_step": ms_loop,
         "batched_speedup": ms_loop / ms,
+        "graph_ms_per_step": ms_graph,
+        "graph_particles_per_s": ntot / (ms_graph * 1e-3),
     }
 
 

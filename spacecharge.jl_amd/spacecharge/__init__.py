@@ -203,8 +203,10 @@ class Mesh3D:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and _lib._lib is not None:
-            _lib._lib.sc_mesh_destroy(h)
+        # at interpreter exit the module globals may already be gone
+        lib = getattr(_lib, "_lib", None) if _lib is not None else None
+        if h is not None and lib is not None:
+            lib.sc_mesh_destroy(h)
             self._h = None
 
     def __repr__(self) -> str:  # mirrors Base.show (src/mesh.jl:240-246)
