@@ -22,8 +22,9 @@ the same bunches through a loop of single-bunch calls.
 particles (sigma = 1 mm), 64^3 mesh with fixed bounds (+-8 sigma); it times
 one forward step and one vector-Jacobian product of the step (interpolation,
 solve and deposit adjoints), and a 100-step drift-kick tracking run through
-torch.autograd whose loss (the final rms size) is differentiated with respect
-to the initial positions.
+torch.autograd (segments of 10 steps checkpointed) whose loss, the final rms
+emittance, is differentiated with respect to the initial positions and
+charges.
 
 Prints one JSON line.  `value` is particles/s of the whole job (one GPU).
 The `roofline` object is for the kernel with the largest share of the step
@@ -221,40 +222,35 @@ def bench_adjoint(args, sc, torch, dev, T):
     fwd()
     ms_vjp = timed(vjp, args.steps)
 
-    # drift-kick tracking: x += dt p, p += k E (normalised units), loss = rms size
-    nsteps = 100
+    # drift-kick tracking (normalised units: p += kick E, x += dt p) through
+    # the autograd layer, segments of 10 steps checkpointed, loss = final rms
+    # emittance, gradients with respect to the initial positions and charges
+    nsteps, every = 100, 10
     dt, kick = 1.0, 1e-3 * sig / 1e7  # a kick of ~1e-3 sigma per step at 1e7 V/m
 
-    def track():
-        xs, ys, zs = (t.clone().requires_grad_(True) for t in (x, y, z))
-        px = torch.zeros_like(xs)
-        py = torch.zeros_like(xs)
-        pz = torch.zeros_like(xs)
-        cx, cy, cz = xs, ys, zs
-        for _ in range(nsteps):
-            Ex, Ey, Ez = scad.step(mesh, cx, cy, cz, q)
-            px, py, pz = px + kick * Ex, py + kick * Ey, pz + kick * Ez
-            cx, cy, cz = cx + dt * px, cy + dt * py, cz + dt * pz
-        loss = (cx.double().square() + cy.double().square() + cz.double().square()).mean()
-        loss.backward()
-        return xs.grad
+    def track(grad, ckpt):
+        xs, ys, zs, qs = (t.clone().requires_grad_(grad) for t in (x, y, z, q))
+        zero = torch.zeros_like(xs)
+        with torch.set_grad_enabled(grad):
+            st = scad.track(mesh, xs, ys, zs, zero, zero, zero, qs, nsteps, dt, kick,
+                            checkpoint_every=ckpt)
+            loss = scad.rms_emittance(*st)
+        if grad:
+            loss.backward()
+            return xs.grad, qs.grad
+        return None, None
 
-    track()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    grad = track()
-    torch.cuda.synchronize()
-    ms_track = (time.perf_counter() - t0) * 1e3
-    with torch.no_grad():
-        t2 = time.perf_counter()
-        cx, cy, cz = x.clone(), y.clone(), z.clone()
-        px, py, pz = (torch.zeros_like(x) for _ in range(3))
-        for _ in range(nsteps):
-            Ex, Ey, Ez = scad.step(mesh, cx, cy, cz, q)
-            px, py, pz = px + kick * Ex, py + kick * Ey, pz + kick * Ez
-            cx, cy, cz = cx + dt * px, cy + dt * py, cz + dt * pz
+    def timed_track(grad, ckpt):
+        track(grad, ckpt)
         torch.cuda.synchronize()
-        ms_track_fwd = (time.perf_counter() - t2) * 1e3
+        t0 = time.perf_counter()
+        out = track(grad, ckpt)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3, out
+
+    ms_track_fwd, _ = timed_track(False, every)
+    ms_track, (gx, gq) = timed_track(True, every)
+    ms_track_all, _ = timed_track(True, 0)
     return {
         "metric": "particles/s per differentiable space-charge step (forward + vector-Jacobian product)",
         "value": n / ((ms_fwd + ms_vjp) * 1e-3),
@@ -281,10 +277,14 @@ def bench_adjoint(args, sc, torch, dev, T):
         "vjp_over_forward": ms_vjp / ms_fwd,
         "tracking": {
             "steps": nsteps,
+            "checkpoint_every": every,
+            "loss": "final rms emittance (sum over planes)",
             "forward_only_ms": ms_track_fwd,
             "forward_and_backward_ms": ms_track,
             "grad_over_forward": ms_track / ms_track_fwd,
-            "grad_finite": bool(torch.isfinite(grad).all()),
+            "forward_and_backward_ms_every_step_stored": ms_track_all,
+            "grad_over_forward_every_step_stored": ms_track_all / ms_track_fwd,
+            "grad_finite": bool(torch.isfinite(gx).all() and torch.isfinite(gq).all()),
         },
     }
 
@@ -353,9 +353,12 @@ def measured_traffic(name, n, grid, dtype_name, cathode):
     return None, None
 
 
-def cpu_baseline(args, parts, grid, dtype_name, max_seconds):
+def cpu_baseline(args, parts, grid, dtype_name, max_seconds, gpu=None):
     """Time the CPU oracle on a bounded sample: whole steps of the same workload
-    (the same particles, copied to the host)."""
+    (the same particles, copied to the host).  With ``gpu`` = (mesh, (Ex, Ey,
+    Ez)) of the GPU's last step, the oracle also checks it: one Float64 step
+    on the GPU mesh's own bounds, and the relative L2 error of the GPU field
+    (mesh nodes and particles) per component."""
     import numpy as np
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as orc
@@ -363,6 +366,25 @@ def cpu_baseline(args, parts, grid, dtype_name, max_seconds):
     x, y, z, q = (t.cpu().numpy() for t in parts)
     n = x.shape[0]
     threads = os.cpu_count() or 1
+    accuracy = None
+    if gpu is not None:
+        mesh, E_gpu = gpu
+        lo, hi, dl = mesh.min_bounds, mesh.max_bounds, mesh.delta
+        x64, y64, z64, q64 = (a.astype(np.float64) for a in (x, y, z, q))
+        rho = orc.deposit(x64, y64, z64, q64, grid, lo, dl, np.float64)
+        ef = orc.solve(rho, lo, hi, dl, float(mesh.gamma), bool(args.cathode), nthreads=threads)
+        E_ref = orc.interpolate(ef, x64, y64, z64, lo, dl, nthreads=threads)
+        ef_gpu = mesh._ef.cpu().numpy().astype(np.float64).reshape(tuple(grid) + (3,), order="F")
+
+        def rel(a, b):
+            return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+        accuracy = {
+            "reference": "oracle Float64 step on the same particles and the GPU mesh's bounds",
+            "mesh_field_rel_l2": [rel(ef_gpu[..., c], ef[..., c]) for c in range(3)],
+            "particle_field_rel_l2": [rel(E_gpu[c].cpu().numpy().astype(np.float64), E_ref[c])
+                                      for c in range(3)],
+        }
     lo, hi, dl = orc.bounds_auto(x, y, z, grid, npdt)
     times = []
     t_start = time.perf_counter()
@@ -385,6 +407,7 @@ def cpu_baseline(args, parts, grid, dtype_name, max_seconds):
                    f"{'cathode' if args.cathode else 'free space'}), "
                    f"min {best * 1e3:.1f} ms, median {sorted(times)[len(times) // 2] * 1e3:.1f} ms; "
                    "deposit single-threaded, FFT/Green/interpolate on all cores"),
+        "accuracy_vs_fp64_oracle": accuracy,
     }
 
 
@@ -480,7 +503,7 @@ def main():
     with sc.KernelTimer(dev) as kt:
         t1 = time.perf_counter()
         for _ in range(args.steps):
-            step()
+            E_last = step()
         torch.cuda.synchronize()
         ms_instr = (time.perf_counter() - t1) * 1e3 / args.steps
     per_kernel = kt.results()
@@ -519,7 +542,8 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(args, (x, y, z, q), grid, args.dtype, args.cpu_seconds)
+        cpu = cpu_baseline(args, (x, y, z, q), grid, args.dtype, args.cpu_seconds,
+                           gpu=(mesh, E_last))
 
     out = {
         "metric": "particles/s per full space-charge step (deposit+solve+interpolate)",

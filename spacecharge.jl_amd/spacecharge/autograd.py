@@ -22,11 +22,12 @@ before it runs.
 from __future__ import annotations
 
 import torch
+import torch.utils.checkpoint
 
 from . import (SpaceChargeError, deposit as _deposit, deposit_adjoint, interpolate_adjoint,
                interpolate_field, solve as _solve, solve_adjoint)
 
-__all__ = ["deposit", "solve", "interpolate", "step"]
+__all__ = ["deposit", "solve", "interpolate", "step", "track", "rms_emittance"]
 
 
 def _geometry(mesh):
@@ -112,3 +113,45 @@ def step(mesh, x, y, z, q, at_cathode: bool = False):
     rho = deposit(mesh, x, y, z, q)
     efield = solve(mesh, rho, at_cathode)
     return interpolate(mesh, efield, x, y, z)
+
+
+def track(mesh, x, y, z, px, py, pz, q, nsteps: int, dt: float, kick: float,
+          checkpoint_every: int = 10, at_cathode: bool = False):
+    """Drift-kick tracking through the space-charge field, differentiable in
+    the initial state and the charges: each step kicks p += kick * E(x)
+    (the field of the current positions, mesh geometry fixed) and drifts
+    x += dt * p.  With checkpoint_every = k > 0 and gradients enabled, the
+    steps run in segments of k under torch.utils.checkpoint: the backward
+    pass recomputes each segment from its stored starting state instead of
+    keeping every step's particles and field.  Returns (x, y, z, px, py, pz)."""
+
+    def segment(k):
+        def run(x, y, z, px, py, pz):
+            for _ in range(k):
+                Ex, Ey, Ez = step(mesh, x, y, z, q, at_cathode)
+                px, py, pz = px + kick * Ex, py + kick * Ey, pz + kick * Ez
+                x, y, z = x + dt * px, y + dt * py, z + dt * pz
+            return x, y, z, px, py, pz
+        return run
+
+    state = (x, y, z, px, py, pz)
+    done = 0
+    while done < nsteps:
+        k = min(checkpoint_every, nsteps - done) if checkpoint_every > 0 else nsteps - done
+        if checkpoint_every > 0 and torch.is_grad_enabled():
+            state = torch.utils.checkpoint.checkpoint(segment(k), *state, use_reentrant=False)
+        else:
+            state = segment(k)(*state)
+        done += k
+    return state
+
+
+def rms_emittance(x, y, z, px, py, pz):
+    """Sum over the three planes of the rms emittance
+    sqrt(<u^2><p^2> - <u p>^2) (centred moments), in Float64."""
+    total = 0.0
+    for u, p in ((x, px), (y, py), (z, pz)):
+        u = u.double() - u.double().mean()
+        p = p.double() - p.double().mean()
+        total = total + torch.sqrt((u * u).mean() * (p * p).mean() - (u * p).mean() ** 2)
+    return total

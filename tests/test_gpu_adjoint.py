@@ -266,3 +266,54 @@ def test_adjoint_argument_errors():
                                   gx.data_ptr(), 0) != 0
     with pytest.raises(sc.SpaceChargeError, match="shape"):
         sc.solve_adjoint(mesh, torch.zeros(8, 8, 8, dtype=torch.float64, device=DEV))
+
+
+def test_tracking_gradient_matches_central_differences():
+    """100 drift-kick steps (Float64, 32^3 mesh with fixed bounds, segments of
+    10 steps under checkpointing), loss = final rms emittance: the autograd
+    gradient with respect to initial positions and charges against central
+    differences of the whole run (steps of 1e-6 of the bunch size and of the
+    charge), and against the same run without checkpointing."""
+    sc = _sc()
+    import spacecharge.autograd as scad
+    n, sig = 2000, 1e-3
+    x, y, z, q = _bunch(n, 42, sigma=(sig, sig, sig))
+    q = q * torch.linspace(0.5, 1.5, n, dtype=torch.float64, device=DEV)
+    mesh = sc.Mesh3D((32, 32, 32), (-6 * sig,) * 3, (6 * sig,) * 3, T=torch.float64)
+    nsteps, dt = 100, 1.0
+    kick = 1e-4 * sig / 9e6  # ~1e-4 sigma of momentum per step at the bunch's peak field
+
+    def loss_of(x0, y0, z0, q0, every):
+        zero = torch.zeros_like(x0)
+        st = scad.track(mesh, x0, y0, z0, zero, zero, zero, q0, nsteps, dt, kick,
+                        checkpoint_every=every)
+        return scad.rms_emittance(*st)
+
+    xs, ys, zs, qs = (t.clone().requires_grad_(True) for t in (x, y, z, q))
+    L = loss_of(xs, ys, zs, qs, 10)
+    L.backward()
+    grads = [xs.grad.clone(), ys.grad.clone(), zs.grad.clone(), qs.grad.clone()]
+    xs2, ys2, zs2, qs2 = (t.clone().requires_grad_(True) for t in (x, y, z, q))
+    loss_of(xs2, ys2, zs2, qs2, 0).backward()
+    for a, b in zip(grads, (xs2.grad, ys2.grad, zs2.grad, qs2.grad)):
+        assert float((a - b).abs().max()) <= 1e-10 * float(b.abs().max())
+    # particles at least 5% of a cell from every face
+    pos = [t.cpu().numpy() for t in (x, y, z)]
+    frac = [((pos[a] - mesh.min_bounds[a]) / mesh.delta[a]) % 1.0 for a in range(3)]
+    ok = np.flatnonzero(np.all([(f > 0.05) & (f < 0.95) for f in frac], axis=0))
+    worst = 0.0
+    with torch.no_grad():
+        for p in ok[[0, ok.size // 3, 2 * ok.size // 3]]:
+            p = int(p)
+            for a in range(4):
+                h = 1e-6 * (sig if a < 3 else float(q[p]))
+                args_up = [x.clone(), y.clone(), z.clone(), q.clone()]
+                args_dn = [x.clone(), y.clone(), z.clone(), q.clone()]
+                args_up[a][p] += h
+                args_dn[a][p] -= h
+                fd = (float(loss_of(*args_up, 0)) - float(loss_of(*args_dn, 0))) / (2 * h)
+                g = float(grads[a][p])
+                err = abs(fd - g) / abs(g)
+                worst = max(worst, err)
+                assert err < 1e-4, (p, a, fd, g)
+    print(f"tracking gradient vs central differences: worst rel err {worst:.2e}")
