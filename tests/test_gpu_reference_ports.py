@@ -291,3 +291,51 @@ def test_analytical_isotropic_gaussian(T):
     err = np.abs(got - ana).max() / np.abs(ana).max()
     print(f"analytical Gaussian {T}: max error {err:.4f}")
     assert err < 0.10
+
+
+@pytest.mark.parametrize("T", [torch.float32, torch.float64])
+@pytest.mark.parametrize("gamma,grid", [(1.0, (128, 128, 128)), (10.0, (128, 128, 512))])
+def test_analytical_anisotropic_gaussian_128(T, gamma, grid):
+    """The Gaussian half of the single-bunch configuration: 2M particles,
+    sigma = (1, 1, 3) mm, 1 nC, auto-bounds.  The field on the three axes
+    against the 1-D integral form, within 1%.  At gamma the lab field is the
+    rest frame's (sigma_z stretched by gamma) with the transverse components
+    multiplied by gamma.  Quasi-random (scrambled Sobol) sampling keeps the
+    shot noise below the discretisation error.  At gamma = 10 the mesh is
+    refined in z: the on-axis Ez of the rest frame's long, thin bunch needs
+    rest-frame z cells no longer than sigma_x (with 128 z cells they are 2.4
+    sigma_x and the reference algorithm is 6% off, converging as dz^2)."""
+    import spacecharge as sc
+    from scipy.stats import norm, qmc
+    from _common import gaussian_field
+    Q, sig = 1e-9, (1e-3, 1e-3, 3e-3)
+    n = 2_000_000
+    u = np.clip(qmc.Sobol(d=3, scramble=True, seed=3).random(n), 1e-12, 1 - 1e-12)
+    g = norm.ppf(u)
+    x, y, z = (g[:, a] * sig[a] for a in range(3))
+    xt, yt, zt = (torch.as_tensor(a, dtype=T, device=DEV) for a in (x, y, z))
+    qt = torch.full_like(xt, Q / n)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=T, gamma=gamma)
+    sc.deposit(mesh, xt, yt, zt, qt)
+    sc.solve(mesh)
+    ef = mesh_to_np(mesh._ef, grid).astype(np.float64)
+    lo, dl = mesh.min_bounds, mesh.delta
+    ix = [int(np.argmin(np.abs(lo[a] + np.arange(grid[a]) * dl[a]))) for a in range(3)]
+    rest_sig = (sig[0], sig[1], gamma * sig[2])
+    worst = 0.0
+    for c in range(3):
+        idx = list(ix)
+        pts = []
+        for k in range(grid[c]):
+            idx[c] = k
+            pts.append([lo[a] + idx[a] * dl[a] for a in range(3)])
+        pts = np.array(pts)
+        rest = pts * np.array([1.0, 1.0, gamma])
+        ana = gaussian_field(rest, rest_sig, Q)[:, c] * (gamma if c < 2 else 1.0)
+        sl = list(ix)
+        sl[c] = slice(None)
+        got = ef[tuple(sl) + (c,)]
+        err = np.abs(got - ana).max() / np.abs(ana).max()
+        worst = max(worst, err)
+        print(f"anisotropic Gaussian {grid} {T} gamma={gamma} axis {c}: max err {err:.4f}")
+    assert worst < 0.01

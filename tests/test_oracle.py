@@ -9,8 +9,10 @@ code or formula:
                            the antiderivative (golden file); the raw last plane.
   * solve                  brute-force direct sum with cells integrated by scipy;
                            the analytical isotropic Gaussian
-                           (test/analytical_test.jl); the cathode solve equals a
-                           free-space solve of the explicitly mirrored charge.
+                           (test/analytical_test.jl) and an anisotropic one (its
+                           1-D integral form, which is itself checked against
+                           the isotropic closed form); the cathode solve equals
+                           a free-space solve of the explicitly mirrored charge.
   * deposit / interpolate  hand-computed CIC weights at exact binary positions;
                            the reference's own assertions (test/test_deposition.jl,
                            test/test_interpolation.jl); exactness on linear fields;
@@ -39,7 +41,7 @@ from scipy import integrate
 from scipy.special import erf
 
 import oracle as orc
-from _common import config_a, gaussian, rel_l2
+from _common import config_a, gaussian, gaussian_field, rel_l2
 
 GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "igf_golden.json")))
 PGOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "pot_golden.json")))
@@ -208,6 +210,51 @@ def test_analytical_isotropic_gaussian():
                                              np.exp(-(r / sigma) ** 2 / 2)) * zc
     err = np.abs(ef[ix, iy, :, 2] - ana).max() / np.abs(ana).max()
     assert err < 0.10
+
+
+def test_gaussian_field_formula_reduces_to_isotropic_closed_form():
+    """The anisotropic Gaussian's 1-D integral (tests/_common.py) at equal
+    sizes equals Q/(4 pi eps0 r^2) (erf(r/(sqrt 2 s)) - sqrt(2/pi) r/s e^(-r^2/2s^2))."""
+    Q, s = 1e-9, 1e-3
+    r = np.array([0.2, 0.7, 1.5, 3.0, 6.0]) * s
+    pts = np.stack([r * 0.6, r * 0.0, r * 0.8], axis=1)
+    E = gaussian_field(pts, (s, s, s), Q)
+    rr = np.linalg.norm(pts, axis=1)
+    Er = FPEI * Q / rr ** 2 * (erf(rr / (np.sqrt(2) * s)) -
+                               np.sqrt(2 / np.pi) * rr / s * np.exp(-rr ** 2 / (2 * s * s)))
+    ana = Er[:, None] * pts / rr[:, None]
+    assert np.abs(E - ana).max() <= 1e-12 * np.abs(ana).max()
+
+
+def _axis_fields(ef, lo, dl, grid):
+    """Nodes and field along the mesh lines through the node nearest the origin."""
+    ix = [int(np.argmin(np.abs(lo[a] + np.arange(grid[a]) * dl[a]))) for a in range(3)]
+    out = []
+    for c in range(3):
+        idx = list(ix)
+        pts = []
+        for k in range(grid[c]):
+            idx[c] = k
+            pts.append([lo[a] + idx[a] * dl[a] for a in range(3)])
+        sl = list(ix)
+        sl[c] = slice(None)
+        out.append((np.array(pts), ef[tuple(sl) + (c,)]))
+    return out
+
+
+def test_analytical_anisotropic_gaussian():
+    """1M particles, sigma = (1, 1, 3) mm, 1 nC, 32^3 auto-bounds: the field
+    on the three axes within 10% (as test/analytical_test.jl asks of the
+    isotropic case) of the 1-D integral form."""
+    Q, sig = 1e-9, (1e-3, 1e-3, 3e-3)
+    x, y, z = gaussian(1_000_000, sig, seed=5)
+    q = np.full(x.size, Q / x.size)
+    grid = (32, 32, 32)
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid)
+    ef = orc.solve(orc.deposit(x, y, z, q, grid, lo, dl), lo, hi, dl)
+    for c, (pts, got) in enumerate(_axis_fields(ef, lo, dl, grid)):
+        ana = gaussian_field(pts, sig, Q)[:, c]
+        assert np.abs(got - ana).max() / np.abs(ana).max() < 0.10, c
 
 
 # ---------------------------------------------------------------- deposit / interpolate
