@@ -240,13 +240,18 @@ def bench_adjoint(args, sc, torch, dev, T):
             return xs.grad, qs.grad
         return None, None
 
-    def timed_track(grad, ckpt):
-        track(grad, ckpt)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
+    def timed_track(grad, ckpt, reps=3):
+        # one untimed run, then the median of `reps` timed runs (the loop is
+        # issued from Python, so single runs vary with the host)
         out = track(grad, ckpt)
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3, out
+        times = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = track(grad, ckpt)
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        return sorted(times)[len(times) // 2], out
 
     ms_track_fwd, _ = timed_track(False, every)
     ms_track, (gx, gq) = timed_track(True, every)

@@ -402,15 +402,6 @@ k_bin_perm(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__res
     }
 }
 
-// Charges of another channel into the records (same tile order).
-template <typename PT>
-__global__ void __launch_bounds__(256)
-k_set_q(Rec<PT> *__restrict__ sorted, const PT *__restrict__ qs, int64_t n)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) sorted[i].q = qs[i];
-}
-
 // CIC contributions of one record, in the reference's arithmetic
 // (src/deposition.jl:38-84), added to the chunk's LDS sub-grid.  Two
 // accumulators, chosen per chunk:
@@ -456,8 +447,11 @@ __device__ __forceinline__ void deposit_one(const Rec<PT> &r, const Geo &g, int 
     }
 }
 
+// qsrc: the chunk's charges of another channel (same order as the
+// records), or null for the records' own charges.
 template <typename PT, typename MT, bool FIXED>
-__device__ __forceinline__ void deposit_chunk(const Rec<PT> *__restrict__ src, int len, const Geo &g,
+__device__ __forceinline__ void deposit_chunk(const Rec<PT> *__restrict__ src,
+                                              const PT *__restrict__ qsrc, int len, const Geo &g,
                                               int ox, int oy, int oz, double *__restrict__ sub,
                                               double S)
 {
@@ -472,6 +466,13 @@ __device__ __forceinline__ void deposit_chunk(const Rec<PT> *__restrict__ src, i
             const int i = i0 + u * kDepThreads + (int)threadIdx.x;
             rb[u] = src[i < len ? i : len - 1];
         }
+        if (qsrc) {  // uniform over the workgroup
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int i = i0 + u * kDepThreads + (int)threadIdx.x;
+                rb[u].q = qsrc[i < len ? i : len - 1];
+            }
+        }
 #pragma unroll
         for (int u = 0; u < kBatch; ++u) {
             if (i0 + u * kDepThreads + (int)threadIdx.x < len)
@@ -480,17 +481,25 @@ __device__ __forceinline__ void deposit_chunk(const Rec<PT> *__restrict__ src, i
     }
 }
 
+// Grid (chunks, channels): channel c takes its charges from the records
+// (c = 0) or from qs[c - 1] in the records' order, its max|q| from the
+// slice maxima kBinMaxSlices * c in, and writes sub-grid set c (pstride
+// doubles apart).
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kDepThreads)
-k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chunks,
+k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
+                const PT *__restrict__ q2s, const int4 *__restrict__ chunks,
                 const unsigned *__restrict__ nchunks, Geo g, double *__restrict__ partials,
-                const unsigned *__restrict__ ext_flags,
+                size_t pstride, const unsigned *__restrict__ ext_flags,
                 const unsigned long long *__restrict__ slice_qmax, int nslices)
 {
     resolve_geo(g);
     __shared__ double sub[kSubVol];
     __shared__ unsigned long long wmax[kDepThreads / 64];
     if (blockIdx.x >= *nchunks) return;
+    const int chan = blockIdx.y;
+    slice_qmax += (size_t)chan * kBinMaxSlices;
+    partials += chan * pstride;
     const int4 ch = chunks[blockIdx.x];
     const int t = ch.x;
     const int ox = (t % g.tx) * kTile, oy = ((t / g.tx) % g.ty) * kTile, oz = (t / (g.tx * g.ty)) * kTile;
@@ -525,8 +534,9 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chu
     for (int i = threadIdx.x; i < kSubVol; i += blockDim.x) sub[i] = 0.0;  // also integer zero
     __syncthreads();
     const Rec<PT> *src = sorted + ch.y;
-    if (fixed) deposit_chunk<PT, MT, true>(src, ch.z, g, ox, oy, oz, sub, S);
-    else deposit_chunk<PT, MT, false>(src, ch.z, g, ox, oy, oz, sub, S);
+    const PT *qsrc = chan == 0 ? nullptr : (chan == 1 ? q1s : q2s) + ch.y;
+    if (fixed) deposit_chunk<PT, MT, true>(src, qsrc, ch.z, g, ox, oy, oz, sub, S);
+    else deposit_chunk<PT, MT, false>(src, qsrc, ch.z, g, ox, oy, oz, sub, S);
     __syncthreads();
     double *out = partials + (size_t)ch.w * kSubVol;
     if (fixed) {
@@ -539,12 +549,14 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const int4 *__restrict__ chu
 
 // Completes the per-tile sums in sub-grid slots [0, ntiles): adds a tile's
 // further chunks (slots extra_base[t], ...) to slot t, or zeroes slot t of
-// an empty tile.  Grid (ceil(kSubVol / 256), tiles), one lane per node.
+// an empty tile.  Grid (ceil(kSubVol / 256), tiles, channels), one lane per
+// node; channel c's sub-grids start c * pstride doubles in.
 __global__ void __launch_bounds__(256)
-k_tile_reduce(double *__restrict__ partials, const unsigned *__restrict__ counts,
+k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__restrict__ counts,
               const unsigned *__restrict__ extra_base)
 {
     const int t = blockIdx.y;
+    partials += blockIdx.z * pstride;
     const unsigned nk = (counts[t] + kChunk - 1) / kChunk;
     if (nk == 1) return;  // slot t already holds the sum
     const int local = blockIdx.x * blockDim.x + threadIdx.x;
@@ -575,10 +587,16 @@ k_tile_reduce(double *__restrict__ partials, const unsigned *__restrict__ counts
 // t's sum is in sub-grid slot t (k_tile_reduce), so every address follows
 // from the node index alone.  Plain stores, fixed summation order; with
 // accumulate the sum is added to the existing rho (deposit!(...; clear=false)).
+// Grid (ceil(M / 256), channels): channel c reads sub-grid set c (pstride
+// doubles apart) and writes rho_c.
 template <typename MT>
 __global__ void __launch_bounds__(256)
-k_gather_tiles(const double *__restrict__ partials, Geo g, MT *__restrict__ rho, int accumulate)
+k_gather_tiles(const double *__restrict__ partials, size_t pstride, Geo g, MT *__restrict__ rho0,
+               MT *__restrict__ rho1, MT *__restrict__ rho2, int accumulate)
 {
+    const int chan = blockIdx.y;
+    partials += chan * pstride;
+    MT *__restrict__ rho = chan == 0 ? rho0 : (chan == 1 ? rho1 : rho2);
     // at most kMaxTiles tiles of 16^3 cells: node indices fit in 32 bits
     const int M = g.nx * g.ny * g.nz;
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -617,7 +635,7 @@ Split split_of(int64_t n)
     return sp;
 }
 
-int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
+int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles, int nchan)
 {
     DepositBins &B = m->bins;
     const size_t rec = 4 * (size_t)pdt;
@@ -633,16 +651,21 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles)
         B.sorted_bytes = need;
     }
     const int64_t maxchunks = ntiles + n / kChunk + 1;
-    if (B.chunk_cap < maxchunks) {
+    if (B.chunk_cap < maxchunks || B.partial_sets < nchan) {
+        const int64_t cap = B.chunk_cap > maxchunks ? B.chunk_cap : maxchunks;
+        const int sets = B.partial_sets > nchan ? B.partial_sets : nchan;
         if (B.chunks) (void)hipFree(B.chunks);
         if (B.partials) (void)hipFree(B.partials);
         B.chunks = B.partials = nullptr;
         B.chunk_cap = 0;
-        if (hipMalloc(&B.chunks, sizeof(int4) * maxchunks) != hipSuccess) return fail();
-        // one sub-grid per chunk, in the wider mesh type (double covers both)
-        if (hipMalloc(&B.partials, sizeof(double) * kSubVol * (size_t)maxchunks) != hipSuccess)
+        B.partial_sets = 0;
+        if (hipMalloc(&B.chunks, sizeof(int4) * cap) != hipSuccess) return fail();
+        // one sub-grid per chunk and channel, in the wider mesh type (double
+        // covers both); channel c's set starts c * cap sub-grids in
+        if (hipMalloc(&B.partials, sizeof(double) * kSubVol * (size_t)cap * sets) != hipSuccess)
             return fail();
-        B.chunk_cap = maxchunks;
+        B.chunk_cap = cap;
+        B.partial_sets = sets;
     }
     if (!B.counts) {
         // totals | tile start | first extra sub-grid slot | extrapolation flags | nchunks, pad |
@@ -703,31 +726,29 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
                            slice_qmax, (const PT *)nullptr, (const PT *)nullptr, (PT *)nullptr,
                            (PT *)nullptr);
     launch_end(ctx, K_BIN_SCATTER);
+    // every channel in each launch (grid axis), on the same chunks
     const unsigned maxchunks = (unsigned)(ntiles + n / kChunk + 1);
     const int64_t M = (int64_t)g.nx * g.ny * g.nz;
-    for (int c = 0; c < nchan; ++c) {
-        if (c > 0) {
-            // the next channel's charges into the records, same tile order
-            hipLaunchKernelGGL((k_set_q<PT>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                               sorted, (const PT *)(c == 1 ? q1s : q2s), n);
-        }
-        launch_begin(ctx, K_DEPOSIT_TILES);
-        hipLaunchKernelGGL((k_deposit_tiles<PT, MT>), dim3(maxchunks), dim3(kDepThreads), 0, st,
-                           (const Rec<PT> *)sorted, (const int4 *)chunks, (const unsigned *)nch, g,
-                           static_cast<double *>(B.partials), (const unsigned *)ext,
-                           (const unsigned long long *)(slice_qmax + (size_t)c * kBinMaxSlices),
-                           sp.G);
-        launch_end(ctx, K_DEPOSIT_TILES);
-        launch_begin(ctx, K_TILE_REDUCE);
-        hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((kSubVol + 255) / 256), (unsigned)ntiles),
-                           dim3(256), 0, st, static_cast<double *>(B.partials),
-                           (const unsigned *)counts, (const unsigned *)cbase);
-        launch_end(ctx, K_TILE_REDUCE);
-        launch_begin(ctx, K_GATHER_TILES);
-        hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256)), dim3(256), 0, st,
-                           (const double *)B.partials, g, static_cast<MT *>(out[c]), clear ? 0 : 1);
-        launch_end(ctx, K_GATHER_TILES);
-    }
+    const size_t pstride = (size_t)kSubVol * (size_t)B.chunk_cap;
+    launch_begin(ctx, K_DEPOSIT_TILES);
+    hipLaunchKernelGGL((k_deposit_tiles<PT, MT>), dim3(maxchunks, nchan), dim3(kDepThreads), 0, st,
+                       (const Rec<PT> *)sorted, (const PT *)q1s, (const PT *)q2s,
+                       (const int4 *)chunks, (const unsigned *)nch, g,
+                       static_cast<double *>(B.partials), pstride, (const unsigned *)ext,
+                       (const unsigned long long *)slice_qmax, sp.G);
+    launch_end(ctx, K_DEPOSIT_TILES);
+    launch_begin(ctx, K_TILE_REDUCE);
+    hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((kSubVol + 255) / 256), (unsigned)ntiles, nchan),
+                       dim3(256), 0, st, static_cast<double *>(B.partials), pstride,
+                       (const unsigned *)counts, (const unsigned *)cbase);
+    launch_end(ctx, K_TILE_REDUCE);
+    launch_begin(ctx, K_GATHER_TILES);
+    MT *o0 = static_cast<MT *>(out[0]);
+    MT *o1 = nchan > 1 ? static_cast<MT *>(out[1]) : o0;
+    MT *o2 = nchan > 2 ? static_cast<MT *>(out[2]) : o0;
+    hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256), nchan), dim3(256), 0,
+                       st, (const double *)B.partials, pstride, g, o0, o1, o2, clear ? 0 : 1);
+    launch_end(ctx, K_GATHER_TILES);
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
 }
@@ -802,7 +823,7 @@ int launch_sort_perm(Mesh *m, int pdt, const void *x, const void *y, const void 
         g.lo[a] = lo ? lo[a] : 0.0;
         g.dl[a] = lo ? dl[a] : 1.0;
     }
-    if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz)) return rc;
+    if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz, 1)) return rc;
     if (pdt == SC_F32 && m->dtype == SC_F32) return perm_binned<float, float>(m, x, y, z, n, g, perm);
     if (pdt == SC_F64 && m->dtype == SC_F64)
         return perm_binned<double, double>(m, x, y, z, n, g, perm);
@@ -830,7 +851,7 @@ int launch_deposit_binned_multi(Mesh *m, int pdt, const void *x, const void *y, 
         g.lo[a] = lo ? lo[a] : 0.0;
         g.dl[a] = lo ? dl[a] : 1.0;
     }
-    if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz)) return rc;
+    if (int rc = ensure_bins(m, n, pdt, g.tx * g.ty * g.tz, nchan)) return rc;
     if (nchan > 1) {
         DepositBins &B = m->bins;
         const size_t need = (size_t)2 * n * pdt;

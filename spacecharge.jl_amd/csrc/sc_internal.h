@@ -148,8 +148,9 @@ struct DepositBins {
     void *sorted = nullptr;       // (x, y, z, q) records in tile order
     size_t sorted_bytes = 0;
     void *chunks = nullptr;       // int4 (tile, start, length, 0)
-    void *partials = nullptr;     // one 17^3 sub-grid per chunk
-    int64_t chunk_cap = 0;
+    void *partials = nullptr;     // one 17^3 sub-grid per chunk and channel
+    int64_t chunk_cap = 0;        // chunks, and sub-grids per channel set of partials
+    int partial_sets = 0;         // channel sets that partials holds (1 or 3)
     unsigned *counts = nullptr;   // per-tile totals | tile start | chunk_base | nchunks
     unsigned *cmat = nullptr;     // [tile][binning workgroup] counts, then offsets
     int64_t cmat_cap = 0;         // entries
