@@ -228,27 +228,27 @@ def bench_adjoint(args, sc, torch, dev, T):
     nsteps, every = 100, 10
     dt, kick = 1.0, 1e-3 * sig / 1e7  # a kick of ~1e-3 sigma per step at 1e7 V/m
 
-    def track(grad, ckpt):
+    def track(grad, ckpt, recompute="trajectory"):
         xs, ys, zs, qs = (t.clone().requires_grad_(grad) for t in (x, y, z, q))
         zero = torch.zeros_like(xs)
         with torch.set_grad_enabled(grad):
             st = scad.track(mesh, xs, ys, zs, zero, zero, zero, qs, nsteps, dt, kick,
-                            checkpoint_every=ckpt)
+                            checkpoint_every=ckpt, recompute=recompute)
             loss = scad.rms_emittance(*st)
         if grad:
             loss.backward()
             return xs.grad, qs.grad
         return None, None
 
-    def timed_track(grad, ckpt, reps=3):
+    def timed_track(grad, ckpt, recompute="trajectory", reps=3):
         # one untimed run, then the median of `reps` timed runs (the loop is
         # issued from Python, so single runs vary with the host)
-        out = track(grad, ckpt)
+        out = track(grad, ckpt, recompute)
         times = []
         for _ in range(reps):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            out = track(grad, ckpt)
+            out = track(grad, ckpt, recompute)
             torch.cuda.synchronize()
             times.append((time.perf_counter() - t0) * 1e3)
         return sorted(times)[len(times) // 2], out
@@ -256,6 +256,7 @@ def bench_adjoint(args, sc, torch, dev, T):
     ms_track_fwd, _ = timed_track(False, every)
     ms_track, (gx, gq) = timed_track(True, every)
     ms_track_all, _ = timed_track(True, 0)
+    ms_track_steps, _ = timed_track(True, every, "step")
     return {
         "metric": "particles/s per differentiable space-charge step (forward + vector-Jacobian product)",
         "value": n / ((ms_fwd + ms_vjp) * 1e-3),
@@ -284,9 +285,12 @@ def bench_adjoint(args, sc, torch, dev, T):
             "steps": nsteps,
             "checkpoint_every": every,
             "loss": "final rms emittance (sum over planes)",
+            "recompute": "particle trajectory from the stored per-step mesh fields",
             "forward_only_ms": ms_track_fwd,
             "forward_and_backward_ms": ms_track,
             "grad_over_forward": ms_track / ms_track_fwd,
+            "forward_and_backward_ms_whole_steps_recomputed": ms_track_steps,
+            "grad_over_forward_whole_steps_recomputed": ms_track_steps / ms_track_fwd,
             "forward_and_backward_ms_every_step_stored": ms_track_all,
             "grad_over_forward_every_step_stored": ms_track_all / ms_track_fwd,
             "grad_finite": bool(torch.isfinite(gx).all() and torch.isfinite(gq).all()),

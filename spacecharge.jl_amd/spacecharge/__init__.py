@@ -70,6 +70,7 @@ class SpaceChargeError(Exception):
 
 _DT = {torch.float32: SC_F32, torch.float64: SC_F64}
 _ctxs = {}
+_ctx_streams = {}  # device index -> stream handle last given to that context
 
 
 def _ctx(device: torch.device) -> ctypes.c_void_p:
@@ -83,8 +84,10 @@ def _ctx(device: torch.device) -> ctypes.c_void_p:
             raise SpaceChargeError(lib.sc_last_error(None).decode())
         c = h
         _ctxs[idx] = c
-    lib = _lib.load()
-    lib.sc_ctx_set_stream(c, ctypes.c_void_p(torch.cuda.current_stream(idx).cuda_stream))
+    stream = torch.cuda.current_stream(idx).cuda_stream
+    if _ctx_streams.get(idx) != stream:
+        _lib.load().sc_ctx_set_stream(c, ctypes.c_void_p(stream))
+        _ctx_streams[idx] = stream
     return c
 
 
@@ -200,6 +203,7 @@ class Mesh3D:
         self._h = h
         self._device_bounds = False
         self._geom_version = 0
+        self._green_reuse = False
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -224,6 +228,7 @@ class Mesh3D:
     def set_green_reuse(self, enable: bool = True) -> None:
         """Reuse the transformed Green function while delta/gamma/cathode are unchanged."""
         _check(_lib.load().sc_mesh_set_green_reuse(self._h, int(bool(enable))), None)
+        self._green_reuse = bool(enable)
 
     def force_generic_transforms(self, enable: bool = True) -> None:
         """Diagnostic: run every transform on the generic mixed-radix engine."""

@@ -270,10 +270,12 @@ def test_adjoint_argument_errors():
 
 def test_tracking_gradient_matches_central_differences():
     """100 drift-kick steps (Float64, 32^3 mesh with fixed bounds, segments of
-    10 steps under checkpointing), loss = final rms emittance: the autograd
-    gradient with respect to initial positions and charges against central
-    differences of the whole run (steps of 1e-6 of the bunch size and of the
-    charge), and against the same run without checkpointing."""
+    10 steps checkpointed, trajectories recomputed from the stored fields),
+    loss = final rms emittance: the gradient with respect to initial
+    positions and charges against central differences of the whole run
+    (steps of 1e-6 of the bunch size and of the charge), against the same run
+    without checkpointing, and against torch.utils.checkpoint re-running
+    whole steps."""
     sc = _sc()
     import spacecharge.autograd as scad
     n, sig = 2000, 1e-3
@@ -283,20 +285,24 @@ def test_tracking_gradient_matches_central_differences():
     nsteps, dt = 100, 1.0
     kick = 1e-4 * sig / 9e6  # ~1e-4 sigma of momentum per step at the bunch's peak field
 
-    def loss_of(x0, y0, z0, q0, every):
+    def loss_of(x0, y0, z0, q0, every, recompute="trajectory"):
         zero = torch.zeros_like(x0)
         st = scad.track(mesh, x0, y0, z0, zero, zero, zero, q0, nsteps, dt, kick,
-                        checkpoint_every=every)
+                        checkpoint_every=every, recompute=recompute)
         return scad.rms_emittance(*st)
 
     xs, ys, zs, qs = (t.clone().requires_grad_(True) for t in (x, y, z, q))
     L = loss_of(xs, ys, zs, qs, 10)
     L.backward()
     grads = [xs.grad.clone(), ys.grad.clone(), zs.grad.clone(), qs.grad.clone()]
-    xs2, ys2, zs2, qs2 = (t.clone().requires_grad_(True) for t in (x, y, z, q))
-    loss_of(xs2, ys2, zs2, qs2, 0).backward()
-    for a, b in zip(grads, (xs2.grad, ys2.grad, zs2.grad, qs2.grad)):
-        assert float((a - b).abs().max()) <= 1e-10 * float(b.abs().max())
+    for every, recompute in ((0, "trajectory"), (10, "step")):
+        xs2, ys2, zs2, qs2 = (t.clone().requires_grad_(True) for t in (x, y, z, q))
+        L2 = loss_of(xs2, ys2, zs2, qs2, every, recompute)
+        # Float64 deposits add in LDS-atomic order, so runs differ in the last bits
+        assert abs(float(L2) - float(L)) <= 1e-12 * abs(float(L))
+        L2.backward()
+        for a, b in zip(grads, (xs2.grad, ys2.grad, zs2.grad, qs2.grad)):
+            assert float((a - b).abs().max()) <= 1e-10 * float(b.abs().max()), (every, recompute)
     # particles at least 5% of a cell from every face
     pos = [t.cpu().numpy() for t in (x, y, z)]
     frac = [((pos[a] - mesh.min_bounds[a]) / mesh.delta[a]) % 1.0 for a in range(3)]
@@ -317,3 +323,46 @@ def test_tracking_gradient_matches_central_differences():
                 worst = max(worst, err)
                 assert err < 1e-4, (p, a, fd, g)
     print(f"tracking gradient vs central differences: worst rel err {worst:.2e}")
+
+
+@pytest.mark.parametrize("T,tol", [(torch.float64, 1e-11), (torch.float32, 2e-4)])
+@pytest.mark.parametrize("cath", [False, True])
+def test_tracking_segments_of_uneven_length(T, tol, cath):
+    """25 steps in segments of 10, 10 and 5 with trajectories recomputed from
+    the stored fields, nonzero initial momenta, loss touching every output:
+    the same loss and the same gradients (all seven inputs) as the run that
+    keeps every step for autograd."""
+    sc = _sc()
+    import spacecharge.autograd as scad
+    n, sig = 5000, 1e-3
+    x, y, z, q = _bunch(n, 43, sigma=(sig, sig, 2 * sig))
+    if cath:
+        z = z + 9 * sig
+    x, y, z, q = (t.to(T) for t in (x, y, z, q))
+    gen = torch.Generator(device=DEV).manual_seed(44)
+    p0 = [1e-3 * sig * torch.randn(n, generator=gen, dtype=torch.float64, device=DEV).to(T)
+          for _ in range(3)]
+    zlo, zhi = (0.5 * sig, 17.5 * sig) if cath else (-8 * sig, 8 * sig)
+    mesh = sc.Mesh3D((32, 32, 48), (-6 * sig, -6 * sig, zlo), (6 * sig, 6 * sig, zhi), T=T)
+    kick = 1e-4 * sig / 9e6
+
+    def run(every):
+        ins = [t.clone().requires_grad_(True) for t in (x, y, z, *p0, q)]
+        st = scad.track(mesh, *ins[:6], ins[6], 25, 1.0, kick, checkpoint_every=every,
+                        at_cathode=cath)
+        w = torch.linspace(0.5, 1.5, n, dtype=T, device=DEV)
+        L = scad.rms_emittance(*st) + sum(((s_ * w).sum() for s_ in st)) * 1e-3
+        L.backward()
+        return float(L), [t.grad for t in ins]
+
+    L1, g1 = run(10)
+    L0, g0 = run(0)
+    # Float32 meshes deposit in fixed point (bit-reproducible); Float64 ones
+    # add in LDS-atomic order
+    if T == torch.float32:
+        assert L1 == L0
+    else:
+        assert abs(L1 - L0) <= 1e-12 * abs(L0)
+    for i, (a, b) in enumerate(zip(g1, g0)):
+        err = float((a - b).abs().max()) / float(b.abs().max())
+        assert err <= tol, (i, err)
