@@ -155,6 +155,20 @@ def bench_batched(args, sc, torch, dev, T):
     with torch.cuda.graph(graph):
         step_batched()
     ms_graph = timed(graph.replay)
+
+    # deposit tilings picked per bunch by the autotuner, then the same
+    # measurements (the graph is captured again: launch shapes changed)
+    sc.autotune_batched(meshes, *[[p[k] for p in parts] for k in range(4)], reps=3)
+    tilings = [list(m.deposit_tiling) for m in meshes]
+    ms_tuned = timed(step_batched)
+    with torch.cuda.stream(side):
+        step_batched()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph_t = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph_t):
+        step_batched()
+    ms_graph_tuned = timed(graph_t.replay)
     return {
         "metric": "particles/s per full space-charge step (deposit+solve+interpolate), many bunches",
         "value": ntot / (ms * 1e-3),
@@ -181,6 +195,10 @@ def bench_batched(args, sc, torch, dev, T):
         "batched_speedup": ms_loop / ms,
         "graph_ms_per_step": ms_graph,
         "graph_particles_per_s": ntot / (ms_graph * 1e-3),
+        "autotuned_ms_per_step": ms_tuned,
+        "autotuned_graph_ms_per_step": ms_graph_tuned,
+        "autotuned_graph_particles_per_s": ntot / (ms_graph_tuned * 1e-3),
+        "deposit_tilings": tilings,
     }
 
 
@@ -441,6 +459,9 @@ def main():
     ap.add_argument("--sorted", action="store_true",
                     help="put the particles in the deposit's tile order once before the warmup "
                          "(sort_permutation), as a tracking code does every few steps")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the deposit tiling by measurement (autotune_deposit) before "
+                         "the warmup; the batched workload always reports both")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="time budget of the CPU-oracle sample (0 disables it)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -480,6 +501,11 @@ def main():
     if args.sorted:
         perm = sc.sort_permutation(mesh, x, y, z)
         x, y, z, q = (torch.index_select(t, 0, perm) for t in (x, y, z, q))
+    tuning = None
+    if args.autotune:
+        times = sc.autotune_deposit(mesh, x, y, z, q, reps=5)
+        tuning = {"tiling": list(mesh.deposit_tiling), "deposit_ms": times[mesh.deposit_tiling],
+                  "default_tiling_deposit_ms": times[(13, 12)]}
 
     def step():
         sc.deposit(mesh, x, y, z, q)
@@ -577,8 +603,10 @@ def main():
             "gamma": args.gamma,
             "at_cathode": bool(args.cathode),
             "particle_order": "tile order (sort_permutation)" if args.sorted else "as generated",
+            "deposit_tiling": list(mesh.deposit_tiling),
             "parallelism": "single GPU",
         },
+        "autotune": tuning,
         "roofline": roofline,
         "step_roofline": step_roofline,
         "cpu_baseline": cpu,

@@ -104,6 +104,28 @@ int sc_mesh_set_green_reuse(sc_mesh *mesh, int enable);
  * the same results. */
 int sc_mesh_force_generic_transforms(sc_mesh *mesh, int enable);
 
+/* ---- deposit tiling and its autotuner (no reference counterpart: the
+ * reference launches one thread per particle with global atomics,
+ * src/deposition.jl:106-158) ---------------------------------------------
+ * The binned deposit (sc_deposit, the interpolation adjoint's cotangent
+ * deposit, sc_sort_permutation) splits each 16^3-cell tile's particles into
+ * chunks of 2^chunk_log2 particles, one deposit workgroup each
+ * (chunk_log2 in [10, 14], default 13), and bins the particles in slices of
+ * about 2^slice_log2 particles, one binning workgroup each (slice_log2 in
+ * [10, 13], default 12).  Results agree to the deposit's round-off for every
+ * setting; Float32 meshes give the same bits for the same setting.
+ * sc_mesh_autotune_deposit times the deposit of the given particles at every
+ * setting (HIP events on the context's stream: one untimed call, then reps
+ * timed calls), keeps the fastest on the mesh and deposits once more with
+ * it, so rho ends holding that deposit.  times_ms (NULL allowed) receives
+ * 5 x 4 mean times, row-major [chunk_log2 - 10][slice_log2 - 10].  It
+ * synchronises and refuses to run under stream capture. */
+int sc_mesh_set_deposit_tiling(sc_mesh *mesh, int chunk_log2, int slice_log2);
+int sc_mesh_get_deposit_tiling(const sc_mesh *mesh, int *chunk_log2, int *slice_log2);
+int sc_mesh_autotune_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y,
+                             const void *z, const void *q, int64_t n, const double min_bounds[3],
+                             const double delta[3], int reps, float *times_ms);
+
 /* Replaces the particle-extrema constructor's bounds computation
  * (src/mesh.jl:95-174): device min/max of x, y, z, then
  *   delta = (max - min)/(n - 1) in the particle type,

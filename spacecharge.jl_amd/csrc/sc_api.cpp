@@ -474,6 +474,95 @@ int sc_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y, const vo
     return launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl, clear);
 }
 
+int sc_mesh_set_deposit_tiling(sc_mesh *mesh, int chunk_log2, int slice_log2)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    if (chunk_log2 < kChunkLog2Min || chunk_log2 > kChunkLog2Max || slice_log2 < kSliceLog2Min ||
+        slice_log2 > kSliceLog2Max)
+        return set_err(M.ctx, SC_ERR_INVALID,
+                       "chunk_log2 must lie in [" + std::to_string(kChunkLog2Min) + ", " +
+                           std::to_string(kChunkLog2Max) + "] and slice_log2 in [" +
+                           std::to_string(kSliceLog2Min) + ", " + std::to_string(kSliceLog2Max) + "]");
+    M.chunk_log2 = chunk_log2;
+    M.slice_log2 = slice_log2;
+    return SC_OK;
+}
+
+int sc_mesh_get_deposit_tiling(const sc_mesh *mesh, int *chunk_log2, int *slice_log2)
+{
+    if (!mesh || !chunk_log2 || !slice_log2) return set_err(nullptr, SC_ERR_INVALID, "NULL argument");
+    *chunk_log2 = mesh->m.chunk_log2;
+    *slice_log2 = mesh->m.slice_log2;
+    return SC_OK;
+}
+
+// Times the binned deposit of these particles at every (chunk, slice)
+// setting with HIP events on the context's stream (one untimed call, then
+// `reps` timed calls each), keeps the fastest on the mesh and deposits once
+// more with it, so rho ends holding this deposit.  times_ms (may be NULL)
+// receives the mean per call, row-major [chunk_log2][slice_log2].
+int sc_mesh_autotune_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y,
+                             const void *z, const void *q, int64_t n, const double lo[3],
+                             const double dl[3], int reps, float *times_ms)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n <= 0) return set_err(c, SC_ERR_INVALID, "the autotuner needs at least one particle");
+    if (!x || !y || !z || !q) return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
+    if (reps < 1 || reps > 1000) return set_err(c, SC_ERR_INVALID, "reps must lie in [1, 1000]");
+    if (int rc = check_geom_m(c, M, lo, dl)) return rc;
+    if (int rc = check_dev(c)) return rc;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SC_HIP(c, hipStreamIsCapturing(c->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return set_err(c, SC_ERR_INVALID, "the autotuner times kernels and cannot run under stream capture");
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    SC_HIP(c, hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return set_err(c, SC_ERR_DEVICE, "hipEventCreate failed");
+    }
+    const int keep_c = M.chunk_log2, keep_s = M.slice_log2;
+    int best_c = keep_c, best_s = keep_s, rc = SC_OK;
+    float best = -1.f;
+    for (int cl = kChunkLog2Min; cl <= kChunkLog2Max && rc == SC_OK; ++cl)
+        for (int sl = kSliceLog2Min; sl <= kSliceLog2Max && rc == SC_OK; ++sl) {
+            M.chunk_log2 = cl;
+            M.slice_log2 = sl;
+            rc = launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl, 1);  // allocates, warms
+            if (rc == SC_OK && hipEventRecord(ev[0], c->stream) != hipSuccess) rc = SC_ERR_DEVICE;
+            for (int r = 0; r < reps && rc == SC_OK; ++r)
+                rc = launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl, 1);
+            float ms = 0.f;
+            if (rc == SC_OK && (hipEventRecord(ev[1], c->stream) != hipSuccess ||
+                                hipEventSynchronize(ev[1]) != hipSuccess ||
+                                hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+                rc = set_err(c, SC_ERR_DEVICE, "timing the deposit failed");
+            if (rc != SC_OK) break;
+            ms /= (float)reps;
+            if (times_ms)
+                times_ms[(cl - kChunkLog2Min) * (kSliceLog2Max - kSliceLog2Min + 1) + (sl - kSliceLog2Min)] = ms;
+            if (best < 0.f || ms < best) {
+                best = ms;
+                best_c = cl;
+                best_s = sl;
+            }
+        }
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc != SC_OK) {
+        M.chunk_log2 = keep_c;
+        M.slice_log2 = keep_s;
+        return rc;
+    }
+    M.chunk_log2 = best_c;
+    M.slice_log2 = best_s;
+    return launch_deposit(&M, pdtype, x, y, z, q, n, lo, dl, 1);
+}
+
 int sc_solve(sc_mesh *mesh, const double lo[3], const double hi[3], const double dl[3],
              double gamma, int at_cathode)
 {

@@ -16,7 +16,7 @@
 //   bin_rowscan one wave per tile: prefix of its row over the slices, and
 //               the tile's total
 //   bin_scan    one workgroup: tile offsets and the (tile, start, length)
-//               chunk table, chunks of at most kChunk particles
+//               chunk table, chunks of at most 2^chunk_log2 particles
 //   bin_scatter each slice writes (x, y, z, q) records into its reserved run
 //               of each tile, slots from an LDS cursor per tile
 //   deposit_tiles  one workgroup per chunk: Float64 LDS sub-grid, CIC with
@@ -46,12 +46,10 @@ constexpr int kSub = kTile + 1;      // nodes per sub-grid edge
 constexpr int kSubVol = kSub * kSub * kSub;
 constexpr int kMaxTiles = 4096;      // LDS histogram size
 constexpr int kBinThreads = 256;
-constexpr int kBinPerThread = 16;    // particles per lane and slice round (slice sizing)
-constexpr int kBinChunk = kBinThreads * kBinPerThread;
 constexpr int kBatch = 8;            // particles a lane loads before computing any
-constexpr int kChunk = 8192;         // particles per deposit workgroup
+// particles per deposit workgroup: 2^chunk_log2, a per-mesh setting
+// (Mesh::chunk_log2, kChunkLog2Min..kChunkLog2Max)
 constexpr int kDepThreads = 512;     // deposit workgroup size
-static_assert(kChunk <= 8192, "the fixed-point deposit scale assumes at most 2^13 particles per chunk");
 constexpr int kBinMaxSlices = 2048;  // binning workgroups (columns of the count matrix)
 
 inline int div_up4(int a) { return (a + 3) / 4; }
@@ -202,8 +200,10 @@ k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restri
 __global__ void __launch_bounds__(256)
 k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
            unsigned *__restrict__ extra_base, int4 *__restrict__ chunks,
-           unsigned *__restrict__ nchunks, unsigned *__restrict__ ext_flags, int ntiles)
+           unsigned *__restrict__ nchunks, unsigned *__restrict__ ext_flags, int ntiles,
+           int chunk_log2)
 {
+    const unsigned chunk = 1u << chunk_log2;
     constexpr int kPer = kMaxTiles / 256;
     __shared__ unsigned s_w[3][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -215,7 +215,7 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
         c[i] = (i < per && t0 + i < ntiles) ? counts[t0 + i] : 0u;
-        const unsigned nk = (c[i] + kChunk - 1) / kChunk;
+        const unsigned nk = (c[i] + chunk - 1) >> chunk_log2;
         a += c[i];
         b += nk;
         x += nk ? nk - 1 : 0u;
@@ -252,11 +252,11 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
             // sub-grid slots: the tile's first chunk uses slot t (which then
             // holds the tile's sum), its further chunks ntiles + ox, ...
             extra_base[t] = (unsigned)ntiles + ox;
-            const unsigned nk = (c[i] + kChunk - 1) / kChunk;
+            const unsigned nk = (c[i] + chunk - 1) >> chunk_log2;
             for (unsigned k = 0; k < nk; ++k) {
-                const unsigned len = (c[i] - k * kChunk) < (unsigned)kChunk ? (c[i] - k * kChunk) : kChunk;
+                const unsigned len = (c[i] - k * chunk) < chunk ? (c[i] - k * chunk) : chunk;
                 const unsigned slot = k == 0 ? (unsigned)t : (unsigned)ntiles + ox + k - 1;
-                chunks[ob + k] = make_int4(t, (int)(oa + k * kChunk), (int)len, (int)slot);
+                chunks[ob + k] = make_int4(t, (int)(oa + k * chunk), (int)len, (int)slot);
             }
             oa += c[i];
             ob += nk;
@@ -407,8 +407,8 @@ k_bin_perm(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__res
 // accumulators, chosen per chunk:
 //  * Float64 LDS atomics (FIXED = false): any mesh type, any weights;
 //  * 64-bit fixed point (FIXED = true, Float32 meshes): the contribution is
-//    rounded to a multiple of 1/S (S a power of two with
-//    kChunk * max|q| * S < 2^62, so no sum can overflow) and added with an
+//    rounded to a multiple of 1/S (S a power of two with max|q| * S < 2^49
+//    and 2^chunk_log2 * max|q| * S < 2^62, so no sum can overflow) and added with an
 //    integer LDS atomic, which gfx950 executes about twice as fast as the
 //    Float64 one.  The sum is exact on that grid; the rounding is at most
 //    2^-49 max|q| per contribution, far below Float32 resolution.  Only
@@ -491,7 +491,7 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
                 const PT *__restrict__ q2s, const int4 *__restrict__ chunks,
                 const unsigned *__restrict__ nchunks, Geo g, double *__restrict__ partials,
                 size_t pstride, const unsigned *__restrict__ ext_flags,
-                const unsigned long long *__restrict__ slice_qmax, int nslices)
+                const unsigned long long *__restrict__ slice_qmax, int nslices, int chunk_log2)
 {
     resolve_geo(g);
     __shared__ double sub[kSubVol];
@@ -525,7 +525,9 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
         if (ext_flags[t] == 0u && qm <= 1.7976931348623157e308 && (qm == 0.0 || qm >= 1e-270)) {
             fixed = true;
             if (qm > 0.0) {
-                const int e = 48 - ilogb(qm);  // kChunk (2^13) * qm * 2^e < 2^62
+                // |q 2^e| < 2^49 (the rounding below needs < 2^51) and
+                // 2^chunk_log2 * qm * 2^e < 2^62 (no sum overflows)
+                const int e = (chunk_log2 <= 13 ? 48 : 61 - chunk_log2) - ilogb(qm);
                 S = ldexp(1.0, e);
                 invS = ldexp(1.0, -e);
             }
@@ -553,11 +555,11 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
 // node; channel c's sub-grids start c * pstride doubles in.
 __global__ void __launch_bounds__(256)
 k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__restrict__ counts,
-              const unsigned *__restrict__ extra_base)
+              const unsigned *__restrict__ extra_base, int chunk_log2)
 {
     const int t = blockIdx.y;
     partials += blockIdx.z * pstride;
-    const unsigned nk = (counts[t] + kChunk - 1) / kChunk;
+    const unsigned nk = (counts[t] + (1u << chunk_log2) - 1) >> chunk_log2;
     if (nk == 1) return;  // slot t already holds the sum
     const int local = blockIdx.x * blockDim.x + threadIdx.x;
     if (local >= kSubVol) return;
@@ -622,12 +624,13 @@ k_gather_tiles(const double *__restrict__ partials, size_t pstride, Geo g, MT *_
     rho[i] = accumulate ? (MT)((double)rho[i] + s) : (MT)s;
 }
 
-// At most kBinMaxSlices slices of whole rounds of kBinChunk particles.
-Split split_of(int64_t n)
+// At most kBinMaxSlices slices of about 2^slice_log2 particles.
+Split split_of(int64_t n, int slice_log2)
 {
     Split sp;
     sp.n = n;
-    int64_t G = (n + kBinChunk - 1) / kBinChunk;
+    const int64_t per = (int64_t)1 << slice_log2;
+    int64_t G = (n + per - 1) / per;
     if (G > kBinMaxSlices) G = kBinMaxSlices;
     if (G < 1) G = 1;
     sp.G = (int)G;
@@ -637,6 +640,7 @@ Split split_of(int64_t n)
 
 int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles, int nchan)
 {
+    const int64_t chunk = (int64_t)1 << m->chunk_log2;
     DepositBins &B = m->bins;
     const size_t rec = 4 * (size_t)pdt;
     const size_t need = (size_t)n * rec;
@@ -650,7 +654,7 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles, int nchan)
         if (hipMalloc(&B.sorted, need) != hipSuccess) return fail();
         B.sorted_bytes = need;
     }
-    const int64_t maxchunks = ntiles + n / kChunk + 1;
+    const int64_t maxchunks = ntiles + n / chunk + 1;
     if (B.chunk_cap < maxchunks || B.partial_sets < nchan) {
         const int64_t cap = B.chunk_cap > maxchunks ? B.chunk_cap : maxchunks;
         const int sets = B.partial_sets > nchan ? B.partial_sets : nchan;
@@ -673,7 +677,7 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles, int nchan)
         const size_t bytes = sizeof(unsigned) * (4 * kMaxTiles + 4 + 3 * 2 * kBinMaxSlices);
         if (hipMalloc((void **)&B.counts, bytes) != hipSuccess) return fail();
     }
-    const int64_t need_cmat = (int64_t)ntiles * split_of(n).G;
+    const int64_t need_cmat = (int64_t)ntiles * split_of(n, m->slice_log2).G;
     if (B.cmat_cap < need_cmat) {
         if (B.cmat) (void)hipFree(B.cmat);
         B.cmat = nullptr;
@@ -698,7 +702,7 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
     unsigned *nch = B.counts + 4 * kMaxTiles;
     auto *slice_qmax = reinterpret_cast<unsigned long long *>(B.counts + 4 * kMaxTiles + 4);
     int4 *chunks = static_cast<int4 *>(B.chunks);
-    const Split sp = split_of(n);
+    const Split sp = split_of(n, m->slice_log2);
     launch_begin(ctx, K_BIN_COUNT);
     hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
                        (const PT *)y, (const PT *)z, sp, g, B.cmat);
@@ -709,7 +713,7 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
     launch_end(ctx, K_BIN_ROWSCAN);
     launch_begin(ctx, K_BIN_SCAN);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
-                       cbase, chunks, nch, ext, ntiles);
+                       cbase, chunks, nch, ext, ntiles, m->chunk_log2);
     launch_end(ctx, K_BIN_SCAN);
     Rec<PT> *sorted = static_cast<Rec<PT> *>(B.sorted);
     PT *q1s = static_cast<PT *>(B.qs), *q2s = q1s ? q1s + n : nullptr;
@@ -727,7 +731,7 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
                            (PT *)nullptr);
     launch_end(ctx, K_BIN_SCATTER);
     // every channel in each launch (grid axis), on the same chunks
-    const unsigned maxchunks = (unsigned)(ntiles + n / kChunk + 1);
+    const unsigned maxchunks = (unsigned)(ntiles + (n >> m->chunk_log2) + 1);
     const int64_t M = (int64_t)g.nx * g.ny * g.nz;
     const size_t pstride = (size_t)kSubVol * (size_t)B.chunk_cap;
     launch_begin(ctx, K_DEPOSIT_TILES);
@@ -735,12 +739,12 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
                        (const Rec<PT> *)sorted, (const PT *)q1s, (const PT *)q2s,
                        (const int4 *)chunks, (const unsigned *)nch, g,
                        static_cast<double *>(B.partials), pstride, (const unsigned *)ext,
-                       (const unsigned long long *)slice_qmax, sp.G);
+                       (const unsigned long long *)slice_qmax, sp.G, m->chunk_log2);
     launch_end(ctx, K_DEPOSIT_TILES);
     launch_begin(ctx, K_TILE_REDUCE);
     hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((kSubVol + 255) / 256), (unsigned)ntiles, nchan),
                        dim3(256), 0, st, static_cast<double *>(B.partials), pstride,
-                       (const unsigned *)counts, (const unsigned *)cbase);
+                       (const unsigned *)counts, (const unsigned *)cbase, m->chunk_log2);
     launch_end(ctx, K_TILE_REDUCE);
     launch_begin(ctx, K_GATHER_TILES);
     MT *o0 = static_cast<MT *>(out[0]);
@@ -781,7 +785,7 @@ int perm_binned(Mesh *m, const void *x, const void *y, const void *z, int64_t n,
     unsigned *cbase = B.counts + 2 * kMaxTiles, *ext = B.counts + 3 * kMaxTiles;
     unsigned *nch = B.counts + 4 * kMaxTiles;
     int4 *chunks = static_cast<int4 *>(B.chunks);
-    const Split sp = split_of(n);
+    const Split sp = split_of(n, m->slice_log2);
     launch_begin(ctx, K_BIN_COUNT);
     hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
                        (const PT *)y, (const PT *)z, sp, g, B.cmat);
@@ -792,7 +796,7 @@ int perm_binned(Mesh *m, const void *x, const void *y, const void *z, int64_t n,
     launch_end(ctx, K_BIN_ROWSCAN);
     launch_begin(ctx, K_BIN_SCAN);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
-                       cbase, chunks, nch, ext, ntiles);
+                       cbase, chunks, nch, ext, ntiles, m->chunk_log2);
     launch_end(ctx, K_BIN_SCAN);
     launch_begin(ctx, K_BIN_SCATTER);
     hipLaunchKernelGGL((k_bin_perm<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,

@@ -175,7 +175,16 @@ struct Mesh {
     // step functions read it when called without bounds
     double *geom = nullptr;
     bool geom_set = false;
+    // binned-deposit tiling: 2^chunk_log2 particles per deposit workgroup
+    // (kChunkLog2Min..kChunkLog2Max) and about 2^slice_log2 particles per
+    // binning workgroup (kSliceLog2Min..kSliceLog2Max); set by hand or by
+    // the autotuner (sc_mesh_autotune_deposit)
+    int chunk_log2 = 13;
+    int slice_log2 = 12;
 };
+
+constexpr int kChunkLog2Min = 10, kChunkLog2Max = 14;
+constexpr int kSliceLog2Min = 10, kSliceLog2Max = 13;
 
 // ---------------------------------------------------------------------------
 // Errors

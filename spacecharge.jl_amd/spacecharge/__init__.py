@@ -57,6 +57,8 @@ __all__ = [
     "solve_batched",
     "interpolate_batched",
     "set_stream_pool_size",
+    "autotune_deposit",
+    "autotune_batched",
 ]
 
 # src/utils.jl:7-8
@@ -230,6 +232,21 @@ class Mesh3D:
         _check(_lib.load().sc_mesh_set_green_reuse(self._h, int(bool(enable))), None)
         self._green_reuse = bool(enable)
 
+    @property
+    def deposit_tiling(self):
+        """(chunk_log2, slice_log2) of the binned deposit: 2^chunk_log2 particles
+        per deposit workgroup, about 2^slice_log2 per binning workgroup."""
+        c, s = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.load().sc_mesh_get_deposit_tiling(self._h, ctypes.byref(c), ctypes.byref(s)),
+               None)
+        return c.value, s.value
+
+    def set_deposit_tiling(self, chunk_log2: int, slice_log2: int) -> None:
+        """Sets the binned deposit's tiling (chunk_log2 in [10, 14], slice_log2 in
+        [10, 13]); autotune_deposit picks it by measurement."""
+        rc = _lib.load().sc_mesh_set_deposit_tiling(self._h, int(chunk_log2), int(slice_log2))
+        _check(rc, _ctx(self.device))
+
     def force_generic_transforms(self, enable: bool = True) -> None:
         """Diagnostic: run every transform on the generic mixed-radix engine."""
         _check(_lib.load().sc_mesh_force_generic_transforms(self._h, int(bool(enable))), None)
@@ -358,6 +375,38 @@ def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
                                         Ex.data_ptr(), Ey.data_ptr(), Ez.data_ptr())
     _check(rc, ctx)
     return Ex, Ey, Ez
+
+
+CHUNK_LOG2 = range(10, 15)
+SLICE_LOG2 = range(10, 14)
+
+
+def autotune_deposit(mesh: Mesh3D, particles_x, particles_y, particles_z, particles_q, *,
+                     reps: int = 5) -> dict:
+    """Times the deposit of these particles at every tiling (chunk_log2 in
+    CHUNK_LOG2, slice_log2 in SLICE_LOG2; HIP events, `reps` timed calls
+    each), keeps the fastest on the mesh and leaves mesh.rho holding the
+    deposit.  Returns {(chunk_log2, slice_log2): ms per call}.  Synchronises;
+    not allowed under graph capture.  No reference counterpart."""
+    x, y, z, q = _particles(mesh, (particles_x, particles_y, particles_z, particles_q), "xyzq")
+    if not (x.numel() == y.numel() == z.numel() == q.numel()):
+        raise SpaceChargeError("Particle coordinate and charge arrays must have the same length.")
+    times = (ctypes.c_float * (len(CHUNK_LOG2) * len(SLICE_LOG2)))()
+    ctx = _ctx(mesh.device)
+    with torch.cuda.device(mesh.device):
+        rc = _lib.load().sc_mesh_autotune_deposit(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
+                                                  z.data_ptr(), q.data_ptr(), x.numel(),
+                                                  *_bounds_ld(mesh), int(reps), times)
+    _check(rc, ctx)
+    ns = len(SLICE_LOG2)
+    return {(c, s): float(times[i * ns + j]) for i, c in enumerate(CHUNK_LOG2)
+            for j, s in enumerate(SLICE_LOG2)}
+
+
+def autotune_batched(meshes: Sequence[Mesh3D], xs, ys, zs, qs, *, reps: int = 5) -> list:
+    """autotune_deposit for every bunch of a batch (each timed alone); the
+    tilings then apply to deposit_batched and the single-bunch calls alike."""
+    return [autotune_deposit(m, x, y, z, q, reps=reps) for m, x, y, z, q in zip(meshes, xs, ys, zs, qs)]
 
 
 def sort_permutation(mesh: Mesh3D, particles_x, particles_y, particles_z) -> torch.Tensor:

@@ -22,6 +22,7 @@ _pi64 = ctypes.POINTER(ctypes.c_int64)
 _sz = ctypes.c_size_t
 _pvp = ctypes.POINTER(_vp)
 _pi = ctypes.POINTER(_i)
+_pf = ctypes.POINTER(ctypes.c_float)
 PROTOTYPES = {
     "sc_ctx_create": (_i, [_i, ctypes.POINTER(_vp)]),
     "sc_ctx_destroy": (_i, [_vp]),
@@ -44,6 +45,9 @@ PROTOTYPES = {
     "sc_mesh_workspace_bytes": (_i64, [_vp]),
     "sc_mesh_set_green_reuse": (_i, [_vp, _i]),
     "sc_mesh_force_generic_transforms": (_i, [_vp, _i]),
+    "sc_mesh_set_deposit_tiling": (_i, [_vp, _i, _i]),
+    "sc_mesh_get_deposit_tiling": (_i, [_vp, _pi, _pi]),
+    "sc_mesh_autotune_deposit": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64, _pd, _pd, _i, _pf]),
     "sc_bounds_from_particles": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pi64, _i, _pd, _pd, _pd]),
     "sc_mesh_auto_bounds": (_i, [_vp, _i, _vp, _vp, _vp, _i64]),
     "sc_mesh_read_bounds": (_i, [_vp, _pd, _pd, _pd]),
