@@ -156,6 +156,7 @@ int sc_ctx_destroy(sc_ctx *ctx)
     }
     for (auto e : c->bjoin) (void)hipEventDestroy(e);
     if (c->bfork) (void)hipEventDestroy(c->bfork);
+    if (c->gtable) (void)hipFree(c->gtable);
     delete ctx;
     return SC_OK;
 }
@@ -685,6 +686,13 @@ int sc_interpolate_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void 
 }
 
 // ---- many bunches per call ----
+int sc_ctx_set_grouped_launch(sc_ctx *ctx, int enable)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    ctx->c.grouped = enable != 0;
+    return SC_OK;
+}
+
 int sc_ctx_set_pool_size(sc_ctx *ctx, int nstreams)
 {
     if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
@@ -789,6 +797,15 @@ int sc_deposit_batched(sc_mesh *const *meshes, int nb, int pdtype, const void *c
             return set_err(c, SC_ERR_INVALID, "NULL particle pointer");
         if (int e = check_geom(c, min_bounds + 3 * b, delta + 3 * b)) return e;
     }
+    if (int e = check_dev(c)) return e;
+    {
+        // one launch per phase for all bunches when the batch qualifies
+        std::vector<Mesh *> ms(nb);
+        for (int b = 0; b < nb; ++b) ms[b] = &meshes[b]->m;
+        const int g = launch_deposit_grouped(c, ms.data(), nb, pdtype, x, y, z, q, n, min_bounds,
+                                             delta, clear);
+        if (g != SC_ERR_UNSUPPORTED) return g;
+    }
     return run_batched(c, nb, [&](int b) {
         return launch_deposit(&meshes[b]->m, pdtype, x[b], y[b], z[b], q[b], n[b], min_bounds + 3 * b,
                               delta + 3 * b, clear);
@@ -833,6 +850,14 @@ int sc_interpolate_batched(sc_mesh *const *meshes, int nb, int pdtype, const voi
         if (n[b] > 0 && (!x[b] || !y[b] || !z[b] || !Ex[b] || !Ey[b] || !Ez[b]))
             return set_err(c, SC_ERR_INVALID, "NULL particle or output pointer");
         if (int e = check_geom(c, min_bounds + 3 * b, delta + 3 * b)) return e;
+    }
+    if (int e = check_dev(c)) return e;
+    {
+        std::vector<Mesh *> ms(nb);
+        for (int b = 0; b < nb; ++b) ms[b] = &meshes[b]->m;
+        const int g = launch_interp_grouped(c, ms.data(), nb, pdtype, x, y, z, n, min_bounds, delta,
+                                            Ex, Ey, Ez);
+        if (g != SC_ERR_UNSUPPORTED) return g;
     }
     return run_batched(c, nb, [&](int b) {
         return launch_interp(&meshes[b]->m, pdtype, x[b], y[b], z[b], n[b], min_bounds + 3 * b,

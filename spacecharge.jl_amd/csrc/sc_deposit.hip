@@ -33,6 +33,7 @@
 // kMaxTiles use the one-pass atomic kernel instead.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <type_traits>
 
 #include "sc_internal.h"
@@ -126,16 +127,16 @@ struct Split {
 // Each lane loads kBatch particles before it computes any, so the loads of
 // a wave overlap.
 template <typename PT, typename MT>
-__global__ void __launch_bounds__(kBinThreads)
-k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
-            Split sp, Geo g, unsigned *__restrict__ cmat)
+__device__ __forceinline__ void bin_count_body(unsigned w, const PT *__restrict__ px,
+                                               const PT *__restrict__ py, const PT *__restrict__ pz,
+                                               Split sp, Geo g, unsigned *__restrict__ cmat)
 {
     resolve_geo(g);
     __shared__ unsigned hist[kMaxTiles];
     const int ntiles = g.tx * g.ty * g.tz;
     for (int t = threadIdx.x; t < ntiles; t += kBinThreads) hist[t] = 0u;
     __syncthreads();
-    const int64_t first = (int64_t)blockIdx.x * sp.slice;
+    const int64_t first = (int64_t)w * sp.slice;
     const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
     for (int64_t base = first; base < end; base += kBatch * kBinThreads) {
         PT x[kBatch], y[kBatch], z[kBatch];
@@ -155,17 +156,25 @@ k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__re
     }
     __syncthreads();
     for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
-        cmat[(size_t)t * sp.G + blockIdx.x] = hist[t];
+        cmat[(size_t)t * sp.G + w] = hist[t];
+}
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(kBinThreads)
+k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
+            Split sp, Geo g, unsigned *__restrict__ cmat)
+{
+    bin_count_body<PT, MT>(blockIdx.x, px, py, pz, sp, g, cmat);
 }
 
 // One wave per tile: exclusive prefix of the tile's row over the slices (in
 // place), and the tile's total.
-__global__ void __launch_bounds__(256)
-k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restrict__ tot)
+__device__ __forceinline__ void bin_rowscan_body(unsigned bx, unsigned *__restrict__ cmat, int G,
+                                                 int ntiles, unsigned *__restrict__ tot)
 {
     constexpr int kPer = kBinMaxSlices / 64;
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int t = (int)bx * 4 + (threadIdx.x >> 6);
     if (t >= ntiles) return;
     unsigned *row = cmat + (size_t)t * G;
     // lane l owns the V consecutive slices [l V, l V + V); all loads first
@@ -193,15 +202,23 @@ k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restri
     if (lane == 63) tot[t] = incl;
 }
 
+__global__ void __launch_bounds__(256)
+k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restrict__ tot)
+{
+    bin_rowscan_body(blockIdx.x, cmat, G, ntiles, tot);
+}
+
 // One workgroup.  Per-tile totals -> tile start (exclusive offsets), each
 // tile's first chunk, the chunk table; clears the extrapolation flags.
 // (Folding this into k_bin_rowscan as "last workgroup done" costs more on
 // MI355X than the launch it saves: the device-scope fences write back L2.)
-__global__ void __launch_bounds__(256)
-k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
-           unsigned *__restrict__ extra_base, int4 *__restrict__ chunks,
-           unsigned *__restrict__ nchunks, unsigned *__restrict__ ext_flags, int ntiles,
-           int chunk_log2)
+__device__ __forceinline__ void bin_scan_body(const unsigned *__restrict__ counts,
+                                              unsigned *__restrict__ cursor,
+                                              unsigned *__restrict__ extra_base,
+                                              int4 *__restrict__ chunks,
+                                              unsigned *__restrict__ nchunks,
+                                              unsigned *__restrict__ ext_flags, int ntiles,
+                                              int chunk_log2)
 {
     const unsigned chunk = 1u << chunk_log2;
     constexpr int kPer = kMaxTiles / 256;
@@ -265,6 +282,15 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
     }
 }
 
+__global__ void __launch_bounds__(256)
+k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
+           unsigned *__restrict__ extra_base, int4 *__restrict__ chunks,
+           unsigned *__restrict__ nchunks, unsigned *__restrict__ ext_flags, int ntiles,
+           int chunk_log2)
+{
+    bin_scan_body(counts, cursor, extra_base, chunks, nchunks, ext_flags, ntiles, chunk_log2);
+}
+
 // Each slice's particles go to its reserved run inside each tile: the LDS
 // cursor of tile t starts at start[t] + (particles of t in earlier slices),
 // and an LDS atomic hands out the slots.  The same split and the same
@@ -275,25 +301,24 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
 // same order to q1s, q2s and their per-slice max|q| after channel 0's
 // (kBinMaxSlices apart).
 template <typename PT, typename MT, int NCH>
-__global__ void __launch_bounds__(kBinThreads)
-k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
-              const PT *__restrict__ pq, Split sp, Geo g, const unsigned *__restrict__ cmat,
-              const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted,
-              unsigned *__restrict__ ext_flags, unsigned long long *__restrict__ slice_qmax,
-              const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s,
-              PT *__restrict__ q2s)
+__device__ __forceinline__ void bin_scatter_body(
+    unsigned w, const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
+    const PT *__restrict__ pq, Split sp, Geo g, const unsigned *__restrict__ cmat,
+    const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted,
+    unsigned *__restrict__ ext_flags, unsigned long long *__restrict__ slice_qmax,
+    const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s, PT *__restrict__ q2s)
 {
     resolve_geo(g);
     __shared__ unsigned cur[kMaxTiles];
     __shared__ unsigned char ext_lds[kMaxTiles];
     const int ntiles = g.tx * g.ty * g.tz;
     for (int t = threadIdx.x; t < ntiles; t += kBinThreads) {
-        cur[t] = start[t] + cmat[(size_t)t * sp.G + blockIdx.x];
+        cur[t] = start[t] + cmat[(size_t)t * sp.G + w];
         ext_lds[t] = 0;
     }
     __syncthreads();
     double qm = 0.0, qm1 = 0.0, qm2 = 0.0;
-    const int64_t first = (int64_t)blockIdx.x * sp.slice;
+    const int64_t first = (int64_t)w * sp.slice;
     const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
     for (int64_t base = first; base < end; base += kBatch * kBinThreads) {
         Rec<PT> r[kBatch];
@@ -361,10 +386,23 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
         const int c = threadIdx.x;
         unsigned long long m = wmax[c][0];
         for (int k = 1; k < kBinThreads / 64; ++k) m = wmax[c][k] > m ? wmax[c][k] : m;
-        slice_qmax[(size_t)c * kBinMaxSlices + blockIdx.x] = m;
+        slice_qmax[(size_t)c * kBinMaxSlices + w] = m;
     }
     for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
         if (ext_lds[t]) atomicOr(&ext_flags[t], 1u);  // rare: particles outside the mesh
+}
+
+template <typename PT, typename MT, int NCH>
+__global__ void __launch_bounds__(kBinThreads)
+k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
+              const PT *__restrict__ pq, Split sp, Geo g, const unsigned *__restrict__ cmat,
+              const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted,
+              unsigned *__restrict__ ext_flags, unsigned long long *__restrict__ slice_qmax,
+              const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s,
+              PT *__restrict__ q2s)
+{
+    bin_scatter_body<PT, MT, NCH>(blockIdx.x, px, py, pz, pq, sp, g, cmat, start, sorted,
+                                  ext_flags, slice_qmax, q1, q2, q1s, q2s);
 }
 
 // The particle order of the binning as a permutation: perm[slot] = particle
@@ -486,21 +524,19 @@ __device__ __forceinline__ void deposit_chunk(const Rec<PT> *__restrict__ src,
 // slice maxima kBinMaxSlices * c in, and writes sub-grid set c (pstride
 // doubles apart).
 template <typename PT, typename MT>
-__global__ void __launch_bounds__(kDepThreads)
-k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
-                const PT *__restrict__ q2s, const int4 *__restrict__ chunks,
-                const unsigned *__restrict__ nchunks, Geo g, double *__restrict__ partials,
-                size_t pstride, const unsigned *__restrict__ ext_flags,
-                const unsigned long long *__restrict__ slice_qmax, int nslices, int chunk_log2)
+__device__ __forceinline__ void deposit_tiles_body(
+    unsigned bx, int chan, const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
+    const PT *__restrict__ q2s, const int4 *__restrict__ chunks, const unsigned *__restrict__ nchunks,
+    Geo g, double *__restrict__ partials, size_t pstride, const unsigned *__restrict__ ext_flags,
+    const unsigned long long *__restrict__ slice_qmax, int nslices, int chunk_log2)
 {
     resolve_geo(g);
     __shared__ double sub[kSubVol];
     __shared__ unsigned long long wmax[kDepThreads / 64];
-    if (blockIdx.x >= *nchunks) return;
-    const int chan = blockIdx.y;
+    if (bx >= *nchunks) return;
     slice_qmax += (size_t)chan * kBinMaxSlices;
     partials += chan * pstride;
-    const int4 ch = chunks[blockIdx.x];
+    const int4 ch = chunks[bx];
     const int t = ch.x;
     const int ox = (t % g.tx) * kTile, oy = ((t / g.tx) % g.ty) * kTile, oz = (t / (g.tx * g.ty)) * kTile;
     // fixed point for Float32 meshes when every weight of the tile is in
@@ -549,19 +585,32 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
     }
 }
 
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(kDepThreads)
+k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
+                const PT *__restrict__ q2s, const int4 *__restrict__ chunks,
+                const unsigned *__restrict__ nchunks, Geo g, double *__restrict__ partials,
+                size_t pstride, const unsigned *__restrict__ ext_flags,
+                const unsigned long long *__restrict__ slice_qmax, int nslices, int chunk_log2)
+{
+    deposit_tiles_body<PT, MT>(blockIdx.x, blockIdx.y, sorted, q1s, q2s, chunks, nchunks, g,
+                               partials, pstride, ext_flags, slice_qmax, nslices, chunk_log2);
+}
+
 // Completes the per-tile sums in sub-grid slots [0, ntiles): adds a tile's
 // further chunks (slots extra_base[t], ...) to slot t, or zeroes slot t of
 // an empty tile.  Grid (ceil(kSubVol / 256), tiles, channels), one lane per
 // node; channel c's sub-grids start c * pstride doubles in.
-__global__ void __launch_bounds__(256)
-k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__restrict__ counts,
-              const unsigned *__restrict__ extra_base, int chunk_log2)
+__device__ __forceinline__ void tile_reduce_body(unsigned bx, int t, int chan,
+                                                 double *__restrict__ partials, size_t pstride,
+                                                 const unsigned *__restrict__ counts,
+                                                 const unsigned *__restrict__ extra_base,
+                                                 int chunk_log2)
 {
-    const int t = blockIdx.y;
-    partials += blockIdx.z * pstride;
+    partials += chan * pstride;
     const unsigned nk = (counts[t] + (1u << chunk_log2) - 1) >> chunk_log2;
     if (nk == 1) return;  // slot t already holds the sum
-    const int local = blockIdx.x * blockDim.x + threadIdx.x;
+    const int local = (int)bx * blockDim.x + threadIdx.x;
     if (local >= kSubVol) return;
     double *dst = partials + (size_t)t * kSubVol + local;
     if (nk == 0) {
@@ -584,6 +633,14 @@ k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__r
     *dst = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
+__global__ void __launch_bounds__(256)
+k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__restrict__ counts,
+              const unsigned *__restrict__ extra_base, int chunk_log2)
+{
+    tile_reduce_body(blockIdx.x, blockIdx.y, blockIdx.z, partials, pstride, counts, extra_base,
+                     chunk_log2);
+}
+
 // Node-parallel sum over the (up to 8) tiles whose sub-grid holds the node:
 // a node lies in two tiles along an axis when it is on a tile face.  Tile
 // t's sum is in sub-grid slot t (k_tile_reduce), so every address follows
@@ -592,16 +649,12 @@ k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__r
 // Grid (ceil(M / 256), channels): channel c reads sub-grid set c (pstride
 // doubles apart) and writes rho_c.
 template <typename MT>
-__global__ void __launch_bounds__(256)
-k_gather_tiles(const double *__restrict__ partials, size_t pstride, Geo g, MT *__restrict__ rho0,
-               MT *__restrict__ rho1, MT *__restrict__ rho2, int accumulate)
+__device__ __forceinline__ void gather_tiles_body(unsigned bx, const double *__restrict__ partials,
+                                                  const Geo &g, MT *__restrict__ rho, int accumulate)
 {
-    const int chan = blockIdx.y;
-    partials += chan * pstride;
-    MT *__restrict__ rho = chan == 0 ? rho0 : (chan == 1 ? rho1 : rho2);
     // at most kMaxTiles tiles of 16^3 cells: node indices fit in 32 bits
     const int M = g.nx * g.ny * g.nz;
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = (int)(bx * blockDim.x + threadIdx.x);
     if (i >= M) return;
     const int gx = i % g.nx;
     const int gy = (i / g.nx) % g.ny;
@@ -622,6 +675,16 @@ k_gather_tiles(const double *__restrict__ partials, size_t pstride, Geo g, MT *_
                 s += partials[(size_t)t * kSubVol + local];
             }
     rho[i] = accumulate ? (MT)((double)rho[i] + s) : (MT)s;
+}
+
+template <typename MT>
+__global__ void __launch_bounds__(256)
+k_gather_tiles(const double *__restrict__ partials, size_t pstride, Geo g, MT *__restrict__ rho0,
+               MT *__restrict__ rho1, MT *__restrict__ rho2, int accumulate)
+{
+    const int chan = blockIdx.y;
+    gather_tiles_body<MT>(blockIdx.x, partials + chan * pstride, g,
+                          chan == 0 ? rho0 : (chan == 1 ? rho1 : rho2), accumulate);
 }
 
 // At most kBinMaxSlices slices of about 2^slice_log2 particles.
@@ -887,5 +950,285 @@ int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const 
 {
     return launch_deposit_binned_multi(m, pdt, x, y, z, &q, 1, n, lo, dl, clear, &out);
 }
+
+
+// ---- grouped launches: the batched deposit with one launch per phase ----
+//
+// Each phase of the binned deposit is one launch over the workgroups of all
+// bunches: workgroup `blk` of a phase belongs to the last bunch b with
+// start[phase][b] <= blk, and runs that bunch's body with local index
+// blk - start[phase][b].  The per-bunch descriptors and the block starts
+// live in a table in device memory written by small store launches whose
+// arguments carry the descriptors by value, so a captured graph rewrites
+// the table on every replay.  All launches are on one stream: a later
+// grouped call's store runs after this call's phases have read the table.
+
+namespace {
+
+enum { kGCount, kGRowscan, kGScan, kGScatter, kGDeposit, kGReduce, kGGather, kGPhases };
+constexpr int kReduceBlocks = (kSubVol + 255) / 256;  // k_tile_reduce workgroups per tile
+
+struct GBunch {
+    const void *x, *y, *z, *q;
+    Split sp;
+    Geo g;
+    unsigned *counts;  // the bins' counts block (totals | cursor | chunk base | flags | nchunks | qmax)
+    unsigned *cmat;
+    int4 *chunks;
+    void *sorted;
+    double *partials;
+    void *rho;
+    int chunk_log2;
+};
+
+struct GTable {
+    unsigned start[kGPhases][kGroupMax + 1];
+    GBunch b[kGroupMax];
+    int nb, accumulate;
+};
+
+// by-value argument of one store launch: descriptors first .. first+count-1
+// and their block starts (entry count = the next bunch's start)
+struct GStore {
+    int first, count, nb, accumulate;
+    unsigned start[kGPhases][kGroupStore + 1];
+    GBunch b[kGroupStore];
+};
+static_assert(sizeof(GStore) <= 2048, "store-launch arguments stay small");
+
+// one lane per 8-byte word (vector stores)
+__global__ void __launch_bounds__(256) k_group_store(GStore s, GTable *__restrict__ T)
+{
+    constexpr int kWords = (int)(sizeof(GBunch) / 8);
+    static_assert(sizeof(GBunch) % 8 == 0, "descriptor of whole words");
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(s.b);
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(T->b + s.first);
+    for (int i = threadIdx.x; i < s.count * kWords; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kGPhases * (s.count + 1); i += blockDim.x) {
+        const int ph = i / (s.count + 1), k = i - ph * (s.count + 1);
+        T->start[ph][s.first + k] = s.start[ph][k];
+    }
+    if (threadIdx.x == 0) {
+        T->nb = s.nb;
+        T->accumulate = s.accumulate;
+    }
+}
+
+// the bunch owning workgroup blk of a phase: the last b with start[b] <= blk
+__device__ __forceinline__ int group_of(const unsigned *__restrict__ start, int nb, unsigned blk)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define SC_GROUP_PROLOGUE(PH)                                    \
+    const int b_ = group_of(T->start[PH], T->nb, blockIdx.x);    \
+    const GBunch &d = T->b[b_];                                  \
+    const unsigned lb = blockIdx.x - T->start[PH][b_];           \
+    unsigned *cnt = d.counts;
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(kBinThreads) k_bin_count_g(const GTable *__restrict__ T)
+{
+    SC_GROUP_PROLOGUE(kGCount)
+    (void)cnt;
+    bin_count_body<PT, MT>(lb, (const PT *)d.x, (const PT *)d.y, (const PT *)d.z, d.sp, d.g, d.cmat);
+}
+
+__global__ void __launch_bounds__(256) k_bin_rowscan_g(const GTable *__restrict__ T)
+{
+    SC_GROUP_PROLOGUE(kGRowscan)
+    bin_rowscan_body(lb, d.cmat, d.sp.G, d.g.tx * d.g.ty * d.g.tz, cnt);
+}
+
+__global__ void __launch_bounds__(256) k_bin_scan_g(const GTable *__restrict__ T)
+{
+    SC_GROUP_PROLOGUE(kGScan)
+    (void)lb;
+    bin_scan_body(cnt, cnt + kMaxTiles, cnt + 2 * kMaxTiles, d.chunks, cnt + 4 * kMaxTiles,
+                  cnt + 3 * kMaxTiles, d.g.tx * d.g.ty * d.g.tz, d.chunk_log2);
+}
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(kBinThreads) k_bin_scatter_g(const GTable *__restrict__ T)
+{
+    SC_GROUP_PROLOGUE(kGScatter)
+    bin_scatter_body<PT, MT, 1>(lb, (const PT *)d.x, (const PT *)d.y, (const PT *)d.z,
+                                (const PT *)d.q, d.sp, d.g, d.cmat, cnt + kMaxTiles,
+                                static_cast<Rec<PT> *>(d.sorted), cnt + 3 * kMaxTiles,
+                                reinterpret_cast<unsigned long long *>(cnt + 4 * kMaxTiles + 4),
+                                nullptr, nullptr, nullptr, nullptr);
+}
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(kDepThreads) k_deposit_tiles_g(const GTable *__restrict__ T)
+{
+    SC_GROUP_PROLOGUE(kGDeposit)
+    deposit_tiles_body<PT, MT>(lb, 0, static_cast<const Rec<PT> *>(d.sorted), nullptr, nullptr,
+                               d.chunks, cnt + 4 * kMaxTiles, d.g, d.partials, 0,
+                               cnt + 3 * kMaxTiles,
+                               reinterpret_cast<const unsigned long long *>(cnt + 4 * kMaxTiles + 4),
+                               d.sp.G, d.chunk_log2);
+}
+
+__global__ void __launch_bounds__(256) k_tile_reduce_g(const GTable *__restrict__ T)
+{
+    SC_GROUP_PROLOGUE(kGReduce)
+    tile_reduce_body(lb % kReduceBlocks, (int)(lb / kReduceBlocks), 0, d.partials, 0, cnt,
+                     cnt + 2 * kMaxTiles, d.chunk_log2);
+}
+
+template <typename MT>
+__global__ void __launch_bounds__(256) k_gather_tiles_g(const GTable *__restrict__ T)
+{
+    SC_GROUP_PROLOGUE(kGGather)
+    (void)cnt;
+    gather_tiles_body<MT>(lb, d.partials, d.g, static_cast<MT *>(d.rho), T->accumulate);
+}
+
+#undef SC_GROUP_PROLOGUE
+
+template <typename PT, typename MT>
+void launch_groups(Ctx *ctx, GTable *T, const unsigned *total)
+{
+    hipStream_t st = ctx->stream;
+    launch_begin(ctx, K_BIN_COUNT);
+    hipLaunchKernelGGL((k_bin_count_g<PT, MT>), dim3(total[kGCount]), dim3(kBinThreads), 0, st,
+                       (const GTable *)T);
+    launch_end(ctx, K_BIN_COUNT);
+    launch_begin(ctx, K_BIN_ROWSCAN);
+    hipLaunchKernelGGL(k_bin_rowscan_g, dim3(total[kGRowscan]), dim3(256), 0, st, (const GTable *)T);
+    launch_end(ctx, K_BIN_ROWSCAN);
+    launch_begin(ctx, K_BIN_SCAN);
+    hipLaunchKernelGGL(k_bin_scan_g, dim3(total[kGScan]), dim3(256), 0, st, (const GTable *)T);
+    launch_end(ctx, K_BIN_SCAN);
+    launch_begin(ctx, K_BIN_SCATTER);
+    hipLaunchKernelGGL((k_bin_scatter_g<PT, MT>), dim3(total[kGScatter]), dim3(kBinThreads), 0, st,
+                       (const GTable *)T);
+    launch_end(ctx, K_BIN_SCATTER);
+    launch_begin(ctx, K_DEPOSIT_TILES);
+    hipLaunchKernelGGL((k_deposit_tiles_g<PT, MT>), dim3(total[kGDeposit]), dim3(kDepThreads), 0, st,
+                       (const GTable *)T);
+    launch_end(ctx, K_DEPOSIT_TILES);
+    launch_begin(ctx, K_TILE_REDUCE);
+    hipLaunchKernelGGL(k_tile_reduce_g, dim3(total[kGReduce]), dim3(256), 0, st, (const GTable *)T);
+    launch_end(ctx, K_TILE_REDUCE);
+    launch_begin(ctx, K_GATHER_TILES);
+    hipLaunchKernelGGL((k_gather_tiles_g<MT>), dim3(total[kGGather]), dim3(256), 0, st,
+                       (const GTable *)T);
+    launch_end(ctx, K_GATHER_TILES);
+}
+
+}  // namespace
+
+int ensure_group_table(Ctx *ctx, size_t bytes)
+{
+    if (ctx->gtable) return SC_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SC_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return SC_ERR_UNSUPPORTED;  // no allocation under capture
+    if (hipMalloc(&ctx->gtable, bytes) != hipSuccess) {
+        ctx->gtable = nullptr;
+        return set_err(ctx, SC_ERR_NOMEM, "device allocation of the batch descriptor table failed");
+    }
+    return SC_OK;
+}
+
+size_t group_table_bytes();
+
+// Qualifies: at most kGroupMax bunches, one mesh type, every bunch with
+// particles (n in [1, 2^31)) and at most kMaxTiles tiles.
+int launch_deposit_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void *const *x,
+                           const void *const *y, const void *const *z, const void *const *q,
+                           const int64_t *n, const double *lo, const double *dl, int clear)
+{
+    if (!ctx->grouped || nb < 1 || nb > kGroupMax) return SC_ERR_UNSUPPORTED;
+    const int mdt = ms[0]->dtype;
+    std::vector<Geo> geos(nb);
+    for (int b = 0; b < nb; ++b) {
+        const Mesh *m = ms[b];
+        if (m->dtype != mdt || n[b] < 1 || n[b] > 0x7fffffffLL) return SC_ERR_UNSUPPORTED;
+        Geo &g = geos[b];
+        g.nx = (int)m->n[0];
+        g.ny = (int)m->n[1];
+        g.nz = (int)m->n[2];
+        g.tx = (g.nx - 1 + kTile - 1) / kTile;
+        g.ty = (g.ny - 1 + kTile - 1) / kTile;
+        g.tz = (g.nz - 1 + kTile - 1) / kTile;
+        if ((int64_t)g.tx * g.ty * g.tz > kMaxTiles) return SC_ERR_UNSUPPORTED;
+        g.dev = nullptr;
+        for (int a = 0; a < 3; ++a) {
+            g.lo[a] = lo[3 * b + a];
+            g.dl[a] = dl[3 * b + a];
+        }
+    }
+    if (int rc = ensure_group_table(ctx, group_table_bytes())) return rc;
+    for (int b = 0; b < nb; ++b)
+        if (int rc = ensure_bins(ms[b], n[b], pdt, geos[b].tx * geos[b].ty * geos[b].tz, 1)) return rc;
+    // where each mesh's deposit starts on the stream (a later solve's Green
+    // build on the side stream waits for it, as after launch_deposit)
+    for (int b = 0; b < nb; ++b) {
+        Workspace &ws = ms[b]->ws;
+        if (ws.deposit_start) {
+            SC_HIP(ctx, hipEventRecord(ws.deposit_start, ctx->stream));
+            ws.deposit_start_recorded = true;
+        }
+    }
+    GTable *T = static_cast<GTable *>(ctx->gtable);
+    // descriptors and block starts, stored kGroupStore at a time
+    unsigned total[kGPhases] = {};
+    GStore s;
+    std::memset(&s, 0, sizeof(s));
+    s.nb = nb;
+    s.accumulate = clear ? 0 : 1;
+    for (int b0 = 0; b0 < nb; b0 += kGroupStore) {
+        s.first = b0;
+        s.count = nb - b0 < kGroupStore ? nb - b0 : kGroupStore;
+        for (int k = 0; k < s.count; ++k) {
+            Mesh *m = ms[b0 + k];
+            DepositBins &B = m->bins;
+            const Geo &g = geos[b0 + k];
+            const int ntiles = g.tx * g.ty * g.tz;
+            const int64_t M = (int64_t)g.nx * g.ny * g.nz;
+            GBunch &d = s.b[k];
+            d.x = x[b0 + k];
+            d.y = y[b0 + k];
+            d.z = z[b0 + k];
+            d.q = q[b0 + k];
+            d.sp = split_of(n[b0 + k], m->slice_log2);
+            d.g = g;
+            d.counts = B.counts;
+            d.cmat = B.cmat;
+            d.chunks = static_cast<int4 *>(B.chunks);
+            d.sorted = B.sorted;
+            d.partials = static_cast<double *>(B.partials);
+            d.rho = m->rho;
+            d.chunk_log2 = m->chunk_log2;
+            const unsigned blocks[kGPhases] = {
+                (unsigned)d.sp.G, (unsigned)div_up4(ntiles), 1u, (unsigned)d.sp.G,
+                (unsigned)(ntiles + (n[b0 + k] >> m->chunk_log2) + 1),
+                (unsigned)(kReduceBlocks * ntiles), (unsigned)((M + 255) / 256)};
+            for (int ph = 0; ph < kGPhases; ++ph) {
+                s.start[ph][k] = total[ph];
+                total[ph] += blocks[ph];
+            }
+        }
+        for (int ph = 0; ph < kGPhases; ++ph) s.start[ph][s.count] = total[ph];
+        hipLaunchKernelGGL(k_group_store, dim3(1), dim3(256), 0, ctx->stream, s, T);
+    }
+    if (mdt == SC_F32 && pdt == SC_F32) launch_groups<float, float>(ctx, T, total);
+    else if (mdt == SC_F64 && pdt == SC_F64) launch_groups<double, double>(ctx, T, total);
+    else if (mdt == SC_F32) launch_groups<double, float>(ctx, T, total);
+    else launch_groups<float, double>(ctx, T, total);
+    SC_HIP(ctx, hipGetLastError());
+    return SC_OK;
+}
+
+size_t group_table_bytes() { return sizeof(GTable); }
 
 }  // namespace sc

@@ -104,6 +104,12 @@ struct Ctx {
     // set while a batched call launches: the solve keeps the Green build on
     // the bunch's own stream (the bunches overlap each other instead)
     bool in_batch = false;
+    // grouped launches (one launch per phase for all bunches of a batched
+    // deposit or interpolation): the per-bunch descriptor table in device
+    // memory, allocated on first use; `grouped` off falls back to one
+    // launch sequence per bunch on the stream pool
+    void *gtable = nullptr;
+    bool grouped = true;
 };
 
 // Device workspace of one mesh (lazy, like src/mesh.jl:49-72).
@@ -209,6 +215,24 @@ int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z
 // instead of m->rho.
 int launch_deposit_to(Mesh *m, int pdt, const void *x, const void *y, const void *z, const void *q,
                       int64_t n, const double lo[3], const double dl[3], int clear, void *out);
+// The deposit (launch_deposit) of nb bunches with one launch per phase,
+// grouped over the bunches, on ctx->stream.  lo and dl hold 3 values per
+// bunch.  Returns SC_ERR_UNSUPPORTED, having launched nothing, when the
+// batch does not qualify (see the definition).
+int launch_deposit_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void *const *x,
+                           const void *const *y, const void *const *z, const void *const *q,
+                           const int64_t *n, const double *lo, const double *dl, int clear);
+constexpr int kGroupMax = 256;  // bunches per grouped call
+constexpr int kGroupStore = 8;  // descriptors per store launch (kernel arguments stay small)
+// The context's descriptor table of grouped launches (allocated once, of
+// group_table_bytes(); SC_ERR_UNSUPPORTED when first needed under capture).
+int ensure_group_table(Ctx *ctx, size_t bytes);
+size_t group_table_bytes();
+// interpolate_field of nb bunches, grouped likewise (interleave, gather).
+int launch_interp_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void *const *x,
+                          const void *const *y, const void *const *z, const int64_t *n,
+                          const double *lo, const double *dl, void *const *Ex, void *const *Ey,
+                          void *const *Ez);
 int launch_deposit_binned(Mesh *m, int pdt, const void *x, const void *y, const void *z,
                           const void *q, int64_t n, const double lo[3], const double dl[3],
                           int clear, void *out);

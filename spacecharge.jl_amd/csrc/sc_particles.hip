@@ -14,6 +14,7 @@
 // which the atomics add up differs.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <type_traits>
 
 #include "sc_internal.h"
@@ -97,10 +98,9 @@ struct V4<double> {
 };
 
 template <typename MT>
-__global__ void __launch_bounds__(256)
-k_interleave(const MT *__restrict__ ef, typename V4<MT>::type *__restrict__ efi, int64_t M)
+__device__ __forceinline__ void interleave_body(int64_t i, const MT *__restrict__ ef,
+                                                typename V4<MT>::type *__restrict__ efi, int64_t M)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M) return;
     typename V4<MT>::type v;
     v.x = ef[i];
@@ -110,17 +110,22 @@ k_interleave(const MT *__restrict__ ef, typename V4<MT>::type *__restrict__ efi,
     efi[i] = v;
 }
 
-template <typename PT, typename MT>
+template <typename MT>
 __global__ void __launch_bounds__(256)
-k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
-         const PT *__restrict__ py, const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz,
-         double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
-         const double *__restrict__ geom, PT *__restrict__ Ex, PT *__restrict__ Ey,
-         PT *__restrict__ Ez)
+k_interleave(const MT *__restrict__ ef, typename V4<MT>::type *__restrict__ efi, int64_t M)
+{
+    interleave_body<MT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, ef, efi, M);
+}
+
+template <typename PT, typename MT>
+__device__ __forceinline__ void interp_body(
+    int64_t p, const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
+    const PT *__restrict__ py, const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz,
+    double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
+    const double *__restrict__ geom, PT *__restrict__ Ex, PT *__restrict__ Ey, PT *__restrict__ Ez)
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     device_geometry(geom, lx_, ly_, lz_, dx_, dy_, dz_);
     const C lx = (C)(MT)lx_, ly = (C)(MT)ly_, lz = (C)(MT)lz_;
@@ -151,6 +156,18 @@ k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ p
     Ey[p] = (PT)SC_ITP(y);
     Ez[p] = (PT)SC_ITP(z);
 #undef SC_ITP
+}
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(256)
+k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
+         const PT *__restrict__ py, const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz,
+         double lx_, double ly_, double lz_, double dx_, double dy_, double dz_,
+         const double *__restrict__ geom, PT *__restrict__ Ex, PT *__restrict__ Ey,
+         PT *__restrict__ Ez)
+{
+    interp_body<PT, MT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, efi, px, py, pz, n, nx, ny,
+                        nz, lx_, ly_, lz_, dx_, dy_, dz_, geom, Ex, Ey, Ez);
 }
 
 // ---- vector-Jacobian products of the CIC weights ----
@@ -693,6 +710,166 @@ int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z
         mn[a] = h[a];
         mx[a] = h[3 + a];
     }
+    return SC_OK;
+}
+
+
+// ---- grouped launches: the batched interpolation, one launch per phase ----
+// (the scheme of launch_deposit_grouped: a descriptor table in the
+// context's device memory, written by store launches whose arguments carry
+// the descriptors, then one interleave and one gather launch for all
+// bunches, workgroup -> bunch by the per-phase block starts)
+
+namespace {
+
+enum { kIInterleave, kIGather, kIPhases };
+
+struct IBunch {
+    const void *ef;
+    void *efi;
+    const void *x, *y, *z;
+    void *Ex, *Ey, *Ez;
+    int64_t n, M;
+    double l[3], d[3];
+    int nx, ny, nz, pad;
+};
+
+struct ITable {
+    unsigned start[kIPhases][kGroupMax + 1];
+    IBunch b[kGroupMax];
+    int nb;
+};
+
+struct IStore {
+    int first, count, nb, pad;
+    unsigned start[kIPhases][kGroupStore + 1];
+    IBunch b[kGroupStore];
+};
+static_assert(sizeof(IStore) <= 2048, "store-launch arguments stay small");
+
+// one lane per 8-byte word (vector stores)
+__global__ void __launch_bounds__(256) k_igroup_store(IStore s, ITable *__restrict__ T)
+{
+    constexpr int kWords = (int)(sizeof(IBunch) / 8);
+    static_assert(sizeof(IBunch) % 8 == 0, "descriptor of whole words");
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(s.b);
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(T->b + s.first);
+    for (int i = threadIdx.x; i < s.count * kWords; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < kIPhases * (s.count + 1); i += blockDim.x) {
+        const int ph = i / (s.count + 1), k = i - ph * (s.count + 1);
+        T->start[ph][s.first + k] = s.start[ph][k];
+    }
+    if (threadIdx.x == 0) T->nb = s.nb;
+}
+
+__device__ __forceinline__ int igroup_of(const unsigned *__restrict__ start, int nb, unsigned blk)
+{
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename MT>
+__global__ void __launch_bounds__(256) k_interleave_g(const ITable *__restrict__ T)
+{
+    const int b = igroup_of(T->start[kIInterleave], T->nb, blockIdx.x);
+    const IBunch &d = T->b[b];
+    const int64_t lb = blockIdx.x - T->start[kIInterleave][b];
+    interleave_body<MT>(lb * blockDim.x + threadIdx.x, static_cast<const MT *>(d.ef),
+                        static_cast<typename V4<MT>::type *>(d.efi), d.M);
+}
+
+template <typename PT, typename MT>
+__global__ void __launch_bounds__(256) k_interp_g(const ITable *__restrict__ T)
+{
+    const int b = igroup_of(T->start[kIGather], T->nb, blockIdx.x);
+    const IBunch &d = T->b[b];
+    const int64_t lb = blockIdx.x - T->start[kIGather][b];
+    interp_body<PT, MT>(lb * blockDim.x + threadIdx.x,
+                        static_cast<const typename V4<MT>::type *>(d.efi), (const PT *)d.x,
+                        (const PT *)d.y, (const PT *)d.z, d.n, d.nx, d.ny, d.nz, d.l[0], d.l[1],
+                        d.l[2], d.d[0], d.d[1], d.d[2], nullptr, (PT *)d.Ex, (PT *)d.Ey, (PT *)d.Ez);
+}
+
+template <typename PT, typename MT>
+void launch_igroups(Ctx *ctx, ITable *T, const unsigned *total)
+{
+    launch_begin(ctx, K_INTERLEAVE);
+    hipLaunchKernelGGL(k_interleave_g<MT>, dim3(total[kIInterleave]), dim3(256), 0, ctx->stream,
+                       (const ITable *)T);
+    launch_end(ctx, K_INTERLEAVE);
+    launch_begin(ctx, K_INTERP);
+    hipLaunchKernelGGL((k_interp_g<PT, MT>), dim3(total[kIGather]), dim3(256), 0, ctx->stream,
+                       (const ITable *)T);
+    launch_end(ctx, K_INTERP);
+}
+
+}  // namespace
+
+// Qualifies: at most kGroupMax bunches, one mesh type, every bunch with
+// particles; the interleaved field of each mesh is allocated outside.
+int launch_interp_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void *const *x,
+                          const void *const *y, const void *const *z, const int64_t *n,
+                          const double *lo, const double *dl, void *const *Ex, void *const *Ey,
+                          void *const *Ez)
+{
+    if (!ctx->grouped || nb < 1 || nb > kGroupMax || sizeof(ITable) > group_table_bytes())
+        return SC_ERR_UNSUPPORTED;
+    const int mdt = ms[0]->dtype;
+    for (int b = 0; b < nb; ++b)
+        if (ms[b]->dtype != mdt || n[b] < 1 || (n[b] + 255) / 256 > 0x7fffffffLL)
+            return SC_ERR_UNSUPPORTED;
+    if (int rc = ensure_group_table(ctx, group_table_bytes())) return rc;
+    for (int b = 0; b < nb; ++b)
+        if (int rc = ensure_efi(ms[b])) return rc;
+    ITable *T = static_cast<ITable *>(ctx->gtable);
+    unsigned total[kIPhases] = {};
+    IStore s;
+    std::memset(&s, 0, sizeof(s));
+    s.nb = nb;
+    for (int b0 = 0; b0 < nb; b0 += kGroupStore) {
+        s.first = b0;
+        s.count = nb - b0 < kGroupStore ? nb - b0 : kGroupStore;
+        for (int k = 0; k < s.count; ++k) {
+            const int b = b0 + k;
+            const Mesh *m = ms[b];
+            IBunch &d = s.b[k];
+            d.ef = m->efield;
+            d.efi = m->efi;
+            d.x = x[b];
+            d.y = y[b];
+            d.z = z[b];
+            d.Ex = Ex[b];
+            d.Ey = Ey[b];
+            d.Ez = Ez[b];
+            d.n = n[b];
+            d.M = m->n[0] * m->n[1] * m->n[2];
+            for (int a = 0; a < 3; ++a) {
+                d.l[a] = lo[3 * b + a];
+                d.d[a] = dl[3 * b + a];
+            }
+            d.nx = (int)m->n[0];
+            d.ny = (int)m->n[1];
+            d.nz = (int)m->n[2];
+            const unsigned blocks[kIPhases] = {(unsigned)blocks_for(d.M, 256),
+                                               (unsigned)blocks_for(d.n, 256)};
+            for (int ph = 0; ph < kIPhases; ++ph) {
+                s.start[ph][k] = total[ph];
+                total[ph] += blocks[ph];
+            }
+        }
+        for (int ph = 0; ph < kIPhases; ++ph) s.start[ph][s.count] = total[ph];
+        hipLaunchKernelGGL(k_igroup_store, dim3(1), dim3(256), 0, ctx->stream, s, T);
+    }
+    if (pdt == SC_F32 && mdt == SC_F32) launch_igroups<float, float>(ctx, T, total);
+    else if (pdt == SC_F64 && mdt == SC_F64) launch_igroups<double, double>(ctx, T, total);
+    else if (pdt == SC_F64 && mdt == SC_F32) launch_igroups<double, float>(ctx, T, total);
+    else launch_igroups<float, double>(ctx, T, total);
+    SC_HIP(ctx, hipGetLastError());
     return SC_OK;
 }
 

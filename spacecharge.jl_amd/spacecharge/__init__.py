@@ -59,6 +59,7 @@ __all__ = [
     "set_stream_pool_size",
     "autotune_deposit",
     "autotune_batched",
+    "set_grouped_launch",
 ]
 
 # src/utils.jl:7-8
@@ -597,6 +598,15 @@ def set_stream_pool_size(nstreams: int, device=None) -> None:
     dev = torch.device(device) if device is not None else _default_device()
     ctx = _ctx(dev)
     _check(_lib.load().sc_ctx_set_pool_size(ctx, int(nstreams)), ctx)
+
+
+def set_grouped_launch(enable: bool = True, device=None) -> None:
+    """Batched deposit and interpolation as one launch per phase for all
+    bunches (default), or as one launch sequence per bunch on the stream
+    pool (enable=False).  No reference counterpart."""
+    dev = torch.device(device) if device is not None else _default_device()
+    ctx = _ctx(dev)
+    _check(_lib.load().sc_ctx_set_grouped_launch(ctx, int(bool(enable))), ctx)
 
 
 def synchronize(device=None) -> None:

@@ -160,3 +160,77 @@ def test_batched_step_captured_in_a_graph():
         for b in range(len(BUNCHES)):
             for c in range(3):
                 assert torch.equal(Eg[b][c], Ee[b][c]), (rep, b, c)
+
+
+def _batch_run(sc, meshes, parts, clear=True):
+    sc.deposit_batched(meshes, *[[p[k] for p in parts] for k in range(4)], clear=clear)
+    rho = [m.rho.clone() for m in meshes]
+    sc.solve_batched(meshes)
+    E = sc.interpolate_batched(meshes, *[[p[k] for p in parts] for k in range(3)])
+    torch.cuda.synchronize()
+    return rho, E
+
+
+@pytest.mark.parametrize("T", [torch.float32, torch.float64])
+def test_grouped_launch_equals_stream_pool(T):
+    """20 uneven bunches (three descriptor-store launches) with per-bunch
+    deposit tilings: the grouped deposit and interpolation (one launch per
+    phase) against the per-bunch launch sequences on the stream pool, with
+    clear=True and clear=False; bit for bit in Float32, 1e-12 in Float64."""
+    sc = _sc()
+    rng = np.random.default_rng(5)
+    specs = [(int(10 ** rng.uniform(3, 5.5)), [(16, 16, 16), (32, 32, 32), (24, 20, 40),
+                                              (64, 64, 64)][b % 4]) for b in range(20)]
+    parts = [_bunch(300 + b, n, T) for b, (n, _) in enumerate(specs)]
+    tilings = [(10 + b % 5, 10 + b % 4) for b in range(20)]
+
+    def meshes():
+        ms = [sc.Mesh3D(g, p[0], p[1], p[2], T=T) for (_, g), p in zip(specs, parts)]
+        for m, t in zip(ms, tilings):
+            m.set_deposit_tiling(*t)
+        return ms
+
+    try:
+        out = {}
+        for grouped in (True, False):
+            sc.set_grouped_launch(grouped)
+            ms = meshes()
+            rho1, E1 = _batch_run(sc, ms, parts)
+            rho2, _ = _batch_run(sc, ms, parts, clear=False)  # adds to the solved rho
+            out[grouped] = (rho1, E1, rho2)
+    finally:
+        sc.set_grouped_launch(True)
+    (ga, Ea, ra), (gb, Eb, rb) = out[True], out[False]
+    for b in range(len(specs)):
+        for a, c in ((ga[b], gb[b]), (ra[b], rb[b])) + tuple((Ea[b][k], Eb[b][k]) for k in range(3)):
+            if T == torch.float32:
+                assert torch.equal(a, c), b
+            else:
+                assert _rel(a, c) <= 1e-12, b
+
+
+def test_batches_that_do_not_group_fall_back():
+    """A bunch without particles, or meshes of two types, take the per-bunch
+    path and still equal the single-bunch calls."""
+    sc = _sc()
+    parts = [_bunch(400 + b, n, torch.float32) for b, n in enumerate((5000, 8000, 3000))]
+    empty = [t[:0] for t in parts[1]]
+    cases = [([torch.float32] * 3, [parts[0], empty, parts[2]]),
+             ([torch.float32, torch.float64, torch.float32], parts)]
+    for types, ps in cases:
+        ms = [sc.Mesh3D((16, 16, 16), (-5e-3,) * 3, (5e-3,) * 3, T=T) for T in types]
+        ref = [sc.Mesh3D((16, 16, 16), (-5e-3,) * 3, (5e-3,) * 3, T=T) for T in types]
+        rho, E = _batch_run(sc, ms, ps)
+        for m, p, r, e in zip(ref, ps, rho, E):
+            sc.deposit(m, *p)
+            sc.solve(m)
+            Es = sc.interpolate_field(m, p[0], p[1], p[2])
+            torch.cuda.synchronize()
+            if m.T == torch.float32:
+                assert torch.equal(r, m.rho)
+                for c in range(3):
+                    assert torch.equal(e[c], Es[c])
+            else:  # Float64 deposits add in LDS-atomic order
+                assert _rel(r, m.rho) <= 1e-12
+                for c in range(3):
+                    assert _rel(e[c], Es[c]) <= 1e-12
