@@ -169,6 +169,21 @@ def bench_batched(args, sc, torch, dev, T):
     with torch.cuda.graph(graph_t):
         step_batched()
     ms_graph_tuned = timed(graph_t.replay)
+
+    # the same captured step with fewer grouped launches (same process):
+    # particle phases only, then one launch sequence per bunch throughout
+    by_mode = {}
+    for name, particles, solve in (("particles_only", True, False), ("none", False, False)):
+        sc.set_grouped_launch(particles, solve)
+        with torch.cuda.stream(side):
+            step_batched()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g_m = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g_m):
+            step_batched()
+        by_mode[name] = {"eager_ms_per_step": timed(step_batched), "graph_ms_per_step": timed(g_m.replay)}
+    sc.set_grouped_launch(True, True)
     return {
         "metric": "particles/s per full space-charge step (deposit+solve+interpolate), many bunches",
         "value": ntot / (ms * 1e-3),
@@ -199,6 +214,8 @@ def bench_batched(args, sc, torch, dev, T):
         "autotuned_graph_ms_per_step": ms_graph_tuned,
         "autotuned_graph_particles_per_s": ntot / (ms_graph_tuned * 1e-3),
         "deposit_tilings": tilings,
+        "grouped_launches": "particle phases and same-shape small-mesh solves (default)",
+        "other_grouping": by_mode,
     }
 
 

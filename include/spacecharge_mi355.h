@@ -260,19 +260,28 @@ int sc_interpolate_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void 
  * stream then waits for all of them, so a batched call is ordered like a
  * sequence of its single-bunch counterparts and computes the same results.
  * Per-bunch arrays have nb entries; bounds, deltas have 3*nb (bunch-major).
- * Grouped launches (sc_ctx_set_grouped_launch, on by default): when every
- * bunch has particles, all meshes share one type and nb <= 256,
- * sc_deposit_batched and sc_interpolate_batched instead run each phase of
- * the binned deposit (7 phases) and of the interpolation (2) as ONE launch
- * over the workgroups of all bunches, on the context's stream.  The
- * per-bunch descriptors go to a table in the context's device memory
+ * Grouped launches (sc_ctx_set_grouped_launch, phases a combination of
+ * SC_GROUP_PARTICLES and SC_GROUP_SOLVE, both by default):
+ *  - SC_GROUP_PARTICLES: when every bunch has particles, all meshes share
+ *    one type and nb <= 256, sc_deposit_batched and sc_interpolate_batched
+ *    run each phase of the binned deposit (7 phases) and of the
+ *    interpolation (2) as ONE launch over the workgroups of all bunches, on
+ *    the context's stream;
+ *  - SC_GROUP_SOLVE: sc_solve_batched solves each class of two or more
+ *    meshes of one shape, type and cathode flag (transform lengths 32..512,
+ *    no Green reuse, at most 64 MB of spectra per class, so small meshes)
+ *    with one launch per Green and transform kernel; classes and the other
+ *    bunches share the stream pool.
+ * The per-bunch descriptors go to tables in the context's device memory
  * through store launches that carry them as kernel arguments, so a graph
  * capture replays them; grouped calls of one context must not overlap
- * (they share the table).  The first grouped call allocates the table and
+ * (they share the tables).  The first grouped call allocates the tables and
  * must run outside stream capture (under capture it falls back to the pool).
  * Results are the same bits as the pool path for Float32 meshes. */
+#define SC_GROUP_PARTICLES 1
+#define SC_GROUP_SOLVE 2
 int sc_ctx_set_pool_size(sc_ctx *ctx, int nstreams);
-int sc_ctx_set_grouped_launch(sc_ctx *ctx, int enable);
+int sc_ctx_set_grouped_launch(sc_ctx *ctx, int phases);
 int sc_deposit_batched(sc_mesh *const *meshes, int nb, int pdtype, const void *const *x,
                        const void *const *y, const void *const *z, const void *const *q,
                        const int64_t *n, const double *min_bounds, const double *delta, int clear);

@@ -157,6 +157,7 @@ int sc_ctx_destroy(sc_ctx *ctx)
     for (auto e : c->bjoin) (void)hipEventDestroy(e);
     if (c->bfork) (void)hipEventDestroy(c->bfork);
     if (c->gtable) (void)hipFree(c->gtable);
+    if (c->stable) (void)hipFree(c->stable);
     delete ctx;
     return SC_OK;
 }
@@ -686,10 +687,12 @@ int sc_interpolate_adjoint(sc_mesh *mesh, int pdtype, const void *x, const void 
 }
 
 // ---- many bunches per call ----
-int sc_ctx_set_grouped_launch(sc_ctx *ctx, int enable)
+int sc_ctx_set_grouped_launch(sc_ctx *ctx, int phases)
 {
     if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
-    ctx->c.grouped = enable != 0;
+    if (phases & ~(SC_GROUP_PARTICLES | SC_GROUP_SOLVE))
+        return set_err(&ctx->c, SC_ERR_INVALID, "phases must combine SC_GROUP_PARTICLES and SC_GROUP_SOLVE");
+    ctx->c.grouped = phases;
     return SC_OK;
 }
 
@@ -827,6 +830,64 @@ int sc_solve_batched(sc_mesh *const *meshes, int nb, const double *min_bounds,
             return set_err(c, SC_ERR_INVALID, "gamma must be finite and positive");
         if (at_cathode[b] && !std::isfinite(max_bounds[3 * b + 2]))
             return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
+    }
+    if (int e = check_dev(c)) return e;
+    if ((c->grouped & SC_GROUP_SOLVE) && nb > 1 && ensure_solve_table(c) == SC_OK) {
+        // classes of two or more same-shape bunches solve with one launch per
+        // kernel; each class, and each remaining bunch, is one unit of the pool
+        std::vector<std::vector<int>> units;
+        std::vector<int> single;
+        std::vector<bool> done(nb, false);
+        for (int b = 0; b < nb; ++b) {
+            if (done[b]) continue;
+            const Mesh &mb = meshes[b]->m;
+            std::vector<int> cls{b};
+            done[b] = true;
+            // a class's spectra stay within kGroupSolveBytes, so that each
+            // grouped pass leaves them in the last-level cache for the next
+            const int64_t per_bunch = solve_group_bytes(&mb);
+            const int64_t cap = per_bunch > 0 ? kGroupSolveBytes / per_bunch : 0;
+            if (solve_groupable(&mb) && cap >= 2)
+                for (int o = b + 1; o < nb && (int64_t)cls.size() < cap; ++o) {
+                    const Mesh &mo = meshes[o]->m;
+                    if (!done[o] && solve_groupable(&mo) && mo.dtype == mb.dtype &&
+                        mo.n[0] == mb.n[0] && mo.n[1] == mb.n[1] && mo.n[2] == mb.n[2] &&
+                        (at_cathode[o] != 0) == (at_cathode[b] != 0)) {
+                        cls.push_back(o);
+                        done[o] = true;
+                    }
+                }
+            units.push_back(cls);
+        }
+        std::vector<int> slot(units.size(), 0);
+        int used = 0;
+        for (size_t u = 0; u < units.size(); ++u) {
+            if (units[u].size() < 2) continue;
+            slot[u] = used;
+            used += (int)units[u].size();
+        }
+        if (used <= kGroupMax)
+            return run_batched(c, (int)units.size(), [&](int u) {
+                const std::vector<int> &cls = units[u];
+                if (cls.size() == 1) {
+                    const int b = cls[0];
+                    return solve_mesh(&meshes[b]->m, min_bounds + 3 * b, max_bounds + 3 * b,
+                                      delta + 3 * b, gamma[b], at_cathode[b]);
+                }
+                const int k = (int)cls.size();
+                std::vector<Mesh *> ms(k);
+                std::vector<const double *> lo(k), hi(k), dl(k);
+                std::vector<double> gm(k);
+                for (int i = 0; i < k; ++i) {
+                    ms[i] = &meshes[cls[i]]->m;
+                    lo[i] = min_bounds + 3 * cls[i];
+                    hi[i] = max_bounds + 3 * cls[i];
+                    dl[i] = delta + 3 * cls[i];
+                    gm[i] = gamma[cls[i]];
+                }
+                return solve_class(c, ms.data(), k, lo.data(), hi.data(), dl.data(), gm.data(),
+                                   at_cathode[cls[0]] != 0, slot[u]);
+            });
     }
     return run_batched(c, nb, [&](int b) {
         return solve_mesh(&meshes[b]->m, min_bounds + 3 * b, max_bounds + 3 * b, delta + 3 * b,

@@ -1147,7 +1147,7 @@ int launch_deposit_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const voi
                            const void *const *y, const void *const *z, const void *const *q,
                            const int64_t *n, const double *lo, const double *dl, int clear)
 {
-    if (!ctx->grouped || nb < 1 || nb > kGroupMax) return SC_ERR_UNSUPPORTED;
+    if (!(ctx->grouped & SC_GROUP_PARTICLES) || nb < 1 || nb > kGroupMax) return SC_ERR_UNSUPPORTED;
     const int mdt = ms[0]->dtype;
     std::vector<Geo> geos(nb);
     for (int b = 0; b < nb; ++b) {

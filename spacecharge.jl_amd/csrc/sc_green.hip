@@ -31,6 +31,9 @@
 // cell integrals are even in every axis: its spectrum is real.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <type_traits>
+
 #include "sc_fft_reg.h"
 #include "sc_solve_common.h"
 
@@ -138,19 +141,37 @@ __device__ __forceinline__ double green_scale(const GreenArgs &a, int c)
     return kFPEI * ((c != 2 ? a.gamma : 1.0) / vol) / P;
 }
 
+// A grouped launch's bunch: its spacings, image offset and scales.
+__device__ __forceinline__ void green_from_table(GreenArgs &a, const SBunch *sb)
+{
+    a.dx = sb->dx;
+    a.dy = sb->dy;
+    a.dz = sb->dz;
+    a.offzg = sb->offzg;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) a.scale[c] = sb->scale[c];
+}
+
 // The function computed by component set q: field component q % 3 (image
 // set q / 3), or the potential (c = 3, odd in no axis).
 __device__ __forceinline__ int comp_of(const GreenArgs &a, int q) { return a.pot ? 3 : q % 3; }
 
 // grid (x-y plane blocks, z node planes, components); in-plane index math
 // in 32 bits (this kernel is bound by its Float64 instruction count)
-__global__ void __launch_bounds__(256) k_green_f(double *__restrict__ F, GreenArgs a)
+__global__ void __launch_bounds__(256)
+k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, int per)
 {
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) {
+        F = sb->F;
+        green_from_table(a, sb);
+    }
     const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int nx1 = a.d.nx + 1, ny1 = a.d.ny + 1;
     const int kn = blockIdx.y;
     if (kn >= z_nodes(a, image)) return;
-    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int pidx = bx * blockDim.x + threadIdx.x;
     if (pidx >= nx1 * ny1) return;
     const int in = pidx % nx1;
     const int jn = pidx / nx1;
@@ -381,19 +402,25 @@ struct RShape {
 template <typename T, int L>
 __global__ void __launch_bounds__(kRThreads)
 k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restrict__ twg,
-           GreenArgs a)
+           GreenArgs a, const SBunch *__restrict__ tab, int per)
 {
     using S = RShape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) {
+        F = sb->F;
+        R = static_cast<T *>(sb->R);
+    }
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
     const int q = blockIdx.y, c = comp_of(a, q), image = q / 3;
     const int nx = a.d.nx, ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int bpairs = (ny + 1) / 2;
     const int npencils = z_cells(a, image) * bpairs;
     const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
-    const int P = blockIdx.x * NB + b;
+    const int P = bx * NB + b;
     const bool live = P < npencils;
     const int k = live ? P / bpairs : 0, ya = live ? 2 * (P - k * bpairs) : 0;
     const bool hasb = ya + 1 < ny;
@@ -463,19 +490,23 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
 
 template <typename T, int L>
 __global__ void __launch_bounds__(kRThreads)
-k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
+k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a,
+           const SBunch *__restrict__ tab, int per)
 {
     using S = RShape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) R = static_cast<T *>(sb->R);
     const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int k = blockIdx.y;
     if (k >= z_cells(a, image)) return;  // uniform over the workgroup
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
     const int ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
+    const int kxa = 2 * (bx * NB + b), kxb = kxa + 1;
     T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kp;
     const T sgn = (c == 1) ? (T)-1 : (T)1;
     // every load is in range (clamped row and column) and issued without a
@@ -514,18 +545,26 @@ k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a)
 template <typename T, int L>
 __global__ void __launch_bounds__(kRThreads)
 k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
-           const cpx<T> *__restrict__ twg, GreenArgs a)
+           const cpx<T> *__restrict__ twg, GreenArgs a, const SBunch *__restrict__ tab, int per)
 {
     using S = RShape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) {
+        R = static_cast<const T *>(sb->R);
+        Gf = static_cast<T *>(sb->Gf);
+        Gi = static_cast<cpx<T> *>(sb->Gi);
+        green_from_table(a, sb);
+    }
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
     const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int ky = blockIdx.y;
     const int nz = a.d.nz, Kx = a.d.Kx, Kp = a.d.Kp, ny1 = a.d.ny + 1;
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    const int kxa = 2 * (blockIdx.x * NB + b), kxb = kxa + 1;
+    const int kxa = 2 * (bx * NB + b), kxb = kxa + 1;
     const T *Rq = R + r_off(a, q);
     const T sgn = (c == 2) ? (T)-1 : (T)1;
     // loads clamped in range and issued without branches (see k_green_2r)
@@ -600,12 +639,14 @@ void green_pass_r(int pass, dim3 grid, hipStream_t st, const Workspace &ws, cons
     T *R = static_cast<T *>(ws.R);
     if (pass == 1)
         hipLaunchKernelGGL((k_green_1r<T, L>), grid, dim3(kRThreads), 0, st, (const double *)ws.F, R,
-                           tw, a);
+                           tw, a, (const SBunch *)nullptr, 0);
     else if (pass == 2)
-        hipLaunchKernelGGL((k_green_2r<T, L>), grid, dim3(kRThreads), 0, st, R, tw, a);
+        hipLaunchKernelGGL((k_green_2r<T, L>), grid, dim3(kRThreads), 0, st, R, tw, a,
+                           (const SBunch *)nullptr, 0);
     else
         hipLaunchKernelGGL((k_green_3r<T, L>), grid, dim3(kRThreads), 0, st, (const T *)R,
-                           static_cast<T *>(ws.Gf), static_cast<cpx<T> *>(ws.Gi), tw, a);
+                           static_cast<T *>(ws.Gf), static_cast<cpx<T> *>(ws.Gi), tw, a,
+                           (const SBunch *)nullptr, 0);
 }
 
 // One register-engine Green pass if its length is supported; false otherwise.
@@ -676,7 +717,7 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
         const int znodes = cath ? d.Lz : d.nz + 1;
         launch_begin(ctx, K_GREEN_F);
         hipLaunchKernelGGL(k_green_f, dim3((unsigned)div_up(plane, 256), znodes, ncomp), dim3(256), 0,
-                           st, ws.F, a);
+                           st, ws.F, a, (const SBunch *)nullptr, 0);
         launch_end(ctx, K_GREEN_F);
     }
     const bool fast_ok = !m->generic_transforms;
@@ -712,6 +753,72 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
 }
+
+// The Green build of nb same-shape bunches, each kernel once for all of
+// them (x stacked); field modes only, register engine lengths only.
+template <typename T>
+void build_green_grouped(const Mesh *m0, int nb, const SBunch *tab, double gamma, int mode,
+                         hipStream_t st)
+{
+    const int cath = mode == kFieldCathode;
+    GreenArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.d = make_dims(m0);
+    const Dims &d = a.d;
+    a.pot = 0;
+    a.geom = nullptr;
+    a.gamma = gamma;
+    a.f_free = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+    a.f_img = (int64_t)(d.nx + 1) * (d.ny + 1) * d.Lz;
+    a.r_free = (int64_t)d.nz * (d.ny + 1) * d.Kp;
+    a.r_img = (int64_t)(d.Lz - 1) * (d.ny + 1) * d.Kp;
+    const int ncomp = cath ? 6 : 3;
+    const Workspace &ws = m0->ws;
+    const cpx<T> *tw[3] = {static_cast<const cpx<T> *>(ws.twT[0]),
+                           static_cast<const cpx<T> *>(ws.twT[1]),
+                           static_cast<const cpx<T> *>(ws.twT[2])};
+    {
+        const int per = div_up((d.nx + 1) * (d.ny + 1), 256);
+        const int znodes = cath ? d.Lz : d.nz + 1;
+        hipLaunchKernelGGL(k_green_f, dim3((unsigned)(per * nb), znodes, ncomp), dim3(256), 0, st,
+                           (double *)nullptr, a, tab, per);
+    }
+    const int zc = cath ? d.Lz - 1 : d.nz;
+    const int kp = (d.Kx + 1) / 2;
+    for (int pass = 1; pass <= 3; ++pass) {
+        const int L = pass == 1 ? d.Lx : (pass == 2 ? d.Ly : d.Lz);
+        auto go = [&](auto lc) {
+            constexpr int LL = decltype(lc)::value;
+            constexpr int NB = RShape<T, LL>::NB;
+            if (pass == 1) {
+                const int per = div_up(zc * ((d.ny + 1) / 2), NB);
+                hipLaunchKernelGGL((k_green_1r<T, LL>), dim3(per * nb, ncomp), dim3(kRThreads), 0, st,
+                                   (const double *)nullptr, (T *)nullptr, tw[0], a, tab, per);
+            } else if (pass == 2) {
+                const int per = div_up(kp, NB);
+                hipLaunchKernelGGL((k_green_2r<T, LL>), dim3(per * nb, zc, ncomp), dim3(kRThreads), 0,
+                                   st, (T *)nullptr, tw[1], a, tab, per);
+            } else {
+                const int per = div_up(kp, NB);
+                hipLaunchKernelGGL((k_green_3r<T, LL>), dim3(per * nb, d.ny + 1, ncomp),
+                                   dim3(kRThreads), 0, st, (const T *)nullptr, (T *)nullptr,
+                                   (cpx<T> *)nullptr, tw[2], a, tab, per);
+            }
+        };
+        switch (L) {
+        case 32: go(std::integral_constant<int, 32>{}); break;
+        case 64: go(std::integral_constant<int, 64>{}); break;
+        case 128: go(std::integral_constant<int, 128>{}); break;
+        case 256: go(std::integral_constant<int, 256>{}); break;
+        case 512: go(std::integral_constant<int, 512>{}); break;
+        default: break;  // the caller groups only supported lengths
+        }
+    }
+}
+
+template void build_green_grouped<float>(const Mesh *, int, const SBunch *, double, int, hipStream_t);
+template void build_green_grouped<double>(const Mesh *, int, const SBunch *, double, int,
+                                          hipStream_t);
 
 template int build_green<float>(Mesh *, const double[3], double, double, int);
 template int build_green<double>(Mesh *, const double[3], double, double, int);

@@ -109,7 +109,9 @@ struct Ctx {
     // memory, allocated on first use; `grouped` off falls back to one
     // launch sequence per bunch on the stream pool
     void *gtable = nullptr;
-    bool grouped = true;
+    int grouped = 3;  // SC_GROUP_PARTICLES | SC_GROUP_SOLVE
+    // grouped batched solves: per-bunch buffer table (device memory, first use)
+    void *stable = nullptr;
 };
 
 // Device workspace of one mesh (lazy, like src/mesh.jl:49-72).
@@ -228,6 +230,20 @@ constexpr int kGroupStore = 8;  // descriptors per store launch (kernel argument
 // group_table_bytes(); SC_ERR_UNSUPPORTED when first needed under capture).
 int ensure_group_table(Ctx *ctx, size_t bytes);
 size_t group_table_bytes();
+// Grouped batched solves (sc_solve.hip): whether a mesh can join a class
+// (register-engine lengths, no Green reuse, no forced generic engine); the
+// context's table (SC_ERR_UNSUPPORTED when first needed under capture);
+// the field solve of k meshes of one shape and type on ctx->stream, table
+// entries slot .. slot + k - 1, per-bunch bounds (3 values each) and gamma.
+bool solve_groupable(const Mesh *m);
+// Bytes of the spectra a solve of m streams between its passes; classes
+// hold at most kGroupSolveBytes of them (larger meshes solve one by one,
+// where each pass finds the previous one's output in the 256 MB cache).
+int64_t solve_group_bytes(const Mesh *m);
+constexpr int64_t kGroupSolveBytes = (int64_t)64 << 20;
+int ensure_solve_table(Ctx *ctx);
+int solve_class(Ctx *ctx, Mesh *const *ms, int k, const double *const *lo, const double *const *hi,
+                const double *const *dl, const double *gamma, int cath, int slot);
 // interpolate_field of nb bunches, grouped likewise (interleave, gather).
 int launch_interp_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void *const *x,
                           const void *const *y, const void *const *z, const int64_t *n,

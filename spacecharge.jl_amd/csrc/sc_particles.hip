@@ -817,7 +817,7 @@ int launch_interp_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void
                           const double *lo, const double *dl, void *const *Ex, void *const *Ey,
                           void *const *Ez)
 {
-    if (!ctx->grouped || nb < 1 || nb > kGroupMax || sizeof(ITable) > group_table_bytes())
+    if (!(ctx->grouped & SC_GROUP_PARTICLES) || nb < 1 || nb > kGroupMax || sizeof(ITable) > group_table_bytes())
         return SC_ERR_UNSUPPORTED;
     const int mdt = ms[0]->dtype;
     for (int b = 0; b < nb; ++b)

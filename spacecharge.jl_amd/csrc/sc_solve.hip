@@ -382,9 +382,15 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         (void)hipStreamIsCapturing(st, &cs);
         overlap = cs == hipStreamCaptureStatusNone && !ctx->in_batch && ensure_side_stream(m) == SC_OK;
         if (overlap) {
+            // the build starts where this step's deposit started; the first
+            // time (no deposit marked yet) it follows everything on `st`,
+            // which may still read this mesh's Green spectra
+            if (!ws.deposit_start_recorded) {
+                SC_HIP(ctx, hipEventRecord(ws.deposit_start, st));
+                ws.deposit_start_recorded = true;
+            }
             if (ws.gf_free_recorded) SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.gf_free, 0));
-            if (ws.deposit_start_recorded)
-                SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.deposit_start, 0));
+            SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.deposit_start, 0));
             ctx->stream = ctx->side;  // the build launches (and times) on ctx->stream
             const int rc = build_green<T>(m, dl, gamma, offz, gmode);
             ctx->stream = st;
@@ -606,6 +612,136 @@ int solve_adjoint_mesh(Mesh *m, const double lo[3], const double hi[3], const do
 {
     SolveIO io{at_cathode ? kAdjointCathode : kAdjointFree, 3, 1, g_efield, g_rho, nullptr, nullptr};
     return run_solve_io(m, lo, hi, dl, gamma, io);
+}
+
+
+// ---- grouped solves: same-shape bunches of a batched solve ----
+// The bunches of one class (mesh shape, type, cathode flag; register-engine
+// lengths) run each Green and transform kernel once for all of them, with
+// their buffers and Green geometry from a table in the context's device
+// memory, written by store launches that carry the entries as arguments
+// (so a captured graph rewrites them on every replay).  Each class owns a
+// disjoint range of the table, so classes may run on different streams.
+
+namespace {
+
+struct SStore {
+    int first, count;
+    SBunch b[kGroupStore];
+};
+static_assert(sizeof(SStore) <= 2048, "store-launch arguments stay small");
+
+// one lane per 8-byte word (vector stores)
+__global__ void __launch_bounds__(256) k_sgroup_store(SStore s, SBunch *__restrict__ tab)
+{
+    constexpr int kWords = (int)(sizeof(SBunch) / 8);
+    static_assert(sizeof(SBunch) % 8 == 0, "entry of whole words");
+    const unsigned long long *src = reinterpret_cast<const unsigned long long *>(s.b);
+    unsigned long long *dst = reinterpret_cast<unsigned long long *>(tab + s.first);
+    for (int i = threadIdx.x; i < s.count * kWords; i += blockDim.x) dst[i] = src[i];
+}
+
+bool reg_length(int L) { return L == 32 || L == 64 || L == 128 || L == 256 || L == 512; }
+
+template <typename T>
+int solve_class_t(Ctx *ctx, Mesh *const *ms, int k, const double *const *lo, const double *const *hi,
+                  const double *const *dl, const double *gamma, int cath, SBunch *tab)
+{
+    hipStream_t st = ctx->stream;
+    const int mode = cath ? kFieldCathode : kFieldFree;
+    for (int i = 0; i < k; ++i)
+        if (int rc = ensure_workspace(ms[i])) return rc;
+    const Dims d = make_dims(ms[0]);
+    const double P = (double)d.Lx * (double)d.Ly * (double)d.Lz;
+    SStore s;
+    std::memset(&s, 0, sizeof(s));
+    for (int b0 = 0; b0 < k; b0 += kGroupStore) {
+        s.first = b0;
+        s.count = k - b0 < kGroupStore ? k - b0 : kGroupStore;
+        for (int j = 0; j < s.count; ++j) {
+            const int i = b0 + j;
+            Mesh *m = ms[i];
+            Workspace &ws = m->ws;
+            SBunch &e = s.b[j];
+            e.in = m->rho;
+            e.out = m->efield;
+            e.fwd = ws.A;
+            e.inv = ws.B;
+            e.Gf = ws.Gf;
+            e.Gi = ws.Gi;
+            e.R = ws.R;
+            e.F = ws.F;
+            // the expressions of build_green and run_solve
+            e.dx = dl[i][0];
+            e.dy = dl[i][1];
+            e.dz = dl[i][2] * gamma[i];
+            e.offzg = (lo[i][2] + hi[i][2]) * gamma[i];
+            const double vol = (e.dx * e.dy) * e.dz;
+            for (int c = 0; c < 3; ++c) e.scale[c] = kFPEI * ((c < 2 ? gamma[i] : 1.0) / vol) / P;
+            e.scale[3] = kFPEI * (gamma[i] / vol) / P;
+        }
+        hipLaunchKernelGGL(k_sgroup_store, dim3(1), dim3(256), 0, st, s, tab);
+    }
+    launch_begin(ctx, K_GREEN_F);
+    build_green_grouped<T>(ms[0], k, tab, 1.0, mode, st);
+    launch_end(ctx, K_GREEN_F);
+    static const int kid[5] = {K_RHO_FWD_X, K_RHO_FWD_Y, K_ZCONV, K_INV_Y, K_INV_X};
+    for (int which = 0; which < 5; ++which) {
+        launch_begin(ctx, kid[which]);
+        launch_fast_grouped<T>(ms[0], k, tab, which, mode, 1, 3, st);
+        launch_end(ctx, kid[which]);
+        if (which == 2) {
+            // a later single solve's side-stream Green build waits for this
+            for (int i = 0; i < k; ++i) {
+                Workspace &ws = ms[i]->ws;
+                if (ws.gf_free) {
+                    SC_HIP(ctx, hipEventRecord(ws.gf_free, st));
+                    ws.gf_free_recorded = true;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < k; ++i) ms[i]->ws.g_valid = false;  // the reuse cache was not kept
+    SC_HIP(ctx, hipGetLastError());
+    return SC_OK;
+}
+
+}  // namespace
+
+bool solve_groupable(const Mesh *m)
+{
+    if (m->generic_transforms || m->green_reuse) return false;
+    const Dims d = make_dims(m);
+    return reg_length(d.Lx) && reg_length(d.Ly) && reg_length(d.Lz);
+}
+
+int64_t solve_group_bytes(const Mesh *m)
+{
+    const Dims d = make_dims(m);
+    const int64_t s = m->dtype == SC_F32 ? 4 : 8;
+    // four spectra (rho and three fields) of [nz][2ny][Kp] complex
+    return 4 * 2 * s * (int64_t)d.nz * d.Ly * d.Kp;
+}
+
+int ensure_solve_table(Ctx *ctx)
+{
+    if (ctx->stable) return SC_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SC_HIP(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return SC_ERR_UNSUPPORTED;
+    if (hipMalloc(&ctx->stable, sizeof(SBunch) * kGroupMax) != hipSuccess) {
+        ctx->stable = nullptr;
+        return set_err(ctx, SC_ERR_NOMEM, "device allocation of the batch descriptor table failed");
+    }
+    return SC_OK;
+}
+
+int solve_class(Ctx *ctx, Mesh *const *ms, int k, const double *const *lo, const double *const *hi,
+                const double *const *dl, const double *gamma, int cath, int slot)
+{
+    SBunch *tab = static_cast<SBunch *>(ctx->stable) + slot;
+    if (ms[0]->dtype == SC_F32) return solve_class_t<float>(ctx, ms, k, lo, hi, dl, gamma, cath, tab);
+    return solve_class_t<double>(ctx, ms, k, lo, hi, dl, gamma, cath, tab);
 }
 
 }  // namespace sc

@@ -77,6 +77,30 @@ struct SolveIO {
     void *inv;
 };
 
+// ---- grouped solves: bunches of one mesh shape, one launch per kernel ----
+// Per-bunch buffers and Green geometry of a grouped solve (a table in device
+// memory).  A grouped launch stacks the bunches along x: workgroup x belongs
+// to bunch x / per and acts as workgroup x % per of that bunch's own launch.
+// Kernels given no table (the single-bunch launch) use their arguments.
+struct SBunch {
+    const void *in;
+    void *out, *fwd, *inv, *Gf, *Gi, *R;
+    double *F;
+    double dx, dy, dz, offzg;  // Green spacings (dz times gamma) and image offset times gamma
+    double scale[4];
+};
+
+__device__ __forceinline__ int group_block(const SBunch *tab, int per, const SBunch *&sb)
+{
+    if (!tab) {
+        sb = nullptr;
+        return (int)blockIdx.x;
+    }
+    const int k = (int)blockIdx.x / per;
+    sb = tab + k;
+    return (int)blockIdx.x - k * per;
+}
+
 // Builds the transformed Green functions for green_mode `mode` into ws.Gf
 // (and ws.Gi for kFieldCathode): three field components, or the
 // potential's one (stored in the first component's slot).
@@ -89,5 +113,16 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
 // launched then).
 template <typename T>
 bool launch_fast(Mesh *m, int which, const SolveIO &io);
+
+// Grouped forms, on `st`, for nb bunches of mesh shape d whose transform
+// lengths are supported powers of two: the Green build (field modes) of
+// every bunch, from m0's twiddles and the table's spacings, and solve
+// kernel `which` of every bunch with the table's buffers.
+template <typename T>
+void build_green_grouped(const Mesh *m0, int nb, const SBunch *tab, double gamma, int mode,
+                         hipStream_t st);
+template <typename T>
+void launch_fast_grouped(const Mesh *m0, int nb, const SBunch *tab, int which, int mode, int nin,
+                         int nout, hipStream_t st);
 
 }  // namespace sc

@@ -41,15 +41,21 @@ constexpr int zconv_threads()
 template <typename T, int L>
 __global__ void __launch_bounds__(kThreads)
 k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> *__restrict__ twg,
-              Dims d)
+              Dims d, const SBunch *__restrict__ tab, int per)
 {
     using S = Shape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) {
+        in = static_cast<const T *>(sb->in);
+        spec = static_cast<cpx<T> *>(sb->fwd);
+    }
     for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
     const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
-    const int r = blockIdx.x * NB + b;
+    const int r = bx * NB + b;
     const int nrows = d.ny * d.nz, nx = d.nx;
     const T *rho = in + (size_t)blockIdx.y * nx * nrows;
     cpx<T> *A = spec + (size_t)blockIdx.y * d.nz * d.Ly * d.Kp;
@@ -76,15 +82,19 @@ k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> 
 // zero-padded to 2ny.
 template <typename T, int L>
 __global__ void __launch_bounds__(kThreads)
-k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
+k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d,
+              const SBunch *__restrict__ tab, int per)
 {
     using S = Shape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) A = static_cast<cpx<T> *>(sb->fwd);
     for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    const int kx = blockIdx.x * NB + b, z = blockIdx.y;
+    const int kx = bx * NB + b, z = blockIdx.y;
     const int Kx = d.Kx, Kp = d.Kp;
     const bool live = kx < Kx;
     cpx<T> *Ap = A + ((size_t)blockIdx.z * d.nz + z) * d.Ly * Kp + kx;
@@ -108,8 +118,17 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d)
 template <typename T, int L, int MODE, int THREADS>
 __global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3)))
 k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
-          const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d)
+          const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d,
+          const SBunch *__restrict__ tab, int per)
 {
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) {
+        A = static_cast<const cpx<T> *>(sb->fwd);
+        B = static_cast<cpx<T> *>(sb->inv);
+        Gf = static_cast<const T *>(sb->Gf);
+        Gi = static_cast<const cpx<T> *>(sb->Gi);
+    }
     constexpr bool CATH = MODE == kFieldCathode, POT = MODE == kPotential;
     constexpr int NCOMP = POT ? 1 : 3;
     using S = Shape<T, L, THREADS>;
@@ -119,7 +138,7 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
     const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
     for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    const int kx = blockIdx.x * NB + b, ky = blockIdx.y;
+    const int kx = bx * NB + b, ky = blockIdx.y;
     const bool live = kx < d.Kx;
     const size_t zs = (size_t)Ly * Kp;
     const cpx<T> *Ap = A + (size_t)ky * Kp + kx;
@@ -255,15 +274,19 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
 // Backward y pass: pencils (kx, z < nz) of each field spectrum, y < ny kept.
 template <typename T, int L>
 __global__ void __launch_bounds__(kThreads)
-k_inv_y_r(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, Dims d)
+k_inv_y_r(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, Dims d,
+          const SBunch *__restrict__ tab, int per)
 {
     using S = Shape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) B = static_cast<cpx<T> *>(sb->inv);
     for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    const int kx = blockIdx.x * NB + b, z = blockIdx.y, c = blockIdx.z;
+    const int kx = bx * NB + b, z = blockIdx.y, c = blockIdx.z;
     const int Kx = d.Kx, Kp = d.Kp;
     const bool live = kx < Kx;
     cpx<T> *Bp = B + ((size_t)c * d.nz + z) * d.Ly * Kp + (live ? kx : 0);
@@ -280,15 +303,22 @@ k_inv_y_r(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, Dims d)
 // Backward x pass (complex to real) with Hermitian completion.
 template <typename T, int L>
 __global__ void __launch_bounds__(kThreads)
-k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__restrict__ twg, Dims d)
+k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__restrict__ twg, Dims d,
+          const SBunch *__restrict__ tab, int per)
 {
     using S = Shape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) {
+        B = static_cast<const cpx<T> *>(sb->inv);
+        E_ = static_cast<T *>(sb->out);
+    }
     for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
     const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
-    const int r = blockIdx.x * NB + b, c = blockIdx.y;
+    const int r = bx * NB + b, c = blockIdx.y;
     const int nrows = d.ny * d.nz, nx = d.nx;
     const bool live = r < nrows;
     const int rr = live ? r : 0;
@@ -332,13 +362,14 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
     case 0:
         launch_begin(ctx, K_RHO_FWD_X);
         hipLaunchKernelGGL((k_rho_fwd_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), io.nin),
-                           dim3(kThreads), 0, st, static_cast<const T *>(io.in), F, tw, d);
+                           dim3(kThreads), 0, st, static_cast<const T *>(io.in), F, tw, d,
+                           (const SBunch *)nullptr, 0);
         launch_end(ctx, K_RHO_FWD_X);
         break;
     case 1:
         launch_begin(ctx, K_RHO_FWD_Y);
         hipLaunchKernelGGL((k_rho_fwd_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, io.nin),
-                           dim3(kThreads), 0, st, F, tw, d);
+                           dim3(kThreads), 0, st, F, tw, d, (const SBunch *)nullptr, 0);
         launch_end(ctx, K_RHO_FWD_Y);
         break;
     case 2: {
@@ -352,11 +383,11 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
         switch (io.mode) {
         case kFieldCathode:
             hipLaunchKernelGGL((k_zconv_r<T, L, kFieldCathode, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf,
-                               Gi, tw, d);
+                               Gi, tw, d, (const SBunch *)nullptr, 0);
             break;
         case kPotential:
             hipLaunchKernelGGL((k_zconv_r<T, L, kPotential, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
-                               tw, d);
+                               tw, d, (const SBunch *)nullptr, 0);
             break;
         case kAdjointFree:
             hipLaunchKernelGGL((k_zconv_adj_r<T, L, false, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
@@ -368,7 +399,7 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
             break;
         default:
             hipLaunchKernelGGL((k_zconv_r<T, L, kFieldFree, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
-                               tw, d);
+                               tw, d, (const SBunch *)nullptr, 0);
             break;
         }
         launch_end(ctx, K_ZCONV);
@@ -377,19 +408,92 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
     case 3:
         launch_begin(ctx, K_INV_Y);
         hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(div_up(d.Kx, S::NB), d.nz, io.nout),
-                           dim3(kThreads), 0, st, B, tw, d);
+                           dim3(kThreads), 0, st, B, tw, d, (const SBunch *)nullptr, 0);
         launch_end(ctx, K_INV_Y);
         break;
     case 4:
         launch_begin(ctx, K_INV_X);
         hipLaunchKernelGGL((k_inv_x_r<T, L>), dim3(div_up(d.ny * d.nz, S::NB), io.nout),
-                           dim3(kThreads), 0, st, (const cpx<T> *)B, static_cast<T *>(io.out), tw, d);
+                           dim3(kThreads), 0, st, (const cpx<T> *)B, static_cast<T *>(io.out), tw, d,
+                           (const SBunch *)nullptr, 0);
         launch_end(ctx, K_INV_X);
         break;
     }
 }
 
+// Solve kernel `which` of nb same-shape bunches in one launch (x stacked).
+template <typename T, int L>
+void launch_all_grouped(const Dims &d, const cpx<T> *tw, int nb, const SBunch *tab, int which,
+                        int mode, int nin, int nout, hipStream_t st)
+{
+    using S = Shape<T, L>;
+    switch (which) {
+    case 0: {
+        const int per = div_up(d.ny * d.nz, S::NB);
+        hipLaunchKernelGGL((k_rho_fwd_x_r<T, L>), dim3(per * nb, nin), dim3(kThreads), 0, st,
+                           (const T *)nullptr, (cpx<T> *)nullptr, tw, d, tab, per);
+        break;
+    }
+    case 1: {
+        const int per = div_up(d.Kx, S::NB);
+        hipLaunchKernelGGL((k_rho_fwd_y_r<T, L>), dim3(per * nb, d.nz, nin), dim3(kThreads), 0, st,
+                           (cpx<T> *)nullptr, tw, d, tab, per);
+        break;
+    }
+    case 2: {
+        constexpr int ZT = zconv_threads<T, L>();
+        using SZ = Shape<T, L, ZT>;
+        const int per = div_up(d.Kx, SZ::NB);
+        const dim3 zgrid(per * nb, d.Ly);
+        if (mode == kFieldCathode)
+            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldCathode, ZT>), zgrid, dim3(ZT), 0, st,
+                               (const cpx<T> *)nullptr, (cpx<T> *)nullptr, (const T *)nullptr,
+                               (const cpx<T> *)nullptr, tw, d, tab, per);
+        else
+            hipLaunchKernelGGL((k_zconv_r<T, L, kFieldFree, ZT>), zgrid, dim3(ZT), 0, st,
+                               (const cpx<T> *)nullptr, (cpx<T> *)nullptr, (const T *)nullptr,
+                               (const cpx<T> *)nullptr, tw, d, tab, per);
+        break;
+    }
+    case 3: {
+        const int per = div_up(d.Kx, S::NB);
+        hipLaunchKernelGGL((k_inv_y_r<T, L>), dim3(per * nb, d.nz, nout), dim3(kThreads), 0, st,
+                           (cpx<T> *)nullptr, tw, d, tab, per);
+        break;
+    }
+    case 4: {
+        const int per = div_up(d.ny * d.nz, S::NB);
+        hipLaunchKernelGGL((k_inv_x_r<T, L>), dim3(per * nb, nout), dim3(kThreads), 0, st,
+                           (const cpx<T> *)nullptr, (T *)nullptr, tw, d, tab, per);
+        break;
+    }
+    }
+}
+
 }  // namespace
+
+template <typename T>
+void launch_fast_grouped(const Mesh *m0, int nb, const SBunch *tab, int which, int mode, int nin,
+                         int nout, hipStream_t st)
+{
+    const Dims d = make_dims(m0);
+    const int axis = (which == 0 || which == 4) ? 0 : (which == 2 ? 2 : 1);
+    const int L = axis == 0 ? d.Lx : (axis == 2 ? d.Lz : d.Ly);
+    const cpx<T> *tw = static_cast<const cpx<T> *>(m0->ws.twT[axis]);
+    switch (L) {
+    case 32: launch_all_grouped<T, 32>(d, tw, nb, tab, which, mode, nin, nout, st); break;
+    case 64: launch_all_grouped<T, 64>(d, tw, nb, tab, which, mode, nin, nout, st); break;
+    case 128: launch_all_grouped<T, 128>(d, tw, nb, tab, which, mode, nin, nout, st); break;
+    case 256: launch_all_grouped<T, 256>(d, tw, nb, tab, which, mode, nin, nout, st); break;
+    case 512: launch_all_grouped<T, 512>(d, tw, nb, tab, which, mode, nin, nout, st); break;
+    default: break;  // the caller groups only supported lengths
+    }
+}
+
+template void launch_fast_grouped<float>(const Mesh *, int, const SBunch *, int, int, int, int,
+                                         hipStream_t);
+template void launch_fast_grouped<double>(const Mesh *, int, const SBunch *, int, int, int, int,
+                                          hipStream_t);
 
 // Runs one solve kernel (0 rho_fwd_x, 1 rho_fwd_y, 2 zconv, 3 inv_y,
 // 4 inv_x) with the register engine if its transform length is a supported

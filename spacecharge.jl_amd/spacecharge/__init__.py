@@ -600,13 +600,17 @@ def set_stream_pool_size(nstreams: int, device=None) -> None:
     _check(_lib.load().sc_ctx_set_pool_size(ctx, int(nstreams)), ctx)
 
 
-def set_grouped_launch(enable: bool = True, device=None) -> None:
+def set_grouped_launch(particles: bool = True, solve=None, device=None) -> None:
     """Batched deposit and interpolation as one launch per phase for all
-    bunches (default), or as one launch sequence per bunch on the stream
-    pool (enable=False).  No reference counterpart."""
+    bunches (particles=True, the default), and batched solves of same-shape
+    small meshes as one launch per kernel (solve, default: as particles);
+    False runs one launch sequence per bunch on the stream pool.  No
+    reference counterpart."""
     dev = torch.device(device) if device is not None else _default_device()
     ctx = _ctx(dev)
-    _check(_lib.load().sc_ctx_set_grouped_launch(ctx, int(bool(enable))), ctx)
+    solve = particles if solve is None else solve
+    phases = (1 if particles else 0) | (2 if solve else 0)
+    _check(_lib.load().sc_ctx_set_grouped_launch(ctx, phases), ctx)
 
 
 def synchronize(device=None) -> None:

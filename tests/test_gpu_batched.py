@@ -162,21 +162,26 @@ def test_batched_step_captured_in_a_graph():
                 assert torch.equal(Eg[b][c], Ee[b][c]), (rep, b, c)
 
 
-def _batch_run(sc, meshes, parts, clear=True):
+def _batch_run(sc, meshes, parts, clear=True, at_cathode=False):
     sc.deposit_batched(meshes, *[[p[k] for p in parts] for k in range(4)], clear=clear)
     rho = [m.rho.clone() for m in meshes]
-    sc.solve_batched(meshes)
+    sc.solve_batched(meshes, at_cathode=at_cathode)
     E = sc.interpolate_batched(meshes, *[[p[k] for p in parts] for k in range(3)])
     torch.cuda.synchronize()
     return rho, E
 
 
+@pytest.mark.parametrize("cath", ["free", "cathode", "mixed"])
 @pytest.mark.parametrize("T", [torch.float32, torch.float64])
-def test_grouped_launch_equals_stream_pool(T):
+def test_grouped_launch_equals_stream_pool(T, cath):
     """20 uneven bunches (three descriptor-store launches) with per-bunch
-    deposit tilings: the grouped deposit and interpolation (one launch per
-    phase) against the per-bunch launch sequences on the stream pool, with
-    clear=True and clear=False; bit for bit in Float32, 1e-12 in Float64."""
+    deposit tilings, meshes of four shapes (three of them form classes of
+    five for the grouped solve; 24x20x40 has lengths the register engine
+    does not take): the grouped deposit, solve and interpolation (one launch
+    per phase or kernel) against the per-bunch launch sequences on the
+    stream pool, with clear=True and clear=False, in free space, with the
+    cathode image, and with both (classes split by the flag); bit for bit in
+    Float32, 1e-12 in Float64."""
     sc = _sc()
     rng = np.random.default_rng(5)
     specs = [(int(10 ** rng.uniform(3, 5.5)), [(16, 16, 16), (32, 32, 32), (24, 20, 40),
@@ -190,13 +195,14 @@ def test_grouped_launch_equals_stream_pool(T):
             m.set_deposit_tiling(*t)
         return ms
 
+    flags = {"free": False, "cathode": True, "mixed": [b % 2 == 1 for b in range(20)]}[cath]
     try:
         out = {}
         for grouped in (True, False):
             sc.set_grouped_launch(grouped)
             ms = meshes()
-            rho1, E1 = _batch_run(sc, ms, parts)
-            rho2, _ = _batch_run(sc, ms, parts, clear=False)  # adds to the solved rho
+            rho1, E1 = _batch_run(sc, ms, parts, at_cathode=flags)
+            rho2, _ = _batch_run(sc, ms, parts, clear=False, at_cathode=flags)  # adds to rho
             out[grouped] = (rho1, E1, rho2)
     finally:
         sc.set_grouped_launch(True)
@@ -234,3 +240,46 @@ def test_batches_that_do_not_group_fall_back():
                 assert _rel(r, m.rho) <= 1e-12
                 for c in range(3):
                     assert _rel(e[c], Es[c]) <= 1e-12
+
+
+def test_grouped_step_captured_in_a_graph():
+    """Twelve bunches of two shapes (classes of six for the grouped solve):
+    the batched step captured once and replayed with new positions, an eager
+    grouped step on other meshes of the same context between replays (it
+    rewrites the descriptor tables): every replay equals the eager step bit
+    for bit (Float32)."""
+    sc = _sc()
+    T = torch.float32
+    specs = [(20_000 + 5_000 * b, (32, 32, 32) if b % 2 else (16, 16, 32)) for b in range(12)]
+    parts = [_bunch(500 + b, n, T) for b, (n, _) in enumerate(specs)]
+    live = [[t.clone() for t in p] for p in parts]
+    meshes = [sc.Mesh3D(g, p[0], p[1], p[2], T=T) for (_, g), p in zip(specs, live)]
+    eager = [sc.Mesh3D(g, p[0], p[1], p[2], T=T) for (_, g), p in zip(specs, live)]
+
+    def step(ms):
+        sc.deposit_batched(ms, *[[p[k] for p in live] for k in range(4)])
+        sc.solve_batched(ms, at_cathode=[b % 3 == 0 for b in range(12)])
+        return sc.interpolate_batched(ms, *[[p[k] for p in live] for k in range(3)])
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step(meshes)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        Eg = step(meshes)
+    for rep in range(3):
+        for p, p0 in zip(live, parts):
+            for k in range(3):
+                p[k].copy_(p0[k] * (1.0 - 0.03 * rep))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        Ee = step(eager)
+        torch.cuda.synchronize()
+        for b in range(12):
+            assert torch.equal(meshes[b].efield, eager[b].efield), (rep, b)
+            for c in range(3):
+                assert torch.equal(Eg[b][c], Ee[b][c]), (rep, b, c)
