@@ -832,7 +832,12 @@ int sc_solve_batched(sc_mesh *const *meshes, int nb, const double *min_bounds,
             return set_err(c, SC_ERR_INVALID, "max_bounds must be finite");
     }
     if (int e = check_dev(c)) return e;
-    if ((c->grouped & SC_GROUP_SOLVE) && nb > 1 && ensure_solve_table(c) == SC_OK) {
+    int table = SC_ERR_UNSUPPORTED;
+    if ((c->grouped & SC_GROUP_SOLVE) && nb > 1) {
+        table = ensure_solve_table(c);  // SC_ERR_UNSUPPORTED: first needed under capture
+        if (table != SC_OK && table != SC_ERR_UNSUPPORTED) return table;
+    }
+    if (table == SC_OK) {
         // classes of two or more same-shape bunches solve with one launch per
         // kernel; each class, and each remaining bunch, is one unit of the pool
         std::vector<std::vector<int>> units;

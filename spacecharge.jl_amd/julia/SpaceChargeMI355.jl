@@ -16,7 +16,8 @@ module SpaceChargeMI355
 using SpaceCharge
 import SpaceCharge: deposit!, clear_mesh!, solve!, interpolate_field, Mesh3D
 
-export MIArray, mi355_mesh, solve_potential
+export MIArray, mi355_mesh, solve_potential, autotune_deposit!, deposit_batched!, solve_batched!,
+       interpolate_batched, set_grouped_launch
 
 const libsc = joinpath(@__DIR__, "..", "libspacecharge_mi355.so")
 const SC_F32 = Cint(4)
@@ -189,5 +190,64 @@ function solve_potential(mesh::MIMesh{T}) where {T}
                 handle(mesh), d3(mesh.delta), Float64(mesh.gamma), phi.ptr), ctx())
     return phi
 end
+
+# ---------------------------------------------------------------- no reference counterpart
+# Picks the binned deposit's tiling for this mesh by timing the deposit of
+# these particles at every setting; leaves mesh.rho holding that deposit.
+# Returns the 5x4 mean times in ms (chunk_log2 10:14 by slice_log2 10:13).
+function autotune_deposit!(mesh::MIMesh, x::MIArray{P}, y::MIArray{P}, z::MIArray{P},
+                           q::MIArray{P}; reps::Integer=5) where {P}
+    times = zeros(Float32, 4, 5)   # column-major: [slice, chunk]
+    check(ccall((:sc_mesh_autotune_deposit, libsc), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int64,
+                 Ptr{Float64}, Ptr{Float64}, Cint, Ptr{Float32}),
+                handle(mesh), dtcode(P), x.ptr, y.ptr, z.ptr, q.ptr, length(x),
+                d3(mesh.min_bounds), d3(mesh.delta), Cint(reps), times), ctx())
+    return permutedims(times)
+end
+
+# Many bunches per call: the grouped launches of the C ABI (one launch per
+# phase for all bunches; same-shape small meshes solve as one class).
+ptrs(as) = Ptr{Cvoid}[a.ptr for a in as]
+flat3(f, meshes) = Float64[Float64(v) for m in meshes for v in f(m)]
+
+function deposit_batched!(meshes::Vector{<:MIMesh}, xs::Vector{<:MIArray{P}},
+                          ys::Vector{<:MIArray{P}}, zs::Vector{<:MIArray{P}},
+                          qs::Vector{<:MIArray{P}}; clear::Bool=true) where {P}
+    check(ccall((:sc_deposit_batched, libsc), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                 Ptr{Ptr{Cvoid}}, Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Cint),
+                [handle(m) for m in meshes], length(meshes), dtcode(P), ptrs(xs), ptrs(ys), ptrs(zs),
+                ptrs(qs), Int64[length(x) for x in xs], flat3(m -> m.min_bounds, meshes),
+                flat3(m -> m.delta, meshes), clear), ctx())
+end
+
+function solve_batched!(meshes::Vector{<:MIMesh}; at_cathode=false)
+    flags = at_cathode isa Bool ? fill(Cint(at_cathode), length(meshes)) : Cint.(at_cathode)
+    check(ccall((:sc_solve_batched, libsc), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64},
+                 Ptr{Cint}),
+                [handle(m) for m in meshes], length(meshes), flat3(m -> m.min_bounds, meshes),
+                flat3(m -> m.max_bounds, meshes), flat3(m -> m.delta, meshes),
+                Float64[Float64(m.gamma) for m in meshes], flags), ctx())
+end
+
+function interpolate_batched(meshes::Vector{<:MIMesh}, xs::Vector{<:MIArray{P}},
+                             ys::Vector{<:MIArray{P}}, zs::Vector{<:MIArray{P}}) where {P}
+    Exs, Eys, Ezs = ([similar(x) for x in xs] for _ in 1:3)
+    check(ccall((:sc_interpolate_batched, libsc), Cint,
+                (Ptr{Ptr{Cvoid}}, Cint, Cint, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                 Ptr{Int64}, Ptr{Float64}, Ptr{Float64}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}},
+                 Ptr{Ptr{Cvoid}}),
+                [handle(m) for m in meshes], length(meshes), dtcode(P), ptrs(xs), ptrs(ys), ptrs(zs),
+                Int64[length(x) for x in xs], flat3(m -> m.min_bounds, meshes),
+                flat3(m -> m.delta, meshes), ptrs(Exs), ptrs(Eys), ptrs(Ezs)), ctx())
+    return Exs, Eys, Ezs
+end
+
+# phases: 1 = deposit and interpolation phases, 2 = same-shape solves, 3 = both (default)
+set_grouped_launch(phases::Integer=3) =
+    check(ccall((:sc_ctx_set_grouped_launch, libsc), Cint, (Ptr{Cvoid}, Cint), ctx(), Cint(phases)),
+          ctx())
 
 end # module

@@ -41,6 +41,37 @@ def test_library_exports_every_declared_symbol():
         assert name in _lib.PROTOTYPES, f"{name} has no ctypes prototype"
 
 
+def _julia_ccalls():
+    """(symbol, number of argument types) of every ccall into the library in
+    the Julia binding."""
+    text = open(os.path.join(REPO, "spacecharge.jl_amd", "julia", "SpaceChargeMI355.jl")).read()
+    out = []
+    for m in re.finditer(r"ccall\(\(:(sc_\w+), libsc\),\s*[\w{}]+,\s*\(", text):
+        depth, i = 1, m.end()
+        while depth:  # the argument-type tuple, up to its closing parenthesis
+            depth += {"(": 1, ")": -1}.get(text[i], 0)
+            i += 1
+        types = text[m.end():i - 1]
+        types = re.sub(r"\{[^{}]*\}", "", re.sub(r"\{[^{}]*\}", "", types))
+        n = len([t for t in types.split(",") if t.strip()])
+        out.append((m.group(1), n))
+    return out
+
+
+def test_julia_binding_calls_declared_symbols():
+    """Every ccall of the Julia binding names a function the header declares
+    and the library exports, with as many argument types as the ctypes
+    prototype (written from the header) has."""
+    calls = _julia_ccalls()
+    assert len(calls) >= 20
+    declared = set(declared_functions())
+    lib = _lib.load()
+    for name, nargs in calls:
+        assert name in declared, name
+        assert hasattr(lib, name), name
+        assert nargs == len(_lib.PROTOTYPES[name][1]), (name, nargs, len(_lib.PROTOTYPES[name][1]))
+
+
 def test_host_only_calls():
     lib = _lib.load()
     n = lib.sc_kernel_count()
