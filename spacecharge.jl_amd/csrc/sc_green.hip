@@ -184,36 +184,30 @@ k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, i
     F[f_off(a, q) + idx] = field_green_comp(c, u, v, w);
 }
 
-// 8-point difference of the F lattice for cell (i, j, k), in the order of
-// apply_8point_differencing! (src/green_functions.jl:108-111).
+// The 8-point difference of apply_8point_differencing!
+// (src/green_functions.jl:108-111), the mixed third difference of F over the
+// cell, on values f<y><z><x> (x, y, z in {0, 1} relative to the cell's lower
+// corner).  It is formed as differences of differences, x neighbours first:
+// neighbouring F values agree in their leading digits, so each first
+// difference is exact or nearly so (Sterbenz) and the later ones subtract
+// small numbers.  Summed in the reference's order instead, the partial sums
+// swing between |F| and the small result and round at ulp(F) several times;
+// this is the dominant Float64 error of the reference's field at large grids.
+__device__ __forceinline__ double diff8v(double f000, double f001, double f010, double f011,
+                                         double f100, double f101, double f110, double f111)
+{
+    const double g00 = f001 - f000, g01 = f011 - f010;  // along x at (y, z) = (0, 0), (0, 1)
+    const double g10 = f101 - f100, g11 = f111 - f110;  // (1, 0), (1, 1)
+    return (g11 - g10) - (g01 - g00);
+}
+
+// The same difference for cell (i, j, k) read from the F lattice.
 __device__ __forceinline__ double diff8(const double *__restrict__ F, int i, int j, int k, int nx1,
                                         int ny1)
 {
     const int64_t sy = nx1, sz = (int64_t)nx1 * ny1;
     const double *p = F + i + sy * j + sz * k;
-    double s = p[1 + sy + sz] - p[sy + sz];
-    s = s - p[1 + sz];
-    s = s - p[1 + sy];
-    s = s - p[0];
-    s = s + p[sz];
-    s = s + p[sy];
-    s = s + p[1];
-    return s;
-}
-
-// The same difference on preloaded values f<y><z><x> (x, y, z in {0, 1}
-// relative to the cell's lower corner), same order of operations.
-__device__ __forceinline__ double diff8v(double f000, double f001, double f010, double f011,
-                                         double f100, double f101, double f110, double f111)
-{
-    double s = f111 - f110;
-    s = s - f011;
-    s = s - f101;
-    s = s - f000;
-    s = s + f010;
-    s = s + f100;
-    s = s + f001;
-    return s;
+    return diff8v(p[0], p[1], p[sz], p[1 + sz], p[sy], p[1 + sy], p[sy + sz], p[1 + sy + sz]);
 }
 
 template <typename T>
