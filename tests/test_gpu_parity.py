@@ -116,7 +116,10 @@ def _check_fp64(label, got, ref, truth):
     print(f"{label}: rel L2 GPU-vs-oracle {['%.2e' % e for e in e_vs]}; vs extended: "
           f"GPU {['%.2e' % e for e in e_gpu]} oracle {['%.2e' % e for e in e_orc]}")
     for c in range(3):
-        assert e_gpu[c] <= e_orc[c] + 1e-14
+        # the pairwise 8-point difference keeps the GPU several times closer
+        # to the truth than the reference algorithm in Float64 (measured
+        # 4-40x at these sizes)
+        assert e_gpu[c] <= e_orc[c] / 3 + 1e-15
         assert e_vs[c] <= max(1e-12, 2 * e_orc[c])
 
 
@@ -128,6 +131,39 @@ SOLVE_CASES = [((4, 4, 4), 1.0), ((6, 6, 6), 1.0), ((8, 8, 8), 2.0), ((16, 8, 12
 @pytest.mark.parametrize("grid,gamma", SOLVE_CASES)
 def test_solve_fp64_matches_oracle(grid, gamma):
     _check_fp64(f"solve f64 {grid} gamma={gamma}", *_solve_pair(grid, torch.float64, gamma))
+
+
+def test_solve_fp64_full_size_accuracy():
+    """4M particles (50/50 Gaussian and uniform ellipsoid) on 128^3, Float64:
+    every field component within 2e-13 rel. L2 of the extended-precision
+    result and at least 20x closer to it than the Float64 oracle (measured
+    8e-14 / 2.5e-14 / 3.4e-14 against the oracle's 1.2e-11 / 3.9e-12 /
+    5.1e-12; with the reference's summation order in the 8-point difference
+    the GPU was at 1.05e-12 / 2.6e-13 / 4.6e-13)."""
+    import os
+    import sys
+    sc = _sc()
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    grid = (128, 128, 128)
+    x, y, z, q = bench.make_particles(4_000_000, torch.float64, torch.device(DEV))
+    mesh = sc.Mesh3D(grid, x, y, z, T=torch.float64)
+    sc.deposit(mesh, x, y, z, q)
+    sc.solve(mesh)
+    torch.cuda.synchronize()
+    rho = np.asfortranarray(mesh_to_np(mesh._rho, grid))
+    got = mesh_to_np(mesh._ef, grid)
+    th = min(16, os.cpu_count() or 1)
+    args = (rho, mesh.min_bounds, mesh.max_bounds, mesh.delta, 1.0, False)
+    ref = orc.solve(*args, nthreads=th)
+    truth = orc.solve_extended(*args, nthreads=th)
+    e_gpu = [rel_l2(got[..., c], truth[..., c]) for c in range(3)]
+    e_orc = [rel_l2(ref[..., c], truth[..., c]) for c in range(3)]
+    print(f"128^3 Float64 vs extended: GPU {['%.2e' % e for e in e_gpu]} "
+          f"oracle {['%.2e' % e for e in e_orc]}")
+    for c in range(3):
+        assert e_gpu[c] <= 2e-13
+        assert e_gpu[c] <= e_orc[c] / 20
 
 
 @pytest.mark.parametrize("grid,gamma", SOLVE_CASES)
