@@ -820,9 +820,13 @@ int launch_interp_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void
     if (!(ctx->grouped & SC_GROUP_PARTICLES) || nb < 1 || nb > kGroupMax || sizeof(ITable) > group_table_bytes())
         return SC_ERR_UNSUPPORTED;
     const int mdt = ms[0]->dtype;
-    for (int b = 0; b < nb; ++b)
-        if (ms[b]->dtype != mdt || n[b] < 1 || (n[b] + 255) / 256 > 0x7fffffffLL)
-            return SC_ERR_UNSUPPORTED;
+    int64_t blocks = 0;  // the gather's grid over all bunches
+    for (int b = 0; b < nb; ++b) {
+        if (ms[b]->dtype != mdt || n[b] < 1) return SC_ERR_UNSUPPORTED;
+        blocks += (n[b] + 255) / 256;
+        const int64_t M = ms[b]->n[0] * ms[b]->n[1] * ms[b]->n[2];
+        if (blocks > 0x7fffffffLL || (M + 255) / 256 > 0x7fffffffLL) return SC_ERR_UNSUPPORTED;
+    }
     if (int rc = ensure_group_table(ctx, group_table_bytes())) return rc;
     for (int b = 0; b < nb; ++b)
         if (int rc = ensure_efi(ms[b])) return rc;
