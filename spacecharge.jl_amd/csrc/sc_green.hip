@@ -41,39 +41,118 @@ namespace sc {
 
 namespace {
 
+// Float64 log and atan of a few ulp (at most 6.9 and 2.4 ulp against long
+// double over 2e7 arguments each, checked on the CPU), about half the
+// instructions of the library's correctly rounded forms; used for the Green
+// function of Float32 meshes, whose cell integrals need F to ~1e-14 only.
+// log: x = 2^e m, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+// s = (m - 1)/(m + 1), |s| <= 0.1716, the atanh series to s^17; ln 2 split
+// so that e * hi is exact.
+__device__ __forceinline__ double fast_log(double x)
+{
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m += m;
+        e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double p = 1.0 / 17.0;
+    p = fma(p, z, 1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, 1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, 1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, 1.0 / 3.0);
+    const double t = s + s;
+    const double de = (double)e;
+    return fma(de, 0.6931471526622772, fma(t * z, p, fma(de, 2.7897668087737545e-08, t)));
+}
+
+// atan(num / den), one quotient for the range reduction:
+// u = min(|num|, |den|) / max(|num|, |den|) (|num| > |den| ->
+// pi/2 - atan(u)); u > tan(pi/12) -> pi/6 + atan((sqrt(3) u - 1)/(u + sqrt(3))),
+// then |u| <= tan(pi/12) and the odd Taylor series to u^25.  The sign is that
+// of num / den (a zero den gives +-pi/2); 0/0 gives NaN, as atan(0/0).
+__device__ __forceinline__ double fast_atan_q(double num, double den)
+{
+    const double an = fabs(num), ad = fabs(den);
+    const bool inv = an > ad;
+    double u = (inv ? ad : an) / (inv ? an : ad);
+    const bool shift = u > 0.2679491924311227;
+    if (shift) u = fma(u, 1.7320508075688772, -1.0) / (u + 1.7320508075688772);
+    const double z = u * u;
+    double p = 1.0 / 25.0;
+    p = fma(p, z, -1.0 / 23.0);
+    p = fma(p, z, 1.0 / 21.0);
+    p = fma(p, z, -1.0 / 19.0);
+    p = fma(p, z, 1.0 / 17.0);
+    p = fma(p, z, -1.0 / 15.0);
+    p = fma(p, z, 1.0 / 13.0);
+    p = fma(p, z, -1.0 / 11.0);
+    p = fma(p, z, 1.0 / 9.0);
+    p = fma(p, z, -1.0 / 7.0);
+    p = fma(p, z, 1.0 / 5.0);
+    p = fma(p, z, -1.0 / 3.0);
+    double r = fma(u * z, p, u);
+    if (shift) r += 0.5235987755982989;
+    if (inv) r = 1.5707963267948966 - r;
+    return (signbit(num) != signbit(den)) ? -r : r;
+}
+
+template <bool FAST>
+__device__ __forceinline__ double log_(double x)
+{
+    if constexpr (FAST) return fast_log(x);
+    else return log(x);
+}
+
+// atan(num / den)
+template <bool FAST>
+__device__ __forceinline__ double atan_q(double num, double den)
+{
+    if constexpr (FAST) return fast_atan_q(num, den);
+    else return atan(num / den);
+}
+
 // src/green_functions.jl:35-38, in Float64.
+template <bool FAST>
 __device__ __forceinline__ double field_green(double x, double y, double z)
 {
     const double r = sqrt((x * x + y * y) + z * z);
-    const double t1 = x * atan((y * z) / (r * x));
-    const double t2 = z * log(r + y);
-    const double t3 = (y * log((r - z) / (r + z))) * 0.5;
+    const double t1 = x * atan_q<FAST>(y * z, r * x);
+    const double t2 = z * log_<FAST>(r + y);
+    const double t3 = (y * log_<FAST>((r - z) / (r + z))) * 0.5;
     return (t1 - t2) + t3;
 }
 
 // src/green_functions.jl:13-22, in Float64 (zero at the origin, which is
 // never a lattice node: node coordinates are odd multiples of half a cell).
+template <bool FAST>
 __device__ __forceinline__ double potential_green(double x, double y, double z)
 {
     const double r = sqrt((x * x + y * y) + z * z);
     if (r == 0.0) return 0.0;
-    double t = -0.5 * (z * z) * atan((x * y) / (z * r));
-    t = t - 0.5 * (y * y) * atan((x * z) / (y * r));
-    t = t - 0.5 * (x * x) * atan((y * z) / (x * r));
-    t = t + (y * z) * log(x + r);
-    t = t + (x * z) * log(y + r);
-    t = t + (x * y) * log(z + r);
+    double t = -0.5 * (z * z) * atan_q<FAST>(x * y, z * r);
+    t = t - 0.5 * (y * y) * atan_q<FAST>(x * z, y * r);
+    t = t - 0.5 * (x * x) * atan_q<FAST>(y * z, x * r);
+    t = t + (y * z) * log_<FAST>(x + r);
+    t = t + (x * z) * log_<FAST>(y + r);
+    t = t + (x * y) * log_<FAST>(z + r);
     return t;
 }
 
 // Component permutation of get_green_kernel! (src/green_functions.jl:90-98);
 // c == 3 is the scalar potential.
+template <bool FAST>
 __device__ __forceinline__ double field_green_comp(int c, double u, double v, double w)
 {
-    if (c == 0) return field_green(u, v, w);
-    if (c == 1) return field_green(v, w, u);
-    if (c == 2) return field_green(w, u, v);
-    return potential_green(u, v, w);
+    if (c == 0) return field_green<FAST>(u, v, w);
+    if (c == 1) return field_green<FAST>(v, w, u);
+    if (c == 2) return field_green<FAST>(w, u, v);
+    return potential_green<FAST>(u, v, w);
 }
 
 struct GreenArgs {
@@ -157,7 +236,9 @@ __device__ __forceinline__ void green_from_table(GreenArgs &a, const SBunch *sb)
 __device__ __forceinline__ int comp_of(const GreenArgs &a, int q) { return a.pot ? 3 : q % 3; }
 
 // grid (x-y plane blocks, z node planes, components); in-plane index math
-// in 32 bits (this kernel is bound by its Float64 instruction count)
+// in 32 bits (this kernel is bound by its Float64 instruction count).
+// FAST: the few-ulp log and atan (Green functions of Float32 meshes).
+template <bool FAST>
 __global__ void __launch_bounds__(256)
 k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, int per)
 {
@@ -181,7 +262,7 @@ k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, i
     const double u = ((double)in - 0.5) * dx;
     const double v = ((double)jn - 0.5) * dy;
     const double w = image ? ((double)(kn - a.d.nz) + 0.5) * dz + offzg : ((double)kn - 0.5) * dz;
-    F[f_off(a, q) + idx] = field_green_comp(c, u, v, w);
+    F[f_off(a, q) + idx] = field_green_comp<FAST>(c, u, v, w);
 }
 
 // The 8-point difference of apply_8point_differencing!
@@ -710,8 +791,8 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
         const int plane = (d.nx + 1) * (d.ny + 1);
         const int znodes = cath ? d.Lz : d.nz + 1;
         launch_begin(ctx, K_GREEN_F);
-        hipLaunchKernelGGL(k_green_f, dim3((unsigned)div_up(plane, 256), znodes, ncomp), dim3(256), 0,
-                           st, ws.F, a, (const SBunch *)nullptr, 0);
+        hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>, dim3((unsigned)div_up(plane, 256), znodes, ncomp),
+                           dim3(256), 0, st, ws.F, a, (const SBunch *)nullptr, 0);
         launch_end(ctx, K_GREEN_F);
     }
     const bool fast_ok = !m->generic_transforms;
@@ -774,8 +855,8 @@ void build_green_grouped(const Mesh *m0, int nb, const SBunch *tab, double gamma
     {
         const int per = div_up((d.nx + 1) * (d.ny + 1), 256);
         const int znodes = cath ? d.Lz : d.nz + 1;
-        hipLaunchKernelGGL(k_green_f, dim3((unsigned)(per * nb), znodes, ncomp), dim3(256), 0, st,
-                           (double *)nullptr, a, tab, per);
+        hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>, dim3((unsigned)(per * nb), znodes, ncomp),
+                           dim3(256), 0, st, (double *)nullptr, a, tab, per);
     }
     const int zc = cath ? d.Lz - 1 : d.nz;
     const int kp = (d.Kx + 1) / 2;
