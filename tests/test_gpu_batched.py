@@ -283,3 +283,28 @@ def test_grouped_step_captured_in_a_graph():
             assert torch.equal(meshes[b].efield, eager[b].efield), (rep, b)
             for c in range(3):
                 assert torch.equal(Eg[b][c], Ee[b][c]), (rep, b, c)
+
+
+@pytest.mark.parametrize("nb", [12, 256, 300])
+def test_batched_edge_sizes_equal_the_loop(nb):
+    """Bunches of one particle and meshes of 2x2x2 among ordinary ones, and
+    batch sizes at (256: 32 descriptor-store launches) and past (300: the
+    per-bunch path) the grouped-launch limit: Float32 results bit-identical
+    to the single-bunch loop."""
+    sc = _sc()
+    T = torch.float32
+    grids = [(2, 2, 2), (3, 2, 5), (8, 8, 8), (16, 16, 16)]
+    specs = [(1 if b % 5 == 0 else 50 + 37 * (b % 7), grids[b % 4]) for b in range(nb)]
+    parts = [_bunch(700 + b, n, T) for b, (n, _) in enumerate(specs)]
+    bounds = ((-6e-3,) * 3, (6e-3,) * 3)
+    batch = [sc.Mesh3D(g, *bounds, T=T) for _, g in specs]
+    single = [sc.Mesh3D(g, *bounds, T=T) for _, g in specs]
+    rho, E = _batch_run(sc, batch, parts)
+    for m, p, r, e in zip(single, parts, rho, E):
+        sc.deposit(m, *p)
+        sc.solve(m)
+        Es = sc.interpolate_field(m, p[0], p[1], p[2])
+        torch.cuda.synchronize()
+        assert torch.equal(r, m.rho)
+        for c in range(3):
+            assert torch.equal(e[c], Es[c])
