@@ -139,6 +139,18 @@ __device__ __forceinline__ void dft(cpx<T> *v)
     else static_assert(R == 2, "radix must be 2, 4, 8 or 16");
 }
 
+// LDS slice of a pencil stored contiguously (es == 1): element e lives at
+// e + e/E, one pad slot per E elements.  After the first pass lane j stores
+// its E outputs to consecutive elements E j .. E j + E - 1, so without the
+// pad the lanes of one store sit E elements apart and land on a few banks
+// (and every pencil's slice starts on the same bank); with it they are E + 1
+// apart.  Pencils stored interleaved (es = pencils per workgroup) need no pad.
+template <int L, int E>
+constexpr int padded() { return L + L / E; }
+
+template <int E>
+__device__ __forceinline__ int slot(int e, int es) { return es == 1 ? e + e / E : e * es; }
+
 // Radix of pass p for length L with E elements per lane.
 template <int L, int E>
 constexpr int npasses()
@@ -187,7 +199,8 @@ __device__ __forceinline__ void pass_compute(cpx<T> *v, int j, const cpx<T> *__r
 }
 
 // LDS exchange after pass P (all lanes of the workgroup participate).
-// `lds` is this pencil's slice: element e at lds[e * es].
+// `lds` is this pencil's slice: element e at lds[slot<E>(e, es)] (a
+// contiguous slice, es == 1, holds padded<L, E>() entries).
 template <typename T, int L, int E, int P>
 __device__ __forceinline__ void pass_exchange(cpx<T> *v, int j, cpx<T> *lds, int es)
 {
@@ -199,14 +212,14 @@ __device__ __forceinline__ void pass_exchange(cpx<T> *v, int j, cpx<T> *lds, int
         const int b = j + s * TPP;
         const int base = (b / NS) * NS * R + (b & (NS - 1));
 #pragma unroll
-        for (int q = 0; q < R; ++q) lds[(base + q * NS) * es] = v[s * R + q];
+        for (int q = 0; q < R; ++q) lds[slot<E>(base + q * NS, es)] = v[s * R + q];
     }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < NBF2; ++s) {
         const int b = j + s * TPP;
 #pragma unroll
-        for (int q = 0; q < R2; ++q) v[s * R2 + q] = lds[(b + q * (L / R2)) * es];
+        for (int q = 0; q < R2; ++q) v[s * R2 + q] = lds[slot<E>(b + q * (L / R2), es)];
     }
 }
 

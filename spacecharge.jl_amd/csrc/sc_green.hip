@@ -482,7 +482,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     using S = RShape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
-    __shared__ cpx<T> lds[NB * L];
+    __shared__ cpx<T> lds[NB * reg::padded<L, E>()];
     const SBunch *sb;
     const int bx = group_block(tab, per, sb);
     if (sb) {
@@ -504,7 +504,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     // The extension's upper half (e > nx) mirrors cells nx - 1 .. 1, so each
     // cell row is differenced once (e < nx, the first E/2 registers when
     // L = 2 nx) and the mirror is read back through this pencil's LDS slice.
-    cpx<T> *row = lds + b * L;
+    cpx<T> *row = lds + b * reg::padded<L, E>();
     cpx<T> v[E];
     // F rows ya, ya+1, ya+2 (clamped: without a second row the third is
     // unused) of planes k, k+1, loaded without branches two cells at a time

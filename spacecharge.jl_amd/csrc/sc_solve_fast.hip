@@ -46,7 +46,7 @@ k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> 
     using S = Shape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
-    __shared__ cpx<T> lds[NB * L];
+    __shared__ cpx<T> lds[NB * reg::padded<L, E>()];
     const SBunch *sb;
     const int bx = group_block(tab, per, sb);
     if (sb) {
@@ -66,7 +66,7 @@ k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> 
         v[q] = cpx<T>{(T)0, (T)0};
         if (q < E / 2 && live) v[q].x = rho[(size_t)r * nx + j + q * TPP];
     }
-    reg::fft<T, L, E, -1>(v, j, lds + b * L, 1, tw);
+    reg::fft<T, L, E, -1>(v, j, lds + b * reg::padded<L, E>(), 1, tw);
     if (live) {
         const int y = r % d.ny, z = r / d.ny;
         cpx<T> *out = A + ((size_t)z * d.Ly + y) * d.Kp;
@@ -309,7 +309,7 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
     using S = Shape<T, L>;
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
-    __shared__ cpx<T> lds[NB * L];
+    __shared__ cpx<T> lds[NB * reg::padded<L, E>()];
     const SBunch *sb;
     const int bx = group_block(tab, per, sb);
     if (sb) {
@@ -338,7 +338,7 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
         const int k = j + q * TPP;
         if (!(q < E / 2 || k == nx)) v[q].y = -v[q].y;
     }
-    reg::fft<T, L, E, +1>(v, j, lds + b * L, 1, tw);
+    reg::fft<T, L, E, +1>(v, j, lds + b * reg::padded<L, E>(), 1, tw);
     if (live) {
         T *out = E_ + (size_t)c * nx * nrows + (size_t)r * nx;
 #pragma unroll
