@@ -265,6 +265,72 @@ k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, i
     F[f_off(a, q) + idx] = field_green_comp<FAST>(c, u, v, w);
 }
 
+// Float32 meshes: the three field components of a node from shared terms.
+// With r = |(u, v, w)| and F_x = f(u, v, w), F_y = f(v, w, u),
+// F_z = f(w, u, v) (field_green), each component's quotient log follows
+// from the logs of r + u, r + v, r + w and of sums of two squares, since
+// (r - w)(r + w) = u^2 + v^2:
+//   log((r - w) / (r + w)) = log(u^2 + v^2) - 2 log(r + w)   (and cyclically),
+// which also avoids the cancellation in r - w.  The three atan terms satisfy
+//   atan(vw/(ru)) + atan(wu/(rv)) + atan(uv/(rw)) = sign(uvw) pi/2
+// (the tangent of the first two summed is rw/(uv), and their product
+// w^2/r^2 < 1), so the third is formed from the other two.  A lane takes a
+// node (x, y) of kF3Planes consecutive z planes, so log(u^2 + v^2) is formed
+// once per lane.  Per node this is 1 sqrt, 2 atan and 5.25 logs against
+// 3 sqrt, 3 atan, 6 logs and 3 quotients for the three components taken one
+// by one, with the few-ulp functions of k_green_f<true>.
+// Grid (x-y plane blocks, z plane groups, image sets).
+constexpr int kF3Planes = 4;
+
+__global__ void __launch_bounds__(256)
+k_green_f3(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, int per)
+{
+    const SBunch *sb;
+    const int bx = group_block(tab, per, sb);
+    if (sb) {
+        F = sb->F;
+        green_from_table(a, sb);
+    }
+    const int image = blockIdx.z;
+    const int nx1 = a.d.nx + 1, ny1 = a.d.ny + 1;
+    const int pidx = bx * blockDim.x + threadIdx.x;
+    if (pidx >= nx1 * ny1) return;
+    const int in = pidx % nx1;
+    const int jn = pidx / nx1;
+    double dx, dy, dz, offzg;
+    green_geometry(a, dx, dy, dz, offzg);
+    const double u = ((double)in - 0.5) * dx;
+    const double v = ((double)jn - 0.5) * dy;
+    const double uu = u * u, vv = v * v;
+    const double Quv = fast_log(uu + vv);
+    const int64_t plane = (int64_t)nx1 * ny1;
+    double *Fx = F + f_off(a, 3 * image);
+    double *Fy = F + f_off(a, 3 * image + 1);
+    double *Fz = F + f_off(a, 3 * image + 2);
+    const int zn = z_nodes(a, image);
+    const int k0 = blockIdx.y * kF3Planes;
+#pragma unroll
+    for (int kk = 0; kk < kF3Planes; ++kk) {
+        const int kn = k0 + kk;
+        if (kn >= zn) break;
+        const double w = image ? ((double)(kn - a.d.nz) + 0.5) * dz + offzg : ((double)kn - 0.5) * dz;
+        const double ww = w * w;
+        const double r = sqrt((uu + vv) + ww);
+        const double Lu = fast_log(r + u), Lv = fast_log(r + v), Lw = fast_log(r + w);
+        const double Qvw = fast_log(vv + ww), Qwu = fast_log(ww + uu);
+        const double ax = fast_atan_q(v * w, r * u);
+        const double ay = fast_atan_q(w * u, r * v);
+        const double hp = w == 0.0 ? 0.0
+                                   : ((signbit(u) != signbit(v)) != signbit(w) ? -1.5707963267948966
+                                                                               : 1.5707963267948966);
+        const double az = (hp - ax) - ay;
+        const int64_t idx = (int64_t)kn * plane + pidx;
+        Fx[idx] = (u * ax - w * Lv) + (v * (Quv - 2.0 * Lw)) * 0.5;
+        Fy[idx] = (v * ay - u * Lw) + (w * (Qvw - 2.0 * Lu)) * 0.5;
+        Fz[idx] = (w * az - v * Lu) + (u * (Qwu - 2.0 * Lv)) * 0.5;
+    }
+}
+
 // The 8-point difference of apply_8point_differencing!
 // (src/green_functions.jl:108-111), the mixed third difference of F over the
 // cell, on values f<y><z><x> (x, y, z in {0, 1} relative to the cell's lower
@@ -791,8 +857,14 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
         const int plane = (d.nx + 1) * (d.ny + 1);
         const int znodes = cath ? d.Lz : d.nz + 1;
         launch_begin(ctx, K_GREEN_F);
-        hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>, dim3((unsigned)div_up(plane, 256), znodes, ncomp),
-                           dim3(256), 0, st, ws.F, a, (const SBunch *)nullptr, 0);
+        if (sizeof(T) == 4 && !a.pot)
+            hipLaunchKernelGGL(k_green_f3, dim3((unsigned)div_up(plane, 256), div_up(znodes, kF3Planes),
+                                                cath ? 2 : 1),
+                               dim3(256), 0, st, ws.F, a, (const SBunch *)nullptr, 0);
+        else
+            hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>,
+                               dim3((unsigned)div_up(plane, 256), znodes, ncomp), dim3(256), 0, st,
+                               ws.F, a, (const SBunch *)nullptr, 0);
         launch_end(ctx, K_GREEN_F);
     }
     const bool fast_ok = !m->generic_transforms;
@@ -855,8 +927,12 @@ void build_green_grouped(const Mesh *m0, int nb, const SBunch *tab, double gamma
     {
         const int per = div_up((d.nx + 1) * (d.ny + 1), 256);
         const int znodes = cath ? d.Lz : d.nz + 1;
-        hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>, dim3((unsigned)(per * nb), znodes, ncomp),
-                           dim3(256), 0, st, (double *)nullptr, a, tab, per);
+        if (sizeof(T) == 4)
+            hipLaunchKernelGGL(k_green_f3, dim3((unsigned)(per * nb), div_up(znodes, kF3Planes), cath ? 2 : 1),
+                               dim3(256), 0, st, (double *)nullptr, a, tab, per);
+        else
+            hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>, dim3((unsigned)(per * nb), znodes, ncomp),
+                               dim3(256), 0, st, (double *)nullptr, a, tab, per);
     }
     const int zc = cath ? d.Lz - 1 : d.nz;
     const int kp = (d.Kx + 1) / 2;
