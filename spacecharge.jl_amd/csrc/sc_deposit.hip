@@ -55,6 +55,14 @@ constexpr int kBinMaxSlices = 2048;  // binning workgroups (columns of the count
 
 inline int div_up4(int a) { return (a + 3) / 4; }
 
+// The deposit chain runs beside the Green-function build of the solve (on
+// the library's second stream) and is the step's critical path while the
+// build has slack.  Its small latency-bound kernels (row scan, tile scan,
+// reduction, gather) take instruction-issue priority over the build's waves
+// where the two share a SIMD; the binning and tile-deposit kernels do not
+// (measured: slower with it).
+constexpr int kChainPrio = 2;
+
 template <typename A, typename B>
 using wider_t = typename std::conditional<(sizeof(A) >= sizeof(B)), A, B>::type;
 
@@ -205,6 +213,7 @@ __device__ __forceinline__ void bin_rowscan_body(unsigned bx, unsigned *__restri
 __global__ void __launch_bounds__(256)
 k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restrict__ tot)
 {
+    __builtin_amdgcn_s_setprio(kChainPrio);
     bin_rowscan_body(blockIdx.x, cmat, G, ntiles, tot);
 }
 
@@ -288,6 +297,7 @@ k_bin_scan(const unsigned *__restrict__ counts, unsigned *__restrict__ cursor,
            unsigned *__restrict__ nchunks, unsigned *__restrict__ ext_flags, int ntiles,
            int chunk_log2)
 {
+    __builtin_amdgcn_s_setprio(kChainPrio);
     bin_scan_body(counts, cursor, extra_base, chunks, nchunks, ext_flags, ntiles, chunk_log2);
 }
 
@@ -637,6 +647,7 @@ __global__ void __launch_bounds__(256)
 k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__restrict__ counts,
               const unsigned *__restrict__ extra_base, int chunk_log2)
 {
+    __builtin_amdgcn_s_setprio(kChainPrio);
     tile_reduce_body(blockIdx.x, blockIdx.y, blockIdx.z, partials, pstride, counts, extra_base,
                      chunk_log2);
 }
@@ -682,6 +693,7 @@ __global__ void __launch_bounds__(256)
 k_gather_tiles(const double *__restrict__ partials, size_t pstride, Geo g, MT *__restrict__ rho0,
                MT *__restrict__ rho1, MT *__restrict__ rho2, int accumulate)
 {
+    __builtin_amdgcn_s_setprio(kChainPrio);
     const int chan = blockIdx.y;
     gather_tiles_body<MT>(blockIdx.x, partials + chan * pstride, g,
                           chan == 0 ? rho0 : (chan == 1 ? rho1 : rho2), accumulate);
