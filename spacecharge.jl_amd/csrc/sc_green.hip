@@ -212,7 +212,9 @@ __device__ __forceinline__ void green_geometry(const GreenArgs &a, double &dx, d
 // FPEI * factor_c / (8 nx ny nz) of component c (3: the potential).
 __device__ __forceinline__ double green_scale(const GreenArgs &a, int c)
 {
-    if (!a.geom) return a.scale[c];
+    // constant indices: a runtime index into the argument copy (which the
+    // grouped launches overwrite) would place it in scratch memory
+    if (!a.geom) return c == 0 ? a.scale[0] : c == 1 ? a.scale[1] : c == 2 ? a.scale[2] : a.scale[3];
     double dx, dy, dz, offzg;
     green_geometry(a, dx, dy, dz, offzg);
     const double vol = (dx * dy) * dz;

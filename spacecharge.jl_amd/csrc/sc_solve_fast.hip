@@ -112,11 +112,15 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d,
 }
 
 // Fused z convolution (see k_zconv in sc_solve.hip for the algebra), MODE
-// a SolveMode.  Three waves per SIMD: the forward pencil stays in registers
-// across the three inverse transforms, and unconstrained the kernel takes
-// 169 VGPRs (two waves); 168 fit without spilling.
+// a SolveMode.  Free space: three waves per SIMD; the forward pencil stays
+// in registers across the three inverse transforms, and unconstrained the
+// kernel takes 169 VGPRs (two waves); 168 fit without spilling.  With the
+// cathode image the z-reversed pencil's transform is held as well, which
+// does not fit in 168 VGPRs (held to it, the compiler spilled 25-62 VGPRs
+// to scratch), so that mode runs two waves per SIMD.
 template <typename T, int L, int MODE, int THREADS>
-__global__ void __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(THREADS)
+__attribute__((amdgpu_waves_per_eu(MODE == kFieldCathode ? 2 : 3)))
 k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
           const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d,
           const SBunch *__restrict__ tab, int per)
