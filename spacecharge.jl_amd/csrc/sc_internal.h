@@ -178,6 +178,7 @@ struct Mesh {
     Workspace ws;
     DepositBins bins;
     void *efi = nullptr;  // node-interleaved field for the interpolation gather
+    int efi_pair = 1;     // entries per node in efi: 2 = node pairs (large Float64 meshes)
     // device-resident geometry (lo[3], hi[3], delta[3] as doubles holding
     // mesh-type values), written by sc_mesh_auto_bounds on the stream; the
     // step functions read it when called without bounds
