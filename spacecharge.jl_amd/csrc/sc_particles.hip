@@ -103,8 +103,9 @@ struct V4<double> {
 // 128-byte line.  Measured on 4M particles, 128^3: the gather 307 -> 245 us,
 // the copy 26 -> 47 us, the step 0.834 -> 0.792 ms.  On 64x64x128 with a
 // compact bunch the gather did not gain and the copy cost more, so smaller
-// meshes and Float32 (whose 16-byte nodes straddle a line in one pair of
-// eight) keep one node per entry.
+// meshes keep one node per entry, and so do Float32 meshes (16-byte nodes
+// straddle a line in one pair of eight; on 4M particles, 128^3, pairs took
+// the gather from 99 to 134 us).
 // With P = 2 a lane writes one 16-byte quarter of an entry, so the stores
 // are contiguous.
 template <typename MT>
