@@ -342,6 +342,26 @@ int sc_mesh_force_generic_transforms(sc_mesh *mesh, int enable)
     return SC_OK;
 }
 
+int sc_mesh_set_field_layout(sc_mesh *mesh, int layout)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    if (layout < 0 || layout > 2)
+        return set_err(M.ctx, SC_ERR_INVALID, "field layout must be 0 (by size), 1 or 2");
+    if (layout == 2 && M.dtype != SC_F64)
+        return set_err(M.ctx, SC_ERR_INVALID, "node pairs (field layout 2) need a Float64 mesh");
+    if (layout == M.efi_layout) return SC_OK;
+    if (M.efi) {
+        // the copy is rebuilt on the next interpolation; wait for work that reads it
+        SC_HIP(M.ctx, hipSetDevice(M.ctx->device));
+        SC_HIP(M.ctx, hipStreamSynchronize(M.ctx->stream));
+        SC_HIP(M.ctx, hipFree(M.efi));
+        M.efi = nullptr;
+    }
+    M.efi_layout = layout;
+    return SC_OK;
+}
+
 // ---- bounds (src/mesh.jl:95-174) ----
 int sc_bounds_from_particles(sc_ctx *ctx, int pdtype, const void *x, const void *y, const void *z,
                              int64_t n, const int64_t grid[3], int dtype, double lo[3],

@@ -179,6 +179,7 @@ struct Mesh {
     DepositBins bins;
     void *efi = nullptr;  // node-interleaved field for the interpolation gather
     int efi_pair = 1;     // entries per node in efi: 2 = node pairs (large Float64 meshes)
+    int efi_layout = 0;   // requested efi_pair: 0 = by size (ensure_efi), 1 or 2 = forced
     // device-resident geometry (lo[3], hi[3], delta[3] as doubles holding
     // mesh-type values), written by sc_mesh_auto_bounds on the stream; the
     // step functions read it when called without bounds

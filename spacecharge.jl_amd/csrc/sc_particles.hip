@@ -531,8 +531,10 @@ int ensure_efi(Mesh *m)
 {
     if (!m->efi) {
         const int64_t M = m->n[0] * m->n[1] * m->n[2];
-        // node pairs for Float64 fields of 64 MiB and more (see interleave_body)
-        m->efi_pair = (m->dtype == SC_F64 && (size_t)M * 32 >= ((size_t)64 << 20)) ? 2 : 1;
+        // node pairs for Float64 fields of 64 MiB and more (see interleave_body),
+        // unless the mesh asks for a layout
+        if (m->efi_layout) m->efi_pair = m->efi_layout;
+        else m->efi_pair = (m->dtype == SC_F64 && (size_t)M * 32 >= ((size_t)64 << 20)) ? 2 : 1;
         if (hipMalloc(&m->efi, (size_t)M * 4 * m->dtype * m->efi_pair) != hipSuccess)
             return set_err(m->ctx, SC_ERR_NOMEM, "device allocation of the interleaved field failed");
     }

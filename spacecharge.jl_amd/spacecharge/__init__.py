@@ -252,6 +252,16 @@ class Mesh3D:
         """Diagnostic: run every transform on the generic mixed-radix engine."""
         _check(_lib.load().sc_mesh_force_generic_transforms(self._h, int(bool(enable))), None)
 
+    def set_field_layout(self, layout: int = 0) -> None:
+        """Diagnostic: layout of the interpolation's node-interleaved field
+        copy: 1 = one entry per node, 2 = node pairs along x (Float64 meshes
+        only; picked by the library for Float64 fields of 64 MiB and more),
+        0 = by size."""
+        ctx = _ctx(self.device)
+        with torch.cuda.device(self.device):
+            rc = _lib.load().sc_mesh_set_field_layout(self._h, int(layout))
+        _check(rc, ctx)
+
     def update_bounds(self, particles_x, particles_y, particles_z) -> None:
         """Recompute the bounds from the particle extrema with the rules of
         Mesh3D(grid, x, y, z) (src/mesh.jl:95-174) on the device, without a

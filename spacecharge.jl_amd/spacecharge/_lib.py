@@ -45,6 +45,7 @@ PROTOTYPES = {
     "sc_mesh_workspace_bytes": (_i64, [_vp]),
     "sc_mesh_set_green_reuse": (_i, [_vp, _i]),
     "sc_mesh_force_generic_transforms": (_i, [_vp, _i]),
+    "sc_mesh_set_field_layout": (_i, [_vp, _i]),
     "sc_ctx_set_grouped_launch": (_i, [_vp, _i]),
     "sc_mesh_set_deposit_tiling": (_i, [_vp, _i, _i]),
     "sc_mesh_get_deposit_tiling": (_i, [_vp, _pi, _pi]),

@@ -112,13 +112,22 @@ def test_solve_adjoint_dot_product_identity(cath):
     assert abs(lhs - rhs) <= 1e-13 * scale
 
 
-@pytest.mark.parametrize("pt,mt", [(torch.float64, torch.float64), (torch.float32, torch.float32),
-                                   (torch.float64, torch.float32), (torch.float32, torch.float64)])
-def test_deposit_and_interpolate_adjoint_match_oracle(pt, mt):
+@pytest.mark.parametrize("pt,mt,layout", [
+    (torch.float64, torch.float64, 1), (torch.float32, torch.float32, 1),
+    (torch.float64, torch.float32, 1), (torch.float32, torch.float64, 1),
+    (torch.float64, torch.float64, 2), (torch.float32, torch.float64, 2)])
+def test_deposit_and_interpolate_adjoint_match_oracle(pt, mt, layout):
+    """layout: the interpolation's field copy, one entry per node (1) or node
+    pairs along x (2, Float64 meshes); particles on and beyond the upper x
+    face included."""
     sc = _sc()
     grid = (17, 12, 20)
     x, y, z, q = _bunch(30_000, 35, dtype=pt)
     mesh = sc.Mesh3D(grid, x, y, z, T=mt)
+    mesh.set_field_layout(layout)
+    hi, lo, dl = mesh.max_bounds, mesh.min_bounds, mesh.delta
+    x[:3] = torch.tensor([hi[0], hi[0] + 0.3 * dl[0], lo[0]], dtype=pt)
+    y[:3] = torch.tensor([0.0, hi[1], lo[1]], dtype=pt)
     rng = np.random.default_rng(36)
     g_rho = np.asfortranarray(rng.standard_normal(grid))
     ef = np.asfortranarray(rng.standard_normal(grid + (3,)))

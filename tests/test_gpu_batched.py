@@ -308,3 +308,33 @@ def test_batched_edge_sizes_equal_the_loop(nb):
         assert torch.equal(r, m.rho)
         for c in range(3):
             assert torch.equal(e[c], Es[c])
+
+
+def test_grouped_interpolation_node_pair_layout_matches_oracle():
+    """Float64 meshes whose interpolation copy uses node pairs along x (the
+    layout the library picks for Float64 fields of 64 MiB and more, forced
+    here on small meshes) and meshes with one entry per node, in one grouped
+    interpolation: every bunch's fields bit-identical to the oracle's
+    interpolation of that mesh's own efield, particles on and beyond the
+    upper x face included."""
+    import oracle as orc
+    sc = _sc()
+    T = torch.float64
+    specs = [(3_000, (16, 16, 16)), (20_000, (32, 32, 32)), (7_000, (24, 20, 40)),
+             (50_000, (64, 64, 64)), (1_000, (9, 14, 5))]
+    parts = [_bunch(500 + b, n, T) for b, (n, _) in enumerate(specs)]
+    meshes = [sc.Mesh3D(g, p[0], p[1], p[2], T=T) for (_, g), p in zip(specs, parts)]
+    for b, (m, p) in enumerate(zip(meshes, parts)):
+        m.set_field_layout(2 if b % 2 == 0 else 1)
+        hi, dl = m.max_bounds, m.delta
+        p[0][:2] = torch.tensor([hi[0], hi[0] + 0.5 * dl[0]], dtype=T)
+    sc.deposit_batched(meshes, *[[p[k] for p in parts] for k in range(4)])
+    sc.solve_batched(meshes)
+    E = sc.interpolate_batched(meshes, *[[p[k] for p in parts] for k in range(3)])
+    torch.cuda.synchronize()
+    for b, (m, p) in enumerate(zip(meshes, parts)):
+        ef = np.asfortranarray(m.efield.cpu().numpy())
+        xs, ys, zs = (t.cpu().numpy() for t in p[:3])
+        ref = orc.interpolate(ef, xs, ys, zs, m.min_bounds, m.delta)
+        for c in range(3):
+            assert np.array_equal(E[b][c].cpu().numpy(), ref[c]), (b, c)

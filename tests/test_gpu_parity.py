@@ -62,9 +62,14 @@ def test_deposit_matches_oracle(pt, mt):
     assert abs(got.sum(dtype=np.float64) - q.sum(dtype=np.float64)) <= 1e-6 * abs(q.sum())
 
 
+@pytest.mark.parametrize("layout", [0, 1, 2])
 @pytest.mark.parametrize("pt,mt", [(torch.float64, torch.float64), (torch.float32, torch.float32),
                                    (torch.float64, torch.float32), (torch.float32, torch.float64)])
-def test_interpolate_bit_identical(pt, mt):
+def test_interpolate_bit_identical(pt, mt, layout):
+    """Every layout of the gather's field copy (0: by size, 1: one entry per
+    node, 2: node pairs along x, whose last entry of a row holds a zero
+    fill; Float64 meshes only), with particles on and beyond the upper x face (clamped into the
+    last cell, extrapolating weights) and on the lower faces."""
     sc = _sc()
     npdt = np.float64 if pt == torch.float64 else np.float32
     mnp = np.float64 if mt == torch.float64 else np.float32
@@ -72,6 +77,17 @@ def test_interpolate_bit_identical(pt, mt):
     grid = (17, 12, 20)
     xt, yt, zt = _tensors((x, y, z), pt)
     mesh = sc.Mesh3D(grid, xt, yt, zt, T=mt)
+    if layout == 2 and mt == torch.float32:
+        with pytest.raises(sc.SpaceChargeError, match="Float64"):
+            mesh.set_field_layout(layout)
+        return
+    if layout:
+        mesh.set_field_layout(layout)
+    lo, hi, dl = mesh.min_bounds, mesh.max_bounds, mesh.delta
+    x[:6] = [hi[0], hi[0], hi[0] + 0.4 * dl[0], lo[0], lo[0] - 0.3 * dl[0], hi[0] + 2.5 * dl[0]]
+    y[:6] = [hi[1], lo[1], 0.0, hi[1] + 0.2 * dl[1], lo[1], 0.0]
+    z[:6] = [hi[2], 0.0, lo[2], lo[2] - 0.1 * dl[2], hi[2], hi[2] + 1.5 * dl[2]]
+    xt, yt, zt = _tensors((x, y, z), pt)
     rng = np.random.default_rng(3)
     ef = rng.standard_normal(grid + (3,)).astype(mnp)
     ef = np.asfortranarray(ef)
