@@ -207,6 +207,22 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
                    int64_t n, const double min_bounds[3], const double delta[3], void *Ex,
                    void *Ey, void *Ez);
 
+/* No counterpart in the reference: sc_interpolate for callers that cannot
+ * allocate device memory (code under stream or graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, same kernels and
+ * the same bits as sc_interpolate, but it calls no hipMalloc, hipFree or
+ * allocating library routine: the node-interleaved field copy lives in a
+ * fixed 64 MiB array per GPU, held from library load until the process ends,
+ * instead of the mesh's own buffer.  That limits the mesh to 4,194,304 nodes
+ * (Float32, e.g. 160^3) or 2,097,152 nodes (Float64, e.g. 128^3); larger
+ * meshes return SC_ERR_INVALID with nothing launched.  All calls on one GPU
+ * share the array, and it keeps the last call's contents: calls that overlap
+ * on one GPU (different streams or contexts, concurrent graphs) race and
+ * must be ordered by the caller.  Stream-ordered on the context's stream. */
+int sc_interpolate_noalloc(sc_mesh *mesh, int pdtype, const void *x, const void *y,
+                           const void *z, int64_t n, const double min_bounds[3],
+                           const double delta[3], void *Ex, void *Ey, void *Ez);
+
 /* Tile order of the particles (no reference counterpart): perm[i] is the
  * index of the particle that goes to position i when the particles are
  * grouped by the 16^3-cell tile of the deposit that holds them (tiles in

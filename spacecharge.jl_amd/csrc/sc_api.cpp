@@ -11,6 +11,8 @@
 #include <new>
 #include <string>
 
+#include <mutex>
+
 #include "sc_internal.h"
 
 namespace sc {
@@ -97,6 +99,34 @@ static int check_dev(Ctx *ctx)
     return SC_OK;
 }
 
+// ---- the allocation-free interpolation's scratch ----
+// One fixed array per GPU, present from the moment the library's code is
+// loaded on it until the process ends: it holds the node-interleaved field
+// copy of sc_interpolate_noalloc.  In host code `scratch` names a host-side
+// stand-in; the array's device address is looked up once per device
+// (scratch_address, called by sc_ctx_create with the device current) and
+// only that address is handed to kernels.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static std::vector<void *> g_scratch_ptr;  // per device index, nullptr until looked up
+
+static int scratch_address(Ctx *ctx, int device, void **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= device) g_scratch_ptr.resize(device + 1, nullptr);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_err(ctx, e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return set_err(ctx, SC_ERR_DEVICE, "scratch array has no device address");
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return SC_OK;
+}
+
 }  // namespace sc
 
 using namespace sc;
@@ -126,10 +156,13 @@ int sc_ctx_create(int device, sc_ctx **out)
                            " visible)");
     e = hipSetDevice(device);
     if (e != hipSuccess) return hip_err(nullptr, e, "hipSetDevice");
+    void *scr = nullptr;
+    if (int rc = scratch_address(nullptr, device, &scr)) return rc;
     sc_ctx *c = new (std::nothrow) sc_ctx();
     if (!c) return set_err(nullptr, SC_ERR_NOMEM, "host allocation failed");
     c->c.device = device;
     c->c.stream = nullptr;
+    c->c.scratch = scr;
     *out = c;
     return SC_OK;
 }
@@ -625,6 +658,30 @@ int sc_interpolate(sc_mesh *mesh, int pdtype, const void *x, const void *y, cons
     if (int rc = check_geom_m(c, M, lo, dl)) return rc;
     if (int rc = check_dev(c)) return rc;
     return launch_interp(&M, pdtype, x, y, z, n, lo, dl, Ex, Ey, Ez);
+}
+
+int sc_interpolate_noalloc(sc_mesh *mesh, int pdtype, const void *x, const void *y,
+                           const void *z, int64_t n, const double lo[3], const double dl[3],
+                           void *Ex, void *Ey, void *Ez)
+{
+    if (!mesh) return set_err(nullptr, SC_ERR_INVALID, "mesh is NULL");
+    Mesh &M = mesh->m;
+    Ctx *c = M.ctx;
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n < 0) return set_err(c, SC_ERR_INVALID, "negative particle count");
+    if (n > 0 && (!x || !y || !z || !Ex || !Ey || !Ez))
+        return set_err(c, SC_ERR_INVALID, "NULL particle or output pointer");
+    // one (Ex, Ey, Ez, 0) entry of the mesh type per node
+    const size_t need = (size_t)M.n[0] * (size_t)M.n[1] * (size_t)M.n[2] * 4 * (size_t)M.dtype;
+    if (need > kScratchBytes)
+        return set_err(c, SC_ERR_INVALID,
+                       "mesh too large for the allocation-free interpolation: its field copy "
+                       "needs " + std::to_string(need) + " bytes, the scratch holds " +
+                           std::to_string(kScratchBytes));
+    if (!c->scratch) return set_err(c, SC_ERR_DEVICE, "context has no scratch address");
+    if (int rc = check_geom_m(c, M, lo, dl)) return rc;
+    if (int rc = check_dev(c)) return rc;
+    return launch_interp_into(&M, c->scratch, 1, pdtype, x, y, z, n, lo, dl, Ex, Ey, Ez);
 }
 
 int sc_sort_permutation(sc_mesh *mesh, int pdtype, const void *x, const void *y, const void *z,

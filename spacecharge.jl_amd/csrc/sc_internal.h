@@ -91,6 +91,8 @@ struct Ctx {
     int64_t launches[K_COUNT] = {};
     // bounds-reduction partials, allocated on first use
     double *red = nullptr;
+    // device address of this GPU's fixed scratch array (sc_interpolate_noalloc)
+    void *scratch = nullptr;
     // second stream (default priority, non-blocking), created on first use:
     // the Green-function build of a solve runs on it, concurrently with the
     // deposit still executing on `stream`
@@ -276,6 +278,11 @@ int launch_interp_vjp(Mesh *m, int pdt, const void *x, const void *y, const void
 void free_bins(Mesh *m);
 int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
                   const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez);
+// launch_interp with the node-interleaved field copy in the caller's buffer
+// `efi` (P entries per node, nx*ny*nz*P*4 mesh-type values); allocates nothing.
+int launch_interp_into(Mesh *m, void *efi, int P, int pdt, const void *x, const void *y,
+                       const void *z, int64_t n, const double lo[3], const double dl[3], void *Ex,
+                       void *Ey, void *Ez);
 int launch_bounds(Ctx *ctx, int pdt, const void *x, const void *y, const void *z, int64_t n,
                   double mn[3], double mx[3]);
 // The particle-extrema bounds of src/mesh.jl:95-174 computed on the device

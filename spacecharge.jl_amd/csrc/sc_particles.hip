@@ -473,27 +473,27 @@ void dep(Mesh *m, const void *x, const void *y, const void *z, const void *q, in
                        h.d[0], h.d[1], h.d[2], h.dev);
 }
 
+// The node-interleaved copy of m->efield in `efi` (P entries per node).
 template <typename MT>
-void interleave(Mesh *m)
+void interleave(Mesh *m, void *efi, int P)
 {
     const int64_t M = m->n[0] * m->n[1] * m->n[2];
-    const int P = m->efi_pair;
     hipLaunchKernelGGL((k_interleave<MT>), dim3((unsigned)blocks_for(P == 2 ? 4 * M : M, 256)),
                        dim3(256), 0,
                        m->ctx->stream, (const MT *)m->efield,
-                       static_cast<typename V4<MT>::type *>(m->efi), M, (int)m->n[0], P);
+                       static_cast<typename V4<MT>::type *>(efi), M, (int)m->n[0], P);
 }
 
 template <typename PT, typename MT>
-void itp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, const double lo[3],
-         const double dl[3], void *Ex, void *Ey, void *Ez)
+void itp(Mesh *m, const void *efi, int P, const void *x, const void *y, const void *z, int64_t n,
+         const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez)
 {
     const LaunchGeo h = launch_geo(m, lo, dl);
     hipLaunchKernelGGL((k_interp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
-                       m->ctx->stream, static_cast<const typename V4<MT>::type *>(m->efi),
+                       m->ctx->stream, static_cast<const typename V4<MT>::type *>(efi),
                        (const PT *)x, (const PT *)y, (const PT *)z, n, (int)m->n[0], (int)m->n[1],
                        (int)m->n[2], h.l[0], h.l[1], h.l[2], h.d[0], h.d[1], h.d[2], h.dev,
-                       (PT *)Ex, (PT *)Ey, (PT *)Ez, m->efi_pair);
+                       (PT *)Ex, (PT *)Ey, (PT *)Ez, P);
 }
 
 }  // namespace
@@ -592,27 +592,39 @@ int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z
     return launch_deposit_to(m, pdt, x, y, z, q, n, lo, dl, clear, m->rho);
 }
 
-int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
-                  const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez)
+int launch_interp_into(Mesh *m, void *efi, int P, int pdt, const void *x, const void *y,
+                       const void *z, int64_t n, const double lo[3], const double dl[3], void *Ex,
+                       void *Ey, void *Ez)
 {
     if (n <= 0) return SC_OK;
     if ((n + 255) / 256 > 0x7fffffffLL)
         return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
-    if (int rc = ensure_efi(m)) return rc;
     // efield may have been written by the caller since the last solve, so the
     // interleaved copy is refreshed on every call
     launch_begin(m->ctx, K_INTERLEAVE);
-    if (m->dtype == SC_F32) interleave<float>(m);
-    else interleave<double>(m);
+    if (m->dtype == SC_F32) interleave<float>(m, efi, P);
+    else interleave<double>(m, efi, P);
     launch_end(m->ctx, K_INTERLEAVE);
     launch_begin(m->ctx, K_INTERP);
-    if (pdt == SC_F32 && m->dtype == SC_F32) itp<float, float>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
-    else if (pdt == SC_F64 && m->dtype == SC_F64) itp<double, double>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
-    else if (pdt == SC_F64 && m->dtype == SC_F32) itp<double, float>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
-    else itp<float, double>(m, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    if (pdt == SC_F32 && m->dtype == SC_F32)
+        itp<float, float>(m, efi, P, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    else if (pdt == SC_F64 && m->dtype == SC_F64)
+        itp<double, double>(m, efi, P, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    else if (pdt == SC_F64 && m->dtype == SC_F32)
+        itp<double, float>(m, efi, P, x, y, z, n, lo, dl, Ex, Ey, Ez);
+    else
+        itp<float, double>(m, efi, P, x, y, z, n, lo, dl, Ex, Ey, Ez);
     launch_end(m->ctx, K_INTERP);
     SC_HIP(m->ctx, hipGetLastError());
     return SC_OK;
+}
+
+int launch_interp(Mesh *m, int pdt, const void *x, const void *y, const void *z, int64_t n,
+                  const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez)
+{
+    if (n <= 0) return SC_OK;
+    if (int rc = ensure_efi(m)) return rc;
+    return launch_interp_into(m, m->efi, m->efi_pair, pdt, x, y, z, n, lo, dl, Ex, Ey, Ez);
 }
 
 
@@ -662,8 +674,8 @@ int launch_interp_vjp(Mesh *m, int pdt, const void *x, const void *y, const void
         return set_err(m->ctx, SC_ERR_INVALID, "too many particles for one launch");
     if (int rc = ensure_efi(m)) return rc;
     launch_begin(m->ctx, K_INTERLEAVE);
-    if (m->dtype == SC_F32) interleave<float>(m);
-    else interleave<double>(m);
+    if (m->dtype == SC_F32) interleave<float>(m, m->efi, m->efi_pair);
+    else interleave<double>(m, m->efi, m->efi_pair);
     launch_end(m->ctx, K_INTERLEAVE);
     launch_begin(m->ctx, K_INTERP_VJP);
     if (pdt == SC_F32 && m->dtype == SC_F32)

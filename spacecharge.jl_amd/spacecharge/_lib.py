@@ -58,6 +58,7 @@ PROTOTYPES = {
     "sc_solve": (_i, [_vp, _pd, _pd, _pd, _d, _i]),
     "sc_solve_potential": (_i, [_vp, _pd, _d, _vp]),
     "sc_interpolate": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pd, _pd, _vp, _vp, _vp]),
+    "sc_interpolate_noalloc": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pd, _pd, _vp, _vp, _vp]),
     "sc_sort_permutation": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pd, _pd, _vp]),
     "sc_deposit_adjoint": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64, _pd, _pd, _vp, _vp, _vp, _vp,
                                 _vp, _i]),
