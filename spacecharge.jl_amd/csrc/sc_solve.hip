@@ -54,9 +54,9 @@ k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + d.Lx;
-    const int Lx = d.Lx, nx = d.nx, nrows = d.ny * d.nz;
+    const int Lx = d.Lx, nx = d.mx, nrows = d.my * d.mz;
     const T *rho = in + (size_t)blockIdx.y * nx * nrows;
-    cpx<T> *A = out + (size_t)blockIdx.y * d.nz * d.Ly * d.Kp;
+    cpx<T> *A = out + (size_t)blockIdx.y * d.mz * d.Ly * d.Kp;
     load_twiddles(tw, twg, Lx);
     const int r0 = blockIdx.x * nb;
     for (int idx = threadIdx.x; idx < nb * Lx; idx += blockDim.x) {
@@ -72,7 +72,7 @@ k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__
         const int b = idx / d.Kx, kx = idx - b * d.Kx;
         const int r = r0 + b;
         if (r < nrows) {
-            const int y = r % d.ny, z = r / d.ny;
+            const int y = r % d.my, z = r / d.my;
             A[((size_t)z * d.Ly + y) * d.Kp + kx] = buf[b * Lx + kx];
         }
     }
@@ -90,12 +90,12 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
     const int Ly = d.Ly, Kx = d.Kx, Kp = d.Kp;
     load_twiddles(tw, twg, Ly);
     const int kx0 = blockIdx.x * nb, z = blockIdx.y;
-    cpx<T> *Ap = A + ((size_t)blockIdx.z * d.nz + z) * Ly * Kp;
+    cpx<T> *Ap = A + ((size_t)blockIdx.z * d.mz + z) * Ly * Kp;
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
         cpx<T> v{(T)0, (T)0};
-        if (kx < Kx && e < d.ny) v = Ap[(size_t)e * Kp + kx];
+        if (kx < Kx && e < d.my) v = Ap[(size_t)e * Kp + kx];
         buf[idx] = v;
     }
     __syncthreads();
@@ -123,7 +123,8 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
     cpx<T> *P0 = tw + d.Lz;
     cpx<T> *W = P0 + nb * d.Lz;
     cpx<T> *P1 = W + nb * d.Lz;  // only with the cathode
-    const int Lz = d.Lz, nz = d.nz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
+    // nz, ny: transform half-lengths (spectral octant); mz: mesh z nodes
+    const int Lz = d.Lz, nz = d.nz, mz = d.mz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
     const int ny1 = ny + 1;
     load_twiddles(tw, twg, Lz);
     const int kx0 = blockIdx.x * nb, ky = blockIdx.y;
@@ -132,10 +133,10 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
         cpx<T> v{(T)0, (T)0}, vi{(T)0, (T)0};
-        if (kx < Kx && e < nz) {
+        if (kx < Kx && e < mz) {
             v = A[(size_t)e * zs + (size_t)ky * Kp + kx];
             if (cathode) {
-                const cpx<T> t = A[(size_t)(nz - 1 - e) * zs + (size_t)ky * Kp + kx];
+                const cpx<T> t = A[(size_t)(mz - 1 - e) * zs + (size_t)ky * Kp + kx];
                 vi = cpx<T>{-t.x, -t.y};
             }
         }
@@ -176,8 +177,8 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
         }
         __syncthreads();
         lds_fft(W, tw, pl, nb, nb, 1, true, +1);
-        cpx<T> *Bc = B + (size_t)c * nz * zs;
-        for (int idx = threadIdx.x; idx < nb * nz; idx += blockDim.x) {
+        cpx<T> *Bc = B + (size_t)c * mz * zs;
+        for (int idx = threadIdx.x; idx < nb * mz; idx += blockDim.x) {
             const int z = idx / nb, b = idx - z * nb;
             const int kx = kx0 + b;
             if (kx < Kx) Bc[(size_t)z * zs + (size_t)ky * Kp + kx] = W[idx];
@@ -201,7 +202,8 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
     cpx<T> *Q = tw + d.Lz;
     cpx<T> *W = Q + nb * d.Lz;
     cpx<T> *W1 = W + nb * d.Lz;  // only with the cathode
-    const int Lz = d.Lz, nz = d.nz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
+    // nz, ny: transform half-lengths (spectral octant); mz: mesh z nodes
+    const int Lz = d.Lz, nz = d.nz, mz = d.mz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
     const int ny1 = ny + 1;
     load_twiddles(tw, twg, Lz);
     const int kx0 = blockIdx.x * nb, ky = blockIdx.y;
@@ -215,12 +217,12 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
         if (cathode) W1[idx] = cpx<T>{(T)0, (T)0};
     }
     for (int c = 0; c < 3; ++c) {
-        const cpx<T> *Gc = Gs + (size_t)c * nz * zs;
+        const cpx<T> *Gc = Gs + (size_t)c * mz * zs;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
             const int e = idx / nb, b = idx - e * nb;
             const int kx = kx0 + b;
             cpx<T> v{(T)0, (T)0};
-            if (kx < Kx && e < nz) v = Gc[(size_t)e * zs + (size_t)ky * Kp + kx];
+            if (kx < Kx && e < mz) v = Gc[(size_t)e * zs + (size_t)ky * Kp + kx];
             Q[idx] = v;
         }
         __syncthreads();
@@ -251,12 +253,12 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
     lds_fft(W, tw, pl, nb, nb, 1, true, +1);
     if (cathode) lds_fft(W1, tw, pl, nb, nb, 1, true, +1);
     __syncthreads();
-    for (int idx = threadIdx.x; idx < nb * nz; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < nb * mz; idx += blockDim.x) {
         const int z = idx / nb, b = idx - z * nb;
         const int kx = kx0 + b;
         if (kx < Kx) {
             cpx<T> v = W[idx];
-            if (cathode) v = c_sub(v, W1[(nz - 1 - z) * nb + b]);
+            if (cathode) v = c_sub(v, W1[(mz - 1 - z) * nb + b]);
             out[(size_t)z * zs + (size_t)ky * Kp + kx] = v;
         }
     }
@@ -273,7 +275,7 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
     const int Ly = d.Ly, Kx = d.Kx, Kp = d.Kp;
     load_twiddles(tw, twg, Ly);
     const int kx0 = blockIdx.x * nb, z = blockIdx.y, c = blockIdx.z;
-    cpx<T> *Bp = B + ((size_t)c * d.nz + z) * Ly * Kp;
+    cpx<T> *Bp = B + ((size_t)c * d.mz + z) * Ly * Kp;
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
@@ -283,7 +285,7 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
     }
     __syncthreads();
     lds_fft(buf, tw, pl, nb, nb, 1, true, +1);
-    for (int idx = threadIdx.x; idx < nb * d.ny; idx += blockDim.x) {
+    for (int idx = threadIdx.x; idx < nb * d.my; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
         if (kx < Kx) Bp[(size_t)e * Kp + kx] = buf[idx];
@@ -300,29 +302,30 @@ k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ E, const cpx<T> *__restric
     extern __shared__ __align__(16) unsigned char smem[];
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *buf = tw + d.Lx;
-    const int Lx = d.Lx, nx = d.nx, Kx = d.Kx, Kp = d.Kp, nrows = d.ny * d.nz;
+    // hx: transform half-length (Hermitian completion); nx: mesh x nodes
+    const int Lx = d.Lx, hx = d.nx, nx = d.mx, Kx = d.Kx, Kp = d.Kp, nrows = d.my * d.mz;
     load_twiddles(tw, twg, Lx);
     const int r0 = blockIdx.x * nb, c = blockIdx.y;
-    const cpx<T> *Bc = B + (size_t)c * d.nz * d.Ly * Kp;
+    const cpx<T> *Bc = B + (size_t)c * d.mz * d.Ly * Kp;
     for (int idx = threadIdx.x; idx < nb * Kx; idx += blockDim.x) {
         const int b = idx / Kx, kx = idx - b * Kx;
         const int r = r0 + b;
         cpx<T> v{(T)0, (T)0};
         if (r < nrows) {
-            const int y = r % d.ny, z = r / d.ny;
+            const int y = r % d.my, z = r / d.my;
             v = Bc[((size_t)z * d.Ly + y) * Kp + kx];
         }
         buf[b * Lx + kx] = v;
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < nb * (nx - 1); idx += blockDim.x) {
-        const int b = idx / (nx - 1), k = idx - b * (nx - 1) + 1;
+    for (int idx = threadIdx.x; idx < nb * (hx - 1); idx += blockDim.x) {
+        const int b = idx / (hx - 1), k = idx - b * (hx - 1) + 1;
         const cpx<T> v = buf[b * Lx + k];
         buf[b * Lx + Lx - k] = cpx<T>{v.x, -v.y};
     }
     __syncthreads();
     lds_fft(buf, tw, pl, nb, 1, Lx, false, +1);
-    T *Ec = E + (size_t)c * d.nx * nrows;
+    T *Ec = E + (size_t)c * nx * nrows;
     for (int idx = threadIdx.x; idx < nb * nx; idx += blockDim.x) {
         const int b = idx / nx, x = idx - b * nx;
         const int r = r0 + b;
@@ -406,7 +409,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
 
     // Each transform kernel runs on the register-resident engine when its
     // length is a supported power of two, else on the generic LDS engine.
-    const bool fast_ok = !m->generic_transforms;
+    const bool fast_ok = !m->generic_transforms && !dims_padded(d);
     auto fast = [&](int which) { return fast_ok && launch_fast<T>(m, which, io); };
 
     // ---- forward x, y ----
@@ -416,7 +419,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
         launch_begin(ctx, K_RHO_FWD_X);
-        hipLaunchKernelGGL(k_rho_fwd_x<T>, dim3(div_up(d.ny * d.nz, nb), io.nin), dim3(kFftThreads),
+        hipLaunchKernelGGL(k_rho_fwd_x<T>, dim3(div_up(d.my * d.mz, nb), io.nin), dim3(kFftThreads),
                            sh, st, static_cast<const T *>(io.in), F, twx, ws.plan[0], d, nb);
         launch_end(ctx, K_RHO_FWD_X);
     }
@@ -424,7 +427,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
         launch_begin(ctx, K_RHO_FWD_Y);
-        hipLaunchKernelGGL(k_rho_fwd_y<T>, dim3(div_up(d.Kx, nb), d.nz, io.nin), dim3(kFftThreads),
+        hipLaunchKernelGGL(k_rho_fwd_y<T>, dim3(div_up(d.Kx, nb), d.mz, io.nin), dim3(kFftThreads),
                            sh, st, F, twy, ws.plan[1], d, nb);
         launch_end(ctx, K_RHO_FWD_Y);
     }
@@ -455,7 +458,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
         launch_begin(ctx, K_INV_Y);
-        hipLaunchKernelGGL(k_inv_y<T>, dim3(div_up(d.Kx, nb), d.nz, io.nout), dim3(kFftThreads), sh,
+        hipLaunchKernelGGL(k_inv_y<T>, dim3(div_up(d.Kx, nb), d.mz, io.nout), dim3(kFftThreads), sh,
                            st, B, twy, ws.plan[1], d, nb);
         launch_end(ctx, K_INV_Y);
     }
@@ -463,7 +466,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
         const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
         const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
         launch_begin(ctx, K_INV_X);
-        hipLaunchKernelGGL(k_inv_x<T>, dim3(div_up(d.ny * d.nz, nb), io.nout), dim3(kFftThreads), sh,
+        hipLaunchKernelGGL(k_inv_x<T>, dim3(div_up(d.my * d.mz, nb), io.nout), dim3(kFftThreads), sh,
                            st, (const cpx<T> *)B, static_cast<T *>(io.out), twx, ws.plan[0], d, nb);
         launch_end(ctx, K_INV_X);
     }
@@ -549,24 +552,26 @@ int ensure_workspace(Mesh *m)
     Workspace &ws = m->ws;
     if (ws.ready) return SC_OK;
     Ctx *ctx = m->ctx;
-    const int64_t nx = m->n[0], ny = m->n[1], nz = m->n[2];
-    const int L[3] = {(int)(2 * nx), (int)(2 * ny), (int)(2 * nz)};
+    const Dims d = make_dims(m);
+    // transform half-lengths (padded beyond the mesh's nodes where 2n is not 7-smooth)
+    const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
+    const int L[3] = {d.Lx, d.Ly, d.Lz};
+    const int64_t mn[3] = {m->n[0], m->n[1], m->n[2]};
     const int cap = (m->dtype == SC_F32) ? fft_cap<float>() : fft_cap<double>();
     for (int a = 0; a < 3; ++a) {
-        if (!make_plan(L[a], &ws.plan[a]))
-            return set_err(ctx, SC_ERR_UNSUPPORTED,
-                           "grid size " + std::to_string(L[a] / 2) +
-                               ": the padded length has a prime factor above 7, which the "
-                               "MI355X transform does not support");
         // one pencil must fit a workgroup's register budget
         if (L[a] > cap)
             return set_err(ctx, SC_ERR_UNSUPPORTED,
-                           "grid size " + std::to_string(L[a] / 2) + " exceeds the maximum of " +
-                               std::to_string(cap / 2) + " per axis");
+                           "grid size " + std::to_string(mn[a]) + " exceeds the maximum of " +
+                               std::to_string(cap / 2) + " per axis for this mesh type");
+        if (!make_plan(L[a], &ws.plan[a]))  // unreachable: transform_half gives 7-smooth lengths
+            return set_err(ctx, SC_ERR_UNSUPPORTED,
+                           "no transform plan for length " + std::to_string(L[a]));
     }
     const size_t s = (m->dtype == SC_F32) ? 4 : 8;
-    const size_t Kp = make_dims(m).Kp;  // row stride of every kx-indexed array
-    const size_t spec = (size_t)nz * 2 * ny * Kp;  // complex elements of one spectrum
+    const size_t Kp = d.Kp;  // row stride of every kx-indexed array
+    // complex elements of one spectrum: [mesh z][Ly][Kp]
+    const size_t spec = (size_t)d.mz * d.Ly * Kp;
     const size_t gf = (size_t)(nz + 1) * (ny + 1) * Kp;
     const size_t gi = (size_t)2 * nz * (ny + 1) * Kp;
     // Green build scratch: F at the nodes (Float64), R rows (T), free + image
@@ -712,15 +717,15 @@ bool solve_groupable(const Mesh *m)
 {
     if (m->generic_transforms || m->green_reuse) return false;
     const Dims d = make_dims(m);
-    return reg_length(d.Lx) && reg_length(d.Ly) && reg_length(d.Lz);
+    return !dims_padded(d) && reg_length(d.Lx) && reg_length(d.Ly) && reg_length(d.Lz);
 }
 
 int64_t solve_group_bytes(const Mesh *m)
 {
     const Dims d = make_dims(m);
     const int64_t s = m->dtype == SC_F32 ? 4 : 8;
-    // four spectra (rho and three fields) of [nz][2ny][Kp] complex
-    return 4 * 2 * s * (int64_t)d.nz * d.Ly * d.Kp;
+    // four spectra (rho and three fields) of [mesh z][Ly][Kp] complex
+    return 4 * 2 * s * (int64_t)d.mz * d.Ly * d.Kp;
 }
 
 int ensure_solve_table(Ctx *ctx)

@@ -11,8 +11,39 @@ namespace sc {
 
 constexpr double kFPEI = 299792458.0 * 299792458.0 * 1.0e-7;  // src/utils.jl:7-8
 
+// true when L has no prime factor above 7 (the radices of the transforms)
+inline bool smooth7(int L)
+{
+    if (L < 1) return false;
+    for (int f : {2, 3, 5, 7})
+        while (L % f == 0) L /= f;
+    return L == 1;
+}
+
+// Transform half-length of an axis of n nodes: the smallest t >= n whose 2t
+// has no prime factor above 7.  The reference zero-pads each axis to exactly
+// 2n (src/mesh.jl:51-61) and FFTW transforms any length.  The convolution it
+// computes is linear: output node m < n sums rho[j] G(m - j) over j < n, lags
+// |m - j| <= n - 1, so any cyclic length of at least 2n - 1 gives the same
+// outputs.  Axes whose 2n has a larger prime factor are therefore padded
+// further, to 2t: rho stays zero beyond its n nodes, the Green function is
+// built on t lags (its first n are the reference's cell integrals; it depends
+// on the spacing, gamma and the image offset only, not on n), and only the n
+// physical outputs are written.  t == n for every axis whose 2n is 7-smooth.
+inline int transform_half(int n)
+{
+    int t = n;
+    while (!smooth7(2 * t)) ++t;
+    return t;
+}
+
+// Shape of a solve.  nx, ny, nz are the transform half-lengths (the Green
+// function's lags and spectral octant); mx, my, mz the mesh's nodes (the
+// extent of rho, efield and the real rows the passes read and write).  They
+// differ only on axes padded beyond 2n (transform_half).
 struct Dims {
-    int nx, ny, nz;
+    int nx, ny, nz;  // transform half-lengths: L = 2n, spectral octant n + 1
+    int mx, my, mz;  // mesh nodes (data extent)
     int Kx;          // nx + 1 kept x frequencies
     int Kp;          // row stride of kx-indexed arrays: Kx rounded up to 32, so that the
                      // 16-lane row segments of every transform start on a 128-byte line
@@ -22,9 +53,12 @@ struct Dims {
 inline Dims make_dims(const Mesh *m)
 {
     Dims d;
-    d.nx = (int)m->n[0];
-    d.ny = (int)m->n[1];
-    d.nz = (int)m->n[2];
+    d.mx = (int)m->n[0];
+    d.my = (int)m->n[1];
+    d.mz = (int)m->n[2];
+    d.nx = transform_half(d.mx);
+    d.ny = transform_half(d.my);
+    d.nz = transform_half(d.mz);
     d.Kx = d.nx + 1;
     d.Kp = (d.Kx + 31) / 32 * 32;
     d.Lx = 2 * d.nx;
@@ -32,6 +66,9 @@ inline Dims make_dims(const Mesh *m)
     d.Lz = 2 * d.nz;
     return d;
 }
+
+// true when some axis is padded beyond twice its nodes (generic engine only)
+inline bool dims_padded(const Dims &d) { return d.nx != d.mx || d.ny != d.my || d.nz != d.mz; }
 
 inline int div_up(int a, int b) { return (a + b - 1) / b; }
 
@@ -63,11 +100,11 @@ inline int green_mode(int mode)
     return mode == kPotential ? kPotential : (mode_cathode(mode) ? kFieldCathode : kFieldFree);
 }
 
-// Buffers of one solve: the real input `in` (nin components of nx*ny*nz) is
-// transformed along x and y into the spectra `fwd` (nin of them), the z
-// convolution writes nout spectra to `inv`, and the inverse y and x passes
-// write the real output `out` (nout components).  Spectra are
-// [nz][2ny][Kp] complex each, consecutive.
+// Buffers of one solve: the real input `in` (nin components of the mesh's
+// mx*my*mz nodes) is transformed along x and y into the spectra `fwd` (nin of
+// them), the z convolution writes nout spectra to `inv`, and the inverse y
+// and x passes write the real output `out` (nout components).  Spectra are
+// [mz][Ly][Kp] complex each, consecutive.
 struct SolveIO {
     int mode;
     int nin, nout;

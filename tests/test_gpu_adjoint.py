@@ -45,7 +45,9 @@ def _bunch(n, seed, dtype=torch.float64, sigma=(1e-3, 1e-3, 2e-3), zshift=0.0):
 
 SOLVE_CASES = [((8, 8, 8), 1.0, False), ((16, 8, 12), 2.0, False), ((24, 20, 40), 10.0, False),
                ((32, 32, 64), 1.0, False), ((8, 8, 8), 1.0, True), ((16, 16, 32), 2.0, True),
-               ((12, 10, 6), 1.0, True), ((32, 64, 32), 1.5, True)]
+               ((12, 10, 6), 1.0, True), ((32, 64, 32), 1.5, True),
+               # 2n with a prime factor above 7 (transforms padded to a 7-smooth length)
+               ((11, 13, 17), 1.0, False), ((13, 11, 19), 2.0, True)]
 
 
 @pytest.mark.parametrize("generic", [False, True])

@@ -204,8 +204,32 @@ def test_generic_engine_matches_oracle(grid, gamma, T):
     assert max(errs) <= 1e-5
 
 
+# Grids whose padded length 2n has a prime factor above 7.  The reference's
+# FFTW plans take any length (src/mesh.jl:51-61); here such an axis is
+# transformed at the next 7-smooth even length, with rho zero beyond its n
+# nodes (the linear convolution needs a cyclic length of only 2n - 1).
+# (31, 33, 62) pads x to a power of two, which the register engine would
+# take for an unpadded axis.
+PADDED_CASES = [((11, 11, 11), 1.0), ((13, 17, 19), 2.0), ((31, 33, 62), 3.0),
+                ((65, 65, 65), 1.0), ((129, 129, 129), 1.0)]
+
+
+@pytest.mark.parametrize("T", [torch.float64, torch.float32])
+@pytest.mark.parametrize("grid,gamma", PADDED_CASES)
+def test_solve_any_grid_size_matches_oracle(grid, gamma, T):
+    res = _solve_pair(grid, T, gamma)
+    if T == torch.float64:
+        _check_fp64(f"padded solve f64 {grid} gamma={gamma}", *res)
+        return
+    got, ref = res
+    errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
+    print(f"padded solve f32 {grid} gamma={gamma}: rel L2 {['%.2e' % e for e in errs]}")
+    assert max(errs) <= 1e-5
+
+
 @pytest.mark.parametrize("grid,gamma", [((8, 8, 8), 1.0), ((16, 16, 32), 2.0), ((12, 10, 6), 1.0),
-                                        ((32, 64, 32), 1.5)])
+                                        ((32, 64, 32), 1.5), ((11, 13, 17), 1.0),
+                                        ((9, 10, 65), 2.0)])
 @pytest.mark.parametrize("T", [torch.float64, torch.float32])
 def test_cathode_matches_oracle(grid, gamma, T):
     res = _solve_pair(grid, T, gamma, at_cathode=True, zshift=4e-3)

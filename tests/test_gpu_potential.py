@@ -51,7 +51,9 @@ def _mesh(grid, T, gamma, n=20_000, seed=4, sigma=(1e-3, 1e-3, 2e-3), generic=Fa
 
 
 CASES = [((4, 4, 4), 1.0), ((8, 8, 8), 2.0), ((16, 8, 12), 1.0), ((10, 14, 20), 3.0),
-         ((32, 32, 32), 1.0), ((24, 20, 40), 10.0), ((64, 32, 128), 2.0), ((64, 64, 128), 1.0)]
+         ((32, 32, 32), 1.0), ((24, 20, 40), 10.0), ((64, 32, 128), 2.0), ((64, 64, 128), 1.0),
+         # 2n with a prime factor above 7: transforms padded to a 7-smooth length
+         ((11, 13, 17), 1.0), ((33, 31, 20), 2.0)]
 
 
 @pytest.mark.parametrize("generic", [False, True])
