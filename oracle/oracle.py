@@ -58,6 +58,10 @@ def _load():
     lib.sco_solve_potential_extended.argtypes = [pll, pd, d, vp, vp, i]
     lib.sco_potential_green_function_f64.argtypes = [d, d, d]
     lib.sco_potential_green_function_f64.restype = d
+    lib.sco_workspace_create.argtypes = [i, pll]
+    lib.sco_workspace_create.restype = vp
+    lib.sco_workspace_free.argtypes = [i, vp]
+    lib.sco_solve_ws.argtypes = [vp, i, pd, pd, pd, d, i, vp, vp, i]
     lib.sco_solve_transpose.argtypes = [i, pll, pd, pd, pd, d, i, vp, vp, i]
     lib.sco_solve_transpose_extended.argtypes = [pll, pd, pd, pd, d, i, vp, vp, i]
     _lib = lib
@@ -153,6 +157,42 @@ def solve(rho, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0):
     if rc != 0:
         raise OracleError(f"solve failed ({rc})")
     return ef
+
+
+class Workspace:
+    """The solve workspace of one mesh (src/mesh.jl:49-72): the three padded
+    complex arrays and the transform plans, made once and reused by every
+    solve (the reference caches them on the mesh after the first solve!)."""
+
+    def __init__(self, grid, T=np.float64):
+        self._lib = _load()
+        self.grid = tuple(int(g) for g in grid)
+        self.T = np.dtype(T)
+        self._h = self._lib.sco_workspace_create(_code(self.T), _ll3(self.grid))
+        if not self._h:
+            raise OracleError("workspace allocation failed")
+
+    def solve(self, rho, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0, out=None):
+        """solve() on this workspace: the same arithmetic, no allocation or
+        planning; writes into ``out`` (a Fortran (nx, ny, nz, 3) array of the
+        workspace's type, like mesh.efield) when given."""
+        assert rho.flags.f_contiguous and rho.shape == self.grid and rho.dtype == self.T
+        if out is None:
+            out = np.zeros(rho.shape + (3,), dtype=rho.dtype, order="F")
+        assert out.flags.f_contiguous and out.shape == rho.shape + (3,) and out.dtype == self.T
+        ef = out
+        rc = self._lib.sco_solve_ws(self._h, _code(self.T), _d3(lo), _d3(hi), _d3(delta),
+                                    float(gamma), int(bool(at_cathode)), _ptr(rho), _ptr(ef),
+                                    nthreads)
+        if rc != 0:
+            raise OracleError(f"solve_ws failed ({rc})")
+        return ef
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.sco_workspace_free(_code(self.T), h)
+            self._h = None
 
 
 def solve_extended(rho, lo, hi, delta, gamma=1.0, at_cathode=False, nthreads=0):

@@ -328,6 +328,40 @@ int sco_solve(int mdt, const long long grid[3], const double lo[3], const double
     return -4;
 }
 
+/* A persistent solve workspace (src/mesh.jl:49-72) of Float64 or Float32
+ * type for a grid; NULL on allocation failure. */
+void *sco_workspace_create(int mdt, const long long grid[3])
+{
+    const int g[3] = {(int)grid[0], (int)grid[1], (int)grid[2]};
+    if (mdt == SCO_F64) return ws_create_f64(g);
+    if (mdt == SCO_F32) return ws_create_f32(g);
+    return NULL;
+}
+
+void sco_workspace_free(int mdt, void *ws)
+{
+    if (mdt == SCO_F64) ws_free_f64((workspace_f64 *)ws);
+    else if (mdt == SCO_F32) ws_free_f32((workspace_f32 *)ws);
+}
+
+/* sco_solve on a workspace from sco_workspace_create (same grid and type). */
+int sco_solve_ws(void *ws, int mdt, const double lo[3], const double hi[3], const double delta[3],
+                 double gamma, int at_cathode, const void *rho, void *efield, int nthreads)
+{
+    if (!ws) return -1;
+    int nth = sco_threads(nthreads);
+    if (mdt == SCO_F64)
+        return solve_ws_f64((workspace_f64 *)ws, rho, efield, lo, hi, delta, gamma, at_cathode, nth);
+    if (mdt == SCO_F32) {
+        const float l[3] = {(float)lo[0], (float)lo[1], (float)lo[2]};
+        const float h[3] = {(float)hi[0], (float)hi[1], (float)hi[2]};
+        const float d[3] = {(float)delta[0], (float)delta[1], (float)delta[2]};
+        return solve_ws_f32((workspace_f32 *)ws, rho, efield, l, h, d, (float)gamma, at_cathode,
+                            nth);
+    }
+    return -4;
+}
+
 /* The solve in x87 extended precision: rho and efield cross as Float64
  * arrays, bounds/spacings/gamma as the Float64 mesh values. */
 int sco_solve_extended(const long long grid[3], const double lo[3], const double hi[3],

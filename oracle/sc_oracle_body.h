@@ -130,13 +130,12 @@ static void SFX(fft1d)(const SFX(fftplan) *p, R *x, R *work, int sign)
 }
 
 /* In-place 3-D complex transform of a column-major (d0, d1, d2) array,
- * x fastest (Julia's layout).  Rows of every axis are transformed
- * independently; OpenMP splits rows over threads (deterministic). */
-static void SFX(fft3d)(R *a, const int d[3], int sign, int nthreads)
+ * x fastest (Julia's layout), with the plans pl[0..2] of the three lengths.
+ * Rows of every axis are transformed independently; OpenMP splits rows over
+ * threads (deterministic). */
+static void SFX(fft3d_pl)(R *a, const SFX(fftplan) pl[3], int sign, int nthreads)
 {
-    SFX(fftplan) pl[3];
-    for (int ax = 0; ax < 3; ++ax) SFX(fftplan_init)(&pl[ax], d[ax]);
-    const long long d0 = d[0], d1 = d[1], d2 = d[2];
+    const long long d0 = pl[0].n, d1 = pl[1].n, d2 = pl[2].n;
     const int B = 16; /* pencils gathered together for the strided axes */
     /* axis 0: contiguous rows */
     #pragma omp parallel num_threads(nthreads)
@@ -201,7 +200,23 @@ static void SFX(fft3d)(R *a, const int d[3], int sign, int nthreads)
         free(buf);
         free(work);
     }
+}
+
+/* fft3d_pl with plans made for this one call. */
+static void SFX(fft3d)(R *a, const int d[3], int sign, int nthreads)
+{
+    SFX(fftplan) pl[3];
+    for (int ax = 0; ax < 3; ++ax) SFX(fftplan_init)(&pl[ax], d[ax]);
+    SFX(fft3d_pl)(a, pl, sign, nthreads);
     for (int ax = 0; ax < 3; ++ax) SFX(fftplan_free)(&pl[ax]);
+}
+
+/* One padded transform: with the cached plans of a workspace when pl is
+ * given (src/mesh.jl:60-61 plans once per mesh), else planned per call. */
+static void SFX(fft3d_any)(R *a, const int d[3], const SFX(fftplan) *pl, int sign, int nthreads)
+{
+    if (pl) SFX(fft3d_pl)(a, pl, sign, nthreads);
+    else SFX(fft3d)(a, d, sign, nthreads);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -314,7 +329,8 @@ static void SFX(get_green_function)(R *cgrn, R *temp, const int grid[3],
  * lazily created workspace of src/mesh.jl:49-72. */
 static void SFX(solve_freespace)(const R *rho, R *efield, const int grid[3],
                                  const R delta[3], R gamma, const R offset[3],
-                                 R *crho, R *cgrn, R *temp, int nthreads, int c0, int c1)
+                                 R *crho, R *cgrn, R *temp, const SFX(fftplan) *pl,
+                                 int nthreads, int c0, int c1)
 {
     const long long nx = grid[0], ny = grid[1], nz = grid[2];
     const long long isz = 2 * nx, jsz = 2 * ny, ksz = 2 * nz;
@@ -327,14 +343,14 @@ static void SFX(solve_freespace)(const R *rho, R *efield, const int grid[3],
             for (long long i = 0; i < nx; ++i)
                 crho[2 * (i + isz * (j + jsz * k))] = rho[i + nx * (j + ny * k)];
     /* fft_plan_inplace * crho  (:72) */
-    SFX(fft3d)(crho, dims, -1, nthreads);
+    SFX(fft3d_any)(crho, dims, pl, -1, nthreads);
     /* factr = T(FPEI)  (:75) */
     const R factr = (R)SCO_FPEI;
     /* AbstractFFTs normalisation of plan_ifft!: one(T)/prod(size) */
     const R scale = (R)1 / (R)P;
     for (int icomp = c0; icomp <= c1; ++icomp) {
         SFX(get_green_function)(cgrn, temp, grid, delta, gamma, icomp, offset, nthreads);
-        SFX(fft3d)(cgrn, dims, -1, nthreads);                       /* :89 */
+        SFX(fft3d_any)(cgrn, dims, pl, -1, nthreads);               /* :89 */
         #pragma omp parallel for schedule(static) num_threads(nthreads)
         for (long long e = 0; e < P; ++e) {                          /* :92 */
             R ar = crho[2 * e], ai = crho[2 * e + 1];
@@ -342,7 +358,7 @@ static void SFX(solve_freespace)(const R *rho, R *efield, const int grid[3],
             temp[2 * e] = ar * br - ai * bi;
             temp[2 * e + 1] = ar * bi + ai * br;
         }
-        SFX(fft3d)(temp, dims, +1, nthreads);                       /* :95 */
+        SFX(fft3d_any)(temp, dims, pl, +1, nthreads);               /* :95 */
         #pragma omp parallel for schedule(static) num_threads(nthreads)
         for (long long e = 0; e < P; ++e) {
             temp[2 * e] = temp[2 * e] * scale;
@@ -362,18 +378,15 @@ static void SFX(solve_freespace)(const R *rho, R *efield, const int grid[3],
 /* src/solvers/free_space.jl:14-47 -- solve!(mesh; at_cathode).  The image
  * charge is -rho flipped in z (:34), solved with z-offset
  * 2*zmin + (zmax - zmin) (:39) and added to the free-space field (:45). */
-static int SFX(solve)(const R *rho, R *efield, const int grid[3], const R lo[3],
-                      const R hi[3], const R delta[3], R gamma, int at_cathode,
-                      int nthreads)
+static int SFX(solve_in)(const R *rho, R *efield, const int grid[3], const R lo[3],
+                         const R hi[3], const R delta[3], R gamma, int at_cathode,
+                         R *crho, R *cgrn, R *temp, const SFX(fftplan) *pl, int nthreads)
 {
     const long long nx = grid[0], ny = grid[1], nz = grid[2];
-    const long long P = 8 * nx * ny * nz, M = nx * ny * nz;
-    R *crho = (R *)malloc(sizeof(R) * 2 * (size_t)P);
-    R *cgrn = (R *)malloc(sizeof(R) * 2 * (size_t)P);
-    R *temp = (R *)malloc(sizeof(R) * 2 * (size_t)P);
-    if (!crho || !cgrn || !temp) { free(crho); free(cgrn); free(temp); return -1; }
+    const long long M = nx * ny * nz;
     const R zero3[3] = {0, 0, 0};
-    SFX(solve_freespace)(rho, efield, grid, delta, gamma, zero3, crho, cgrn, temp, nthreads, 1, 3);
+    SFX(solve_freespace)(rho, efield, grid, delta, gamma, zero3, crho, cgrn, temp, pl, nthreads,
+                         1, 3);
     if (at_cathode) {
         R *irho = (R *)malloc(sizeof(R) * (size_t)M);
         R *ief = (R *)malloc(sizeof(R) * 3 * (size_t)M);
@@ -383,15 +396,78 @@ static int SFX(solve)(const R *rho, R *efield, const int grid[3], const R lo[3],
                     irho[i + nx * (j + ny * k)] = -rho[i + nx * (j + ny * (nz - 1 - k))];
         const R offz = (R)2 * lo[2] + (hi[2] - lo[2]);
         const R off[3] = {0, 0, offz};
-        SFX(solve_freespace)(irho, ief, grid, delta, gamma, off, crho, cgrn, temp, nthreads, 1, 3);
+        SFX(solve_freespace)(irho, ief, grid, delta, gamma, off, crho, cgrn, temp, pl, nthreads, 1,
+                             3);
         for (long long e = 0; e < 3 * M; ++e) efield[e] = efield[e] + ief[e];
         free(irho);
         free(ief);
     }
+    return 0;
+}
+
+/* solve! with a workspace allocated (and plans made) for this one call. */
+static int SFX(solve)(const R *rho, R *efield, const int grid[3], const R lo[3],
+                      const R hi[3], const R delta[3], R gamma, int at_cathode,
+                      int nthreads)
+{
+    const long long P = 8 * (long long)grid[0] * grid[1] * grid[2];
+    R *crho = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    R *cgrn = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    R *temp = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    int rc = -1;
+    if (crho && cgrn && temp)
+        rc = SFX(solve_in)(rho, efield, grid, lo, hi, delta, gamma, at_cathode, crho, cgrn, temp,
+                           NULL, nthreads);
     free(crho);
     free(cgrn);
     free(temp);
-    return 0;
+    return rc;
+}
+
+/* The mesh's lazily created workspace, src/mesh.jl:49-72: the three padded
+ * complex arrays (crho, cgrn, temp_result) and the transform plans, made once
+ * and reused by every solve of the mesh. */
+typedef struct {
+    int grid[3];
+    R *crho, *cgrn, *temp;
+    SFX(fftplan) pl[3];
+} SFX(workspace);
+
+static inline void SFX(ws_free)(SFX(workspace) *w)
+{
+    if (!w) return;
+    free(w->crho);
+    free(w->cgrn);
+    free(w->temp);
+    for (int a = 0; a < 3; ++a)
+        if (w->pl[a].tw) SFX(fftplan_free)(&w->pl[a]);
+    free(w);
+}
+
+static inline SFX(workspace) *SFX(ws_create)(const int grid[3])
+{
+    SFX(workspace) *w = (SFX(workspace) *)calloc(1, sizeof(SFX(workspace)));
+    if (!w) return NULL;
+    const long long P = 8 * (long long)grid[0] * grid[1] * grid[2];
+    for (int a = 0; a < 3; ++a) w->grid[a] = grid[a];
+    w->crho = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    w->cgrn = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    w->temp = (R *)malloc(sizeof(R) * 2 * (size_t)P);
+    if (!w->crho || !w->cgrn || !w->temp) { SFX(ws_free)(w); return NULL; }
+    /* touch every page once, as the first solve! does */
+    memset(w->crho, 0, sizeof(R) * 2 * (size_t)P);
+    memset(w->cgrn, 0, sizeof(R) * 2 * (size_t)P);
+    memset(w->temp, 0, sizeof(R) * 2 * (size_t)P);
+    for (int a = 0; a < 3; ++a) SFX(fftplan_init)(&w->pl[a], 2 * grid[a]);
+    return w;
+}
+
+static inline int SFX(solve_ws)(SFX(workspace) *w, const R *rho, R *efield, const R lo[3],
+                                const R hi[3], const R delta[3], R gamma, int at_cathode,
+                                int nthreads)
+{
+    return SFX(solve_in)(rho, efield, w->grid, lo, hi, delta, gamma, at_cathode, w->crho, w->cgrn,
+                         w->temp, w->pl, nthreads);
 }
 
 /* Scalar potential, free space: phi = FPEI * gamma/(dx dy dz_rest) *
@@ -408,7 +484,8 @@ static int SFX(solve_potential)(const R *rho, R *phi, const int grid[3], const R
     R *temp = (R *)malloc(sizeof(R) * 2 * (size_t)P);
     if (!crho || !cgrn || !temp) { free(crho); free(cgrn); free(temp); return -1; }
     const R zero3[3] = {0, 0, 0};
-    SFX(solve_freespace)(rho, phi, grid, delta, gamma, zero3, crho, cgrn, temp, nthreads, 4, 4);
+    SFX(solve_freespace)(rho, phi, grid, delta, gamma, zero3, crho, cgrn, temp, NULL, nthreads, 4,
+                         4);
     free(crho);
     free(cgrn);
     free(temp);
@@ -444,7 +521,7 @@ static int SFX(solve_transpose)(const R *g, R *out, const int grid[3], const R l
     for (long long e = 0; e < M; ++e) out[e] = (R)0;
     for (int c = 0; c < 3; ++c) {
         SFX(solve_freespace)(g + (size_t)c * M, ef, grid, delta, gamma, zero3, crho, cgrn, temp,
-                             nthreads, c + 1, c + 1);
+                             NULL, nthreads, c + 1, c + 1);
         for (long long e = 0; e < M; ++e) out[e] = out[e] - ef[e];
     }
     if (at_cathode) {
@@ -452,7 +529,7 @@ static int SFX(solve_transpose)(const R *g, R *out, const int grid[3], const R l
         for (long long e = 0; e < M; ++e) acc[e] = (R)0;
         for (int c = 0; c < 3; ++c) {
             SFX(solve_freespace)(g + (size_t)c * M, ef, grid, delta, gamma, off, crho, cgrn, temp,
-                                 nthreads, c + 1, c + 1);
+                                 NULL, nthreads, c + 1, c + 1);
             for (long long e = 0; e < M; ++e) acc[e] = acc[e] + ef[e];
         }
         for (long long k = 0; k < nz; ++k)

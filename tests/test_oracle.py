@@ -648,3 +648,24 @@ def test_interpolate_vjp_matches_central_differences():
         dn = sum(e * w for e, w in zip(orc.interpolate(ef, *pos, lo, d), gE))
         fd = (up - dn) / (2 * h)
         assert np.abs(fd - gref).max() <= 1e-7 * np.abs(gref).max(), a
+
+
+@pytest.mark.parametrize("T", [np.float64, np.float32])
+@pytest.mark.parametrize("cathode", [False, True])
+def test_workspace_solve_equals_solve(T, cathode):
+    """The persistent workspace (arrays and plans made once, as the reference
+    caches them on the mesh) gives the allocating solve's bits, twice in a
+    row on one workspace."""
+    grid = (12, 10, 14)
+    x, y, z = gaussian(4000, (1e-3, 1e-3, 2e-3), 77)
+    z = z + (5e-3 if cathode else 0.0)
+    q = np.full(4000, 1e-12)
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid, T)
+    rho = orc.deposit(x.astype(T), y.astype(T), z.astype(T), q.astype(T), grid, lo, dl, T)
+    ref = orc.solve(rho, lo, hi, dl, 2.0, cathode)
+    ws = orc.Workspace(grid, T)
+    out = np.zeros(grid + (3,), dtype=T, order="F")
+    for _ in range(2):
+        got = ws.solve(rho, lo, hi, dl, 2.0, cathode, out=out)
+        assert got is out
+        assert np.array_equal(got, ref)
