@@ -147,6 +147,13 @@ int sc_bounds_from_particles(sc_ctx *ctx, int pdtype, const void *x, const void 
                              int64_t n, const int64_t grid[3], int dtype, double min_bounds[3],
                              double max_bounds[3], double delta[3]);
 
+/* Replaces extrema(particles_x) on a device array (src/mesh.jl:120-122), for
+ * hosts that run the reference's particle-extrema constructor themselves
+ * (the Julia binding's Base.extrema(::MIArray)): *mn, *mx = the smallest and
+ * largest of x[0..n-1], as doubles holding particle-type values.
+ * Synchronises the stream.  Errors: n == 0. */
+int sc_extrema(sc_ctx *ctx, int pdtype, const void *x, int64_t n, double *mn, double *mx);
+
 /* ---- device-resident bounds (no reference counterpart; serve tracking
  * loops that re-mesh every step without a host round trip) ----------------
  * sc_mesh_auto_bounds computes the bounds of Mesh3D(grid, x, y, z; T)

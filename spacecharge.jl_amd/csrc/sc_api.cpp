@@ -438,6 +438,23 @@ int sc_bounds_from_particles(sc_ctx *ctx, int pdtype, const void *x, const void 
     return SC_OK;
 }
 
+int sc_extrema(sc_ctx *ctx, int pdtype, const void *x, int64_t n, double *mn, double *mx)
+{
+    if (!ctx) return set_err(nullptr, SC_ERR_INVALID, "ctx is NULL");
+    Ctx *c = &ctx->c;
+    if (!mn || !mx) return set_err(c, SC_ERR_INVALID, "NULL argument");
+    if (!dtype_ok(pdtype)) return set_err(c, SC_ERR_INVALID, "dtype must be SC_F32 or SC_F64");
+    if (n <= 0) return set_err(c, SC_ERR_INVALID, "collection must be non-empty");
+    if (!x) return set_err(c, SC_ERR_INVALID, "NULL array pointer");
+    if (int rc = check_dev(c)) return rc;
+    double lo[3], hi[3];
+    // the three-axis reduction with the one array on every axis
+    if (int rc = launch_bounds(c, pdtype, x, x, x, n, lo, hi)) return rc;
+    *mn = lo[0];
+    *mx = hi[0];
+    return SC_OK;
+}
+
 // ---- the step ----
 static int check_geom(Ctx *c, const double lo[3], const double dl[3])
 {

@@ -51,6 +51,7 @@ PROTOTYPES = {
     "sc_mesh_get_deposit_tiling": (_i, [_vp, _pi, _pi]),
     "sc_mesh_autotune_deposit": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64, _pd, _pd, _i, _pf]),
     "sc_bounds_from_particles": (_i, [_vp, _i, _vp, _vp, _vp, _i64, _pi64, _i, _pd, _pd, _pd]),
+    "sc_extrema": (_i, [_vp, _i, _vp, _i64, _pd, _pd]),
     "sc_mesh_auto_bounds": (_i, [_vp, _i, _vp, _vp, _vp, _i64]),
     "sc_mesh_read_bounds": (_i, [_vp, _pd, _pd, _pd]),
     "sc_clear_mesh": (_i, [_vp]),

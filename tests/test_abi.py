@@ -41,10 +41,15 @@ def test_library_exports_every_declared_symbol():
         assert name in _lib.PROTOTYPES, f"{name} has no ctypes prototype"
 
 
+JL_PKG = os.path.join(REPO, "spacecharge.jl_amd", "julia", "SpaceChargeMI355")
+JL_SRC = os.path.join(JL_PKG, "src", "SpaceChargeMI355.jl")
+JL_EXT = os.path.join(JL_PKG, "ext", "SpaceChargeMI355ChainRulesCoreExt.jl")
+
+
 def _julia_ccalls():
     """(symbol, number of argument types) of every ccall into the library in
     the Julia binding."""
-    text = open(os.path.join(REPO, "spacecharge.jl_amd", "julia", "SpaceChargeMI355.jl")).read()
+    text = open(JL_SRC).read()
     out = []
     for m in re.finditer(r"ccall\(\(:(sc_\w+), libsc\),\s*[\w{}]+,\s*\(", text):
         depth, i = 1, m.end()
@@ -116,3 +121,104 @@ def test_constants():
     """src/utils.jl:7-8."""
     assert sc.CLIGHT == 299792458.0
     assert sc.FPEI == 299792458.0 ** 2 * 1e-7
+
+
+# ---- the Julia binding (Julia is not installed here: these read its text) ----
+
+def _julia_code(path):
+    """Source without comments, docstrings and string literals."""
+    text = open(path).read()
+    text = re.sub(r'"""[\s\S]*?"""', '""', text)
+    text = re.sub(r"#=[\s\S]*?=#", "", text)
+    text = re.sub(r'"(?:\\.|[^"\\])*"', '""', text)
+    text = re.sub(r"'(?:\\.|[^'\\])'", "''", text)
+    return re.sub(r"#[^\n]*", "", text)
+
+
+def _block_balance(code):
+    """Block openers minus `end`s, both outside brackets: inside them `for`
+    and `if` belong to comprehensions and `end` is an index."""
+    openers = {"function", "macro", "module", "baremodule", "struct", "if", "for", "while",
+               "begin", "let", "do", "try", "quote"}
+    depth_br, bal = 0, 0
+    for tok in re.finditer(r"[A-Za-z_]\w*!?|[\[\](){}]", code):
+        t = tok.group(0)
+        if t in "[({":
+            depth_br += 1
+        elif t in "])}":
+            depth_br -= 1
+        elif t == "end" and depth_br == 0:
+            bal -= 1
+        elif t in openers and depth_br == 0:
+            # `mutable struct` opens one block; `abstract type ... end` does not occur here
+            bal += 1
+    return bal, depth_br
+
+
+@pytest.mark.parametrize("path", [JL_SRC, JL_EXT])
+def test_julia_files_are_balanced(path):
+    """Every block opened is closed and every bracket matched (a syntax sanity
+    check in the absence of a Julia parser)."""
+    code = _julia_code(path)
+    bal, br = _block_balance(code)
+    assert bal == 0, f"{path}: {bal} unclosed blocks"
+    assert br == 0, f"{path}: unbalanced brackets"
+    # the check sees a dropped `end`
+    k = code.rindex("end")
+    assert _block_balance(code[:k] + code[k + 3:])[0] == 1
+
+
+def test_julia_binding_is_a_dropin_for_mesh3d():
+    """The reference's own Mesh3D constructors (src/mesh.jl:95-174, 196-238)
+    build MI355X meshes: get_backend, allocate/zeros, fill! and extrema are
+    defined for the device arrays, the library mesh lives in the mesh's
+    _workspace slot (no global table holding meshes), and the four exported
+    calls have MIArray-mesh methods."""
+    code = _julia_code(JL_SRC)
+    for pat in (r"KA\.get_backend\(::MIArray\)\s*=\s*MI355X\(\)",
+                r"KA\.allocate\(::MI355X,",
+                r"KA\.zeros\(b::MI355X,",
+                r"KA\.synchronize\(::MI355X\)",
+                r"function Base\.extrema\(a::MIArray",
+                r"function Base\.fill!\(a::MIArray",
+                r"struct MI355X <: KA\.GPU end",
+                r"mesh\._workspace = \(mi355 = mh,\)",
+                r"function deposit!\(mesh::MIMesh",
+                r"function solve!\(mesh::MIMesh; at_cathode::Bool=false\)",
+                r"function interpolate_field\(mesh::MIMesh",
+                r"clear_mesh!\(mesh::MIMesh\)"):
+        assert re.search(pat, code), pat
+    assert "IdDict" not in code and "HANDLES" not in code
+    assert "mi355_mesh" not in code
+
+
+def test_julia_chainrules_extension():
+    """The rrules ship as a package extension on ChainRulesCore, one per
+    differentiable function, and the step's passes at_cathode to the solve
+    adjoint."""
+    proj = open(os.path.join(JL_PKG, "Project.toml")).read()
+    assert re.search(r'\[weakdeps\]\s*\nChainRulesCore = "d360d2e6-b24c-11e9-a2a3-2a2ae2dbcce4"',
+                     proj)
+    assert re.search(r'\[extensions\]\s*\nSpaceChargeMI355ChainRulesCoreExt = "ChainRulesCore"',
+                     proj)
+    assert 'SpaceCharge = "886cafed-6ce3-46b8-9147-fe45a2210a4b"' in proj
+    code = _julia_code(JL_EXT)
+    assert re.search(r"^module SpaceChargeMI355ChainRulesCoreExt", code, re.M)
+    for fn in ("space_charge_fields", "deposit_charge", "solve_field", "interpolate_at"):
+        assert re.search(r"function ChainRulesCore\.rrule\(::typeof\(%s\)" % fn, code), fn
+        assert re.search(r"^function %s\(" % fn, _julia_code(JL_SRC), re.M), fn
+    assert code.count("solve_adjoint(mesh, g_ef; at_cathode=at_cathode)") == 2
+    for fn in ("deposit_adjoint", "solve_adjoint", "interpolate_adjoint"):
+        assert re.search(r"^function %s\(" % fn, _julia_code(JL_SRC), re.M), fn
+
+
+def test_integration_julia_example_uses_mesh3d():
+    """The documented Julia usage is the reference's own: Mesh3D on MIArrays."""
+    text = open(os.path.join(REPO, "INTEGRATION.md")).read()
+    i = text.index("## Julia")
+    j = text.index("\n## ", i + 5)
+    julia = text[i:j]
+    assert "mi355_mesh" not in julia
+    assert re.search(r"mesh = Mesh3D\(\(128, 128, 128\), x, y, z; T=Float32\)", julia)
+    assert "backend=MI355X()" in julia
+    assert "rrule" in julia and "space_charge_fields" in julia
