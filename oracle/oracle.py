@@ -18,7 +18,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libsc_oracle.so")
+# $SC_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of the Makefile's sanitize-test target)
+_LIB_PATH = os.environ.get("SC_ORACLE_LIB", os.path.join(_HERE, "libsc_oracle.so"))
 _lib = None
 
 F32, F64 = 4, 8

@@ -339,3 +339,27 @@ def test_device_bounds_equal_oracle_bitwise(pt, mt):
             got = mesh.fetch_bounds()
             want = orc.bounds_auto(cx, cy, cz, grid, mnp)
             assert got == want, (grid, cx.size, got, want)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_extrema_of_a_device_array(dtype):
+    """sc_extrema (the Julia binding's extrema of a device array, which the
+    reference's Mesh3D constructor calls on the particles, src/mesh.jl:120-122)
+    equals numpy's min and max, also for one element and for negative data."""
+    sc = _sc()
+    from spacecharge import _lib
+    import ctypes
+    lib = _lib.load()
+    rng = np.random.default_rng(52)
+    for a in (rng.standard_normal(1_000_003) * 3e-3 - 7.0, np.array([2.5e-3]),
+              -np.abs(rng.standard_normal(4097))):
+        a = a.astype(dtype)
+        t = torch.as_tensor(a, device=DEV)
+        mn, mx = ctypes.c_double(), ctypes.c_double()
+        ctx = sc._ctx(t.device)
+        rc = lib.sc_extrema(ctx, _lib.SC_F32 if dtype == np.float32 else _lib.SC_F64,
+                            t.data_ptr(), t.numel(), ctypes.byref(mn), ctypes.byref(mx))
+        assert rc == 0, lib.sc_last_error(ctx).decode()
+        assert mn.value == float(a.min()) and mx.value == float(a.max())
+    assert lib.sc_extrema(ctx, _lib.SC_F64, t.data_ptr(), 0, ctypes.byref(mn),
+                          ctypes.byref(mx)) != 0
