@@ -29,8 +29,10 @@
 //
 // A particle's eight contributions are computed exactly as in the reference
 // (same operation order, no FMA contraction); only the order of the sums
-// differs, and the sums run in Float64.  Grids whose tile count exceeds
-// kMaxTiles use the one-pass atomic kernel instead.
+// differs, and the sums run in Float64.  Meshes with more tiles than one
+// LDS histogram holds (kRange) bin in ranges of kRange tiles, one per
+// workgroup row of the count and scatter launches; grids whose tile count
+// exceeds kMaxTiles use the one-pass atomic kernel instead.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -45,7 +47,9 @@ namespace {
 constexpr int kTile = 16;            // cells per tile edge
 constexpr int kSub = kTile + 1;      // nodes per sub-grid edge
 constexpr int kSubVol = kSub * kSub * kSub;
-constexpr int kMaxTiles = 4096;      // LDS histogram size
+constexpr int kRange = 4096;         // tiles per LDS histogram (one binning range)
+constexpr int kMaxTiles = 65535;     // tiles of the binned path (a grid axis of the reduction)
+constexpr int64_t kCmatBudget = (int64_t)1 << 22;  // count-matrix entries (slices x tiles)
 constexpr int kBinThreads = 256;
 constexpr int kBatch = 8;            // particles a lane loads before computing any
 // particles per deposit workgroup: 2^chunk_log2, a per-mesh setting
@@ -134,15 +138,16 @@ struct Split {
 // Per-slice tile histogram, written as column w of the [tile][G] matrix.
 // Each lane loads kBatch particles before it computes any, so the loads of
 // a wave overlap.
+// Tiles [r0, r0 + rn) only (rn <= kRange): the workgroup row of this range.
 template <typename PT, typename MT>
 __device__ __forceinline__ void bin_count_body(unsigned w, const PT *__restrict__ px,
                                                const PT *__restrict__ py, const PT *__restrict__ pz,
-                                               Split sp, Geo g, unsigned *__restrict__ cmat)
+                                               Split sp, Geo g, unsigned *__restrict__ cmat, int r0,
+                                               int rn)
 {
     resolve_geo(g);
-    __shared__ unsigned hist[kMaxTiles];
-    const int ntiles = g.tx * g.ty * g.tz;
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads) hist[t] = 0u;
+    __shared__ unsigned hist[kRange];
+    for (int t = threadIdx.x; t < rn; t += kBinThreads) hist[t] = 0u;
     __syncthreads();
     const int64_t first = (int64_t)w * sp.slice;
     const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
@@ -158,21 +163,26 @@ __device__ __forceinline__ void bin_count_body(unsigned w, const PT *__restrict_
         }
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            const bool ok = base + s * kBinThreads + threadIdx.x < end;
-            atomicAdd(&hist[tile_of<PT, MT>(x[s], y[s], z[s], g)], ok ? 1u : 0u);
+            const unsigned lt = (unsigned)(tile_of<PT, MT>(x[s], y[s], z[s], g) - r0);
+            const bool ok = base + s * kBinThreads + threadIdx.x < end && lt < (unsigned)rn;
+            atomicAdd(&hist[ok ? lt : 0u], ok ? 1u : 0u);
         }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
-        cmat[(size_t)t * sp.G + w] = hist[t];
+    for (int t = threadIdx.x; t < rn; t += kBinThreads)
+        cmat[(size_t)(r0 + t) * sp.G + w] = hist[t];
 }
 
+// Grid (slices, tile ranges).
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
             Split sp, Geo g, unsigned *__restrict__ cmat)
 {
-    bin_count_body<PT, MT>(blockIdx.x, px, py, pz, sp, g, cmat);
+    const int ntiles = g.tx * g.ty * g.tz;
+    const int r0 = (int)blockIdx.y * kRange;
+    bin_count_body<PT, MT>(blockIdx.x, px, py, pz, sp, g, cmat, r0,
+                           ntiles - r0 < kRange ? ntiles - r0 : kRange);
 }
 
 // One wave per tile: exclusive prefix of the tile's row over the slices (in
@@ -219,6 +229,8 @@ k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restri
 
 // One workgroup.  Per-tile totals -> tile start (exclusive offsets), each
 // tile's first chunk, the chunk table; clears the extrapolation flags.
+// Tiles go in blocks of 256 * 16, each block's prefix offset by the totals
+// of the blocks before it.
 // (Folding this into k_bin_rowscan as "last workgroup done" costs more on
 // MI355X than the launch it saves: the device-scope fences write back L2.)
 __device__ __forceinline__ void bin_scan_body(const unsigned *__restrict__ counts,
@@ -230,65 +242,80 @@ __device__ __forceinline__ void bin_scan_body(const unsigned *__restrict__ count
                                               int chunk_log2)
 {
     const unsigned chunk = 1u << chunk_log2;
-    constexpr int kPer = kMaxTiles / 256;
+    constexpr int kPer = 16;  // at most this many tiles per lane per block
     __shared__ unsigned s_w[3][4];
+    __shared__ unsigned s_carry[3];  // particles, chunks, further chunks before this block
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // thread i owns the `per` consecutive tiles [i per, i per + per)
-    const int per = (ntiles + 255) / 256;
-    const int t0 = threadIdx.x * per;
-    unsigned c[kPer];
-    unsigned a = 0u, b = 0u, x = 0u;  // particles, chunks, chunks after each tile's first
+    if (threadIdx.x < 3) s_carry[threadIdx.x] = 0u;
+    // one block of 256 per tiles when ntiles <= 256 kPer, spread over all lanes
+    const int per = (ntiles + 255) / 256 < kPer ? (ntiles + 255) / 256 : kPer;
+    for (int blk = 0; blk < ntiles; blk += 256 * per) {
+        // thread i owns the per consecutive tiles [blk + i per, blk + i per + per)
+        const int t0 = blk + (int)threadIdx.x * per;
+        unsigned c[kPer];
+        unsigned a = 0u, b = 0u, x = 0u;  // particles, chunks, chunks after each tile's first
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        c[i] = (i < per && t0 + i < ntiles) ? counts[t0 + i] : 0u;
-        const unsigned nk = (c[i] + chunk - 1) >> chunk_log2;
-        a += c[i];
-        b += nk;
-        x += nk ? nk - 1 : 0u;
-    }
-    unsigned ia = a, ib = b, ix = x;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const unsigned oa = __shfl_up(ia, d, 64), ob = __shfl_up(ib, d, 64), ox = __shfl_up(ix, d, 64);
-        if (lane >= d) {
-            ia += oa;
-            ib += ob;
-            ix += ox;
-        }
-    }
-    if (lane == 63) {
-        s_w[0][wv] = ia;
-        s_w[1][wv] = ib;
-        s_w[2][wv] = ix;
-    }
-    __syncthreads();
-    unsigned oa = ia - a, ob = ib - b, ox = ix - x;
-    for (int k = 0; k < wv; ++k) {
-        oa += s_w[0][k];
-        ob += s_w[1][k];
-        ox += s_w[2][k];
-    }
-    if (threadIdx.x == 255) *nchunks = ob + b;
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int t = t0 + i;
-        if (i < per && t < ntiles) {
-            ext_flags[t] = 0u;
-            cursor[t] = oa;
-            // sub-grid slots: the tile's first chunk uses slot t (which then
-            // holds the tile's sum), its further chunks ntiles + ox, ...
-            extra_base[t] = (unsigned)ntiles + ox;
+        for (int i = 0; i < kPer; ++i) {
+            c[i] = (i < per && t0 + i < ntiles) ? counts[t0 + i] : 0u;
             const unsigned nk = (c[i] + chunk - 1) >> chunk_log2;
-            for (unsigned k = 0; k < nk; ++k) {
-                const unsigned len = (c[i] - k * chunk) < chunk ? (c[i] - k * chunk) : chunk;
-                const unsigned slot = k == 0 ? (unsigned)t : (unsigned)ntiles + ox + k - 1;
-                chunks[ob + k] = make_int4(t, (int)(oa + k * chunk), (int)len, (int)slot);
-            }
-            oa += c[i];
-            ob += nk;
-            ox += nk ? nk - 1 : 0u;
+            a += c[i];
+            b += nk;
+            x += nk ? nk - 1 : 0u;
         }
+        unsigned ia = a, ib = b, ix = x;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned oa = __shfl_up(ia, d, 64), ob = __shfl_up(ib, d, 64),
+                           ox = __shfl_up(ix, d, 64);
+            if (lane >= d) {
+                ia += oa;
+                ib += ob;
+                ix += ox;
+            }
+        }
+        if (lane == 63) {
+            s_w[0][wv] = ia;
+            s_w[1][wv] = ib;
+            s_w[2][wv] = ix;
+        }
+        __syncthreads();
+        unsigned oa = s_carry[0] + ia - a, ob = s_carry[1] + ib - b, ox = s_carry[2] + ix - x;
+        for (int k = 0; k < wv; ++k) {
+            oa += s_w[0][k];
+            ob += s_w[1][k];
+            ox += s_w[2][k];
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int t = t0 + i;
+            if (i < per && t < ntiles) {
+                ext_flags[t] = 0u;
+                cursor[t] = oa;
+                // sub-grid slots: the tile's first chunk uses slot t (which then
+                // holds the tile's sum), its further chunks ntiles + ox, ...
+                extra_base[t] = (unsigned)ntiles + ox;
+                const unsigned nk = (c[i] + chunk - 1) >> chunk_log2;
+                for (unsigned k = 0; k < nk; ++k) {
+                    const unsigned len = (c[i] - k * chunk) < chunk ? (c[i] - k * chunk) : chunk;
+                    const unsigned slot = k == 0 ? (unsigned)t : (unsigned)ntiles + ox + k - 1;
+                    chunks[ob + k] = make_int4(t, (int)(oa + k * chunk), (int)len, (int)slot);
+                }
+                oa += c[i];
+                ob += nk;
+                ox += nk ? nk - 1 : 0u;
+            }
+        }
+        __syncthreads();  // every lane has read s_w and the carry
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < 4; ++k) {
+                s_carry[0] += s_w[0][k];
+                s_carry[1] += s_w[1][k];
+                s_carry[2] += s_w[2][k];
+            }
+        }
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *nchunks = s_carry[1];
 }
 
 __global__ void __launch_bounds__(256)
@@ -316,14 +343,14 @@ __device__ __forceinline__ void bin_scatter_body(
     const PT *__restrict__ pq, Split sp, Geo g, const unsigned *__restrict__ cmat,
     const unsigned *__restrict__ start, Rec<PT> *__restrict__ sorted,
     unsigned *__restrict__ ext_flags, unsigned long long *__restrict__ slice_qmax,
-    const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s, PT *__restrict__ q2s)
+    const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s, PT *__restrict__ q2s,
+    int r0, int rn)
 {
     resolve_geo(g);
-    __shared__ unsigned cur[kMaxTiles];
-    __shared__ unsigned char ext_lds[kMaxTiles];
-    const int ntiles = g.tx * g.ty * g.tz;
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads) {
-        cur[t] = start[t] + cmat[(size_t)t * sp.G + w];
+    __shared__ unsigned cur[kRange];
+    __shared__ unsigned char ext_lds[kRange];
+    for (int t = threadIdx.x; t < rn; t += kBinThreads) {
+        cur[t] = start[r0 + t] + cmat[(size_t)(r0 + t) * sp.G + w];
         ext_lds[t] = 0;
     }
     __syncthreads();
@@ -350,27 +377,35 @@ __device__ __forceinline__ void bin_scatter_body(
         // tiles first, then the slot atomics back to back, then the flags
         // and stores (an LDS store between the atomics would order them)
         int tl[kBatch];
-        bool ex[kBatch], ok[kBatch];
+        bool ex[kBatch], ok[kBatch], in[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
             ok[s] = base + s * kBinThreads + threadIdx.x < end;
-            tl[s] = tile_of<PT, MT>(r[s].x, r[s].y, r[s].z, g, ex[s]);
+            const unsigned lt = (unsigned)(tile_of<PT, MT>(r[s].x, r[s].y, r[s].z, g, ex[s]) - r0);
+            in[s] = ok[s] && lt < (unsigned)rn;  // this workgroup row's tile range
+            tl[s] = in[s] ? (int)lt : 0;
         }
-        // no branch around the atomics (a lane past the slice end adds 0 to
-        // the valid tile of its clamped particle)
+        // no branch around the atomics (a lane past the slice end or outside
+        // the range adds 0)
 #pragma unroll
-        for (int s = 0; s < kBatch; ++s) slot[s] = atomicAdd(&cur[tl[s]], ok[s] ? 1u : 0u);
+        for (int s = 0; s < kBatch; ++s) slot[s] = atomicAdd(&cur[tl[s]], in[s] ? 1u : 0u);
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
             if (ok[s]) {
-                if (ex[s]) ext_lds[tl[s]] = 1;  // same value from every writer
+                // every range's workgroup takes max|q| over the whole slice,
+                // so their plain stores below agree
                 const double aq = fabs((double)r[s].q);
                 qm = (aq > qm || aq != aq) ? aq : qm;  // NaN sticks
-                sorted[slot[s]] = r[s];
                 if constexpr (NCH == 3) {
                     const double a1 = fabs((double)c1[s]), a2 = fabs((double)c2[s]);
                     qm1 = (a1 > qm1 || a1 != a1) ? a1 : qm1;
                     qm2 = (a2 > qm2 || a2 != a2) ? a2 : qm2;
+                }
+            }
+            if (in[s]) {
+                if (ex[s]) ext_lds[tl[s]] = 1;  // same value from every writer
+                sorted[slot[s]] = r[s];
+                if constexpr (NCH == 3) {
                     q1s[slot[s]] = c1[s];
                     q2s[slot[s]] = c2[s];
                 }
@@ -398,8 +433,8 @@ __device__ __forceinline__ void bin_scatter_body(
         for (int k = 1; k < kBinThreads / 64; ++k) m = wmax[c][k] > m ? wmax[c][k] : m;
         slice_qmax[(size_t)c * kBinMaxSlices + w] = m;
     }
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
-        if (ext_lds[t]) atomicOr(&ext_flags[t], 1u);  // rare: particles outside the mesh
+    for (int t = threadIdx.x; t < rn; t += kBinThreads)
+        if (ext_lds[t]) atomicOr(&ext_flags[r0 + t], 1u);  // rare: particles outside the mesh
 }
 
 template <typename PT, typename MT, int NCH>
@@ -411,8 +446,12 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
               const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s,
               PT *__restrict__ q2s)
 {
+    // grid (slices, tile ranges)
+    const int ntiles = g.tx * g.ty * g.tz;
+    const int r0 = (int)blockIdx.y * kRange;
     bin_scatter_body<PT, MT, NCH>(blockIdx.x, px, py, pz, pq, sp, g, cmat, start, sorted,
-                                  ext_flags, slice_qmax, q1, q2, q1s, q2s);
+                                  ext_flags, slice_qmax, q1, q2, q1s, q2s, r0,
+                                  ntiles - r0 < kRange ? ntiles - r0 : kRange);
 }
 
 // The particle order of the binning as a permutation: perm[slot] = particle
@@ -424,10 +463,13 @@ k_bin_perm(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__res
            int *__restrict__ perm)
 {
     resolve_geo(g);
-    __shared__ unsigned cur[kMaxTiles];
+    __shared__ unsigned cur[kRange];
+    // grid (slices, tile ranges), as the scatter
     const int ntiles = g.tx * g.ty * g.tz;
-    for (int t = threadIdx.x; t < ntiles; t += kBinThreads)
-        cur[t] = start[t] + cmat[(size_t)t * sp.G + blockIdx.x];
+    const int r0 = (int)blockIdx.y * kRange;
+    const int rn = ntiles - r0 < kRange ? ntiles - r0 : kRange;
+    for (int t = threadIdx.x; t < rn; t += kBinThreads)
+        cur[t] = start[r0 + t] + cmat[(size_t)(r0 + t) * sp.G + blockIdx.x];
     __syncthreads();
     const int64_t first = (int64_t)blockIdx.x * sp.slice;
     const int64_t end = first + sp.slice < sp.n ? first + sp.slice : sp.n;
@@ -439,7 +481,9 @@ k_bin_perm(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__res
             int64_t p = base + s * kBinThreads + threadIdx.x;
             ok[s] = p < end;
             p = ok[s] ? p : end - 1;
-            tl[s] = tile_of<PT, MT>(px[p], py[p], pz[p], g);
+            const unsigned lt = (unsigned)(tile_of<PT, MT>(px[p], py[p], pz[p], g) - r0);
+            ok[s] = ok[s] && lt < (unsigned)rn;
+            tl[s] = ok[s] ? (int)lt : 0;
         }
         unsigned slot[kBatch];
 #pragma unroll
@@ -608,18 +652,19 @@ k_deposit_tiles(const Rec<PT> *__restrict__ sorted, const PT *__restrict__ q1s,
 }
 
 // Completes the per-tile sums in sub-grid slots [0, ntiles): adds a tile's
-// further chunks (slots extra_base[t], ...) to slot t, or zeroes slot t of
-// an empty tile.  Grid (ceil(kSubVol / 256), tiles, channels), one lane per
+// further chunks (slots extra_base[t], ...) to slot t.  An empty tile's
+// slot is zeroed, unless skip_empty (meshes of more than one binning range,
+// where most tiles are empty): the gather then skips empty tiles.  Grid (ceil(kSubVol / 256), tiles, channels), one lane per
 // node; channel c's sub-grids start c * pstride doubles in.
 __device__ __forceinline__ void tile_reduce_body(unsigned bx, int t, int chan,
                                                  double *__restrict__ partials, size_t pstride,
                                                  const unsigned *__restrict__ counts,
                                                  const unsigned *__restrict__ extra_base,
-                                                 int chunk_log2)
+                                                 int chunk_log2, int skip_empty)
 {
     partials += chan * pstride;
     const unsigned nk = (counts[t] + (1u << chunk_log2) - 1) >> chunk_log2;
-    if (nk == 1) return;  // slot t already holds the sum
+    if (nk == 1 || (nk == 0 && skip_empty)) return;  // slot t holds the sum, or stays unset
     const int local = (int)bx * blockDim.x + threadIdx.x;
     if (local >= kSubVol) return;
     double *dst = partials + (size_t)t * kSubVol + local;
@@ -645,11 +690,11 @@ __device__ __forceinline__ void tile_reduce_body(unsigned bx, int t, int chan,
 
 __global__ void __launch_bounds__(256)
 k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__restrict__ counts,
-              const unsigned *__restrict__ extra_base, int chunk_log2)
+              const unsigned *__restrict__ extra_base, int chunk_log2, int skip_empty)
 {
     __builtin_amdgcn_s_setprio(kChainPrio);
     tile_reduce_body(blockIdx.x, blockIdx.y, blockIdx.z, partials, pstride, counts, extra_base,
-                     chunk_log2);
+                     chunk_log2, skip_empty);
 }
 
 // Node-parallel sum over the (up to 8) tiles whose sub-grid holds the node:
@@ -659,9 +704,13 @@ k_tile_reduce(double *__restrict__ partials, size_t pstride, const unsigned *__r
 // accumulate the sum is added to the existing rho (deposit!(...; clear=false)).
 // Grid (ceil(M / 256), channels): channel c reads sub-grid set c (pstride
 // doubles apart) and writes rho_c.
+// With skip_empty, empty tiles' slots are unset (tile_reduce_body) and are
+// not added.
 template <typename MT>
 __device__ __forceinline__ void gather_tiles_body(unsigned bx, const double *__restrict__ partials,
-                                                  const Geo &g, MT *__restrict__ rho, int accumulate)
+                                                  const unsigned *__restrict__ counts, const Geo &g,
+                                                  MT *__restrict__ rho, int accumulate,
+                                                  int skip_empty)
 {
     // at most kMaxTiles tiles of 16^3 cells: node indices fit in 32 bits
     const int M = g.nx * g.ny * g.nz;
@@ -683,35 +732,50 @@ __device__ __forceinline__ void gather_tiles_body(unsigned bx, const double *__r
             for (int a = 0; a < nxc; ++a) {
                 const int t = cx[a] + g.tx * (cy[b] + g.ty * cz[c]);
                 const int local = (gx - kTile * cx[a]) + kSub * ((gy - kTile * cy[b]) + kSub * (gz - kTile * cz[c]));
-                s += partials[(size_t)t * kSubVol + local];
+                const double v = partials[(size_t)t * kSubVol + local];
+                if (skip_empty) {
+                    // read beside the count (no dependent load), not added when empty
+                    s += counts[t] != 0u ? v : 0.0;
+                } else {
+                    s += v;
+                }
             }
     rho[i] = accumulate ? (MT)((double)rho[i] + s) : (MT)s;
 }
 
 template <typename MT>
 __global__ void __launch_bounds__(256)
-k_gather_tiles(const double *__restrict__ partials, size_t pstride, Geo g, MT *__restrict__ rho0,
-               MT *__restrict__ rho1, MT *__restrict__ rho2, int accumulate)
+k_gather_tiles(const double *__restrict__ partials, size_t pstride,
+               const unsigned *__restrict__ counts, Geo g, MT *__restrict__ rho0,
+               MT *__restrict__ rho1, MT *__restrict__ rho2, int accumulate, int skip_empty)
 {
     __builtin_amdgcn_s_setprio(kChainPrio);
     const int chan = blockIdx.y;
-    gather_tiles_body<MT>(blockIdx.x, partials + chan * pstride, g,
-                          chan == 0 ? rho0 : (chan == 1 ? rho1 : rho2), accumulate);
+    gather_tiles_body<MT>(blockIdx.x, partials + chan * pstride, counts, g,
+                          chan == 0 ? rho0 : (chan == 1 ? rho1 : rho2), accumulate, skip_empty);
 }
 
-// At most kBinMaxSlices slices of about 2^slice_log2 particles.
-Split split_of(int64_t n, int slice_log2)
+// At most kBinMaxSlices slices of about 2^slice_log2 particles, and at
+// most kCmatBudget / ntiles (the count matrix has a row per tile).
+Split split_of(int64_t n, int slice_log2, int ntiles)
 {
     Split sp;
     sp.n = n;
     const int64_t per = (int64_t)1 << slice_log2;
     int64_t G = (n + per - 1) / per;
     if (G > kBinMaxSlices) G = kBinMaxSlices;
+    const int64_t gmax = kCmatBudget / (ntiles > 0 ? ntiles : 1);
+    if (G > gmax) G = gmax;
     if (G < 1) G = 1;
     sp.G = (int)G;
     sp.slice = (n + G - 1) / G;
     return sp;
 }
+
+// Entries of each per-tile block of the counts array (totals, tile start,
+// first extra slot, extrapolation flags): kRange for meshes of at most one
+// binning range (the layout the grouped launches assume), else the tiles.
+int counts_stride(int ntiles) { return ntiles > kRange ? ntiles : kRange; }
 
 int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles, int nchan)
 {
@@ -749,10 +813,11 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles, int nchan)
     if (!B.counts) {
         // totals | tile start | first extra sub-grid slot | extrapolation flags | nchunks, pad |
         // per-slice max|q| of up to three channels (u64 bit patterns)
-        const size_t bytes = sizeof(unsigned) * (4 * kMaxTiles + 4 + 3 * 2 * kBinMaxSlices);
+        const size_t bytes =
+            sizeof(unsigned) * (4 * (size_t)counts_stride(ntiles) + 4 + 3 * 2 * kBinMaxSlices);
         if (hipMalloc((void **)&B.counts, bytes) != hipSuccess) return fail();
     }
-    const int64_t need_cmat = (int64_t)ntiles * split_of(n, m->slice_log2).G;
+    const int64_t need_cmat = (int64_t)ntiles * split_of(n, m->slice_log2, ntiles).G;
     if (B.cmat_cap < need_cmat) {
         if (B.cmat) (void)hipFree(B.cmat);
         B.cmat = nullptr;
@@ -772,15 +837,17 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
     hipStream_t st = ctx->stream;
     DepositBins &B = m->bins;
     const int ntiles = g.tx * g.ty * g.tz;
-    unsigned *counts = B.counts, *cursor = B.counts + kMaxTiles;
-    unsigned *cbase = B.counts + 2 * kMaxTiles, *ext = B.counts + 3 * kMaxTiles;
-    unsigned *nch = B.counts + 4 * kMaxTiles;
-    auto *slice_qmax = reinterpret_cast<unsigned long long *>(B.counts + 4 * kMaxTiles + 4);
+    const int cs = counts_stride(ntiles);
+    const unsigned ranges = (unsigned)((ntiles + kRange - 1) / kRange);
+    unsigned *counts = B.counts, *cursor = B.counts + cs;
+    unsigned *cbase = B.counts + 2 * cs, *ext = B.counts + 3 * cs;
+    unsigned *nch = B.counts + 4 * cs;
+    auto *slice_qmax = reinterpret_cast<unsigned long long *>(B.counts + 4 * cs + 4);
     int4 *chunks = static_cast<int4 *>(B.chunks);
-    const Split sp = split_of(n, m->slice_log2);
+    const Split sp = split_of(n, m->slice_log2, ntiles);
     launch_begin(ctx, K_BIN_COUNT);
-    hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
-                       (const PT *)y, (const PT *)z, sp, g, B.cmat);
+    hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G, ranges), dim3(kBinThreads), 0, st,
+                       (const PT *)x, (const PT *)y, (const PT *)z, sp, g, B.cmat);
     launch_end(ctx, K_BIN_COUNT);
     launch_begin(ctx, K_BIN_ROWSCAN);
     hipLaunchKernelGGL(k_bin_rowscan, dim3((unsigned)div_up4(ntiles)), dim3(256), 0, st, B.cmat,
@@ -794,12 +861,12 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
     PT *q1s = static_cast<PT *>(B.qs), *q2s = q1s ? q1s + n : nullptr;
     launch_begin(ctx, K_BIN_SCATTER);
     if (nchan == 3)
-        hipLaunchKernelGGL((k_bin_scatter<PT, MT, 3>), dim3(sp.G), dim3(kBinThreads), 0, st,
+        hipLaunchKernelGGL((k_bin_scatter<PT, MT, 3>), dim3(sp.G, ranges), dim3(kBinThreads), 0, st,
                            (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q[0], sp, g,
                            (const unsigned *)B.cmat, (const unsigned *)cursor, sorted, ext,
                            slice_qmax, (const PT *)q[1], (const PT *)q[2], q1s, q2s);
     else
-        hipLaunchKernelGGL((k_bin_scatter<PT, MT, 1>), dim3(sp.G), dim3(kBinThreads), 0, st,
+        hipLaunchKernelGGL((k_bin_scatter<PT, MT, 1>), dim3(sp.G, ranges), dim3(kBinThreads), 0, st,
                            (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)q[0], sp, g,
                            (const unsigned *)B.cmat, (const unsigned *)cursor, sorted, ext,
                            slice_qmax, (const PT *)nullptr, (const PT *)nullptr, (PT *)nullptr,
@@ -819,14 +886,16 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
     launch_begin(ctx, K_TILE_REDUCE);
     hipLaunchKernelGGL(k_tile_reduce, dim3((unsigned)((kSubVol + 255) / 256), (unsigned)ntiles, nchan),
                        dim3(256), 0, st, static_cast<double *>(B.partials), pstride,
-                       (const unsigned *)counts, (const unsigned *)cbase, m->chunk_log2);
+                       (const unsigned *)counts, (const unsigned *)cbase, m->chunk_log2,
+                       ranges > 1 ? 1 : 0);
     launch_end(ctx, K_TILE_REDUCE);
     launch_begin(ctx, K_GATHER_TILES);
     MT *o0 = static_cast<MT *>(out[0]);
     MT *o1 = nchan > 1 ? static_cast<MT *>(out[1]) : o0;
     MT *o2 = nchan > 2 ? static_cast<MT *>(out[2]) : o0;
     hipLaunchKernelGGL((k_gather_tiles<MT>), dim3((unsigned)((M + 255) / 256), nchan), dim3(256), 0,
-                       st, (const double *)B.partials, pstride, g, o0, o1, o2, clear ? 0 : 1);
+                       st, (const double *)B.partials, pstride, (const unsigned *)counts, g, o0, o1,
+                       o2, clear ? 0 : 1, ranges > 1 ? 1 : 0);
     launch_end(ctx, K_GATHER_TILES);
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
@@ -856,14 +925,16 @@ int perm_binned(Mesh *m, const void *x, const void *y, const void *z, int64_t n,
     hipStream_t st = ctx->stream;
     DepositBins &B = m->bins;
     const int ntiles = g.tx * g.ty * g.tz;
-    unsigned *counts = B.counts, *cursor = B.counts + kMaxTiles;
-    unsigned *cbase = B.counts + 2 * kMaxTiles, *ext = B.counts + 3 * kMaxTiles;
-    unsigned *nch = B.counts + 4 * kMaxTiles;
+    const int cs = counts_stride(ntiles);
+    const unsigned ranges = (unsigned)((ntiles + kRange - 1) / kRange);
+    unsigned *counts = B.counts, *cursor = B.counts + cs;
+    unsigned *cbase = B.counts + 2 * cs, *ext = B.counts + 3 * cs;
+    unsigned *nch = B.counts + 4 * cs;
     int4 *chunks = static_cast<int4 *>(B.chunks);
-    const Split sp = split_of(n, m->slice_log2);
+    const Split sp = split_of(n, m->slice_log2, ntiles);
     launch_begin(ctx, K_BIN_COUNT);
-    hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
-                       (const PT *)y, (const PT *)z, sp, g, B.cmat);
+    hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G, ranges), dim3(kBinThreads), 0, st,
+                       (const PT *)x, (const PT *)y, (const PT *)z, sp, g, B.cmat);
     launch_end(ctx, K_BIN_COUNT);
     launch_begin(ctx, K_BIN_ROWSCAN);
     hipLaunchKernelGGL(k_bin_rowscan, dim3((unsigned)div_up4(ntiles)), dim3(256), 0, st, B.cmat,
@@ -874,7 +945,7 @@ int perm_binned(Mesh *m, const void *x, const void *y, const void *z, int64_t n,
                        cbase, chunks, nch, ext, ntiles, m->chunk_log2);
     launch_end(ctx, K_BIN_SCAN);
     launch_begin(ctx, K_BIN_SCATTER);
-    hipLaunchKernelGGL((k_bin_perm<PT, MT>), dim3(sp.G), dim3(kBinThreads), 0, st, (const PT *)x,
+    hipLaunchKernelGGL((k_bin_perm<PT, MT>), dim3(sp.G, ranges), dim3(kBinThreads), 0, st, (const PT *)x,
                        (const PT *)y, (const PT *)z, sp, g, (const unsigned *)B.cmat,
                        (const unsigned *)cursor, perm);
     launch_end(ctx, K_BIN_SCATTER);
@@ -1049,7 +1120,8 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_count_g(const GTable *__res
 {
     SC_GROUP_PROLOGUE(kGCount)
     (void)cnt;
-    bin_count_body<PT, MT>(lb, (const PT *)d.x, (const PT *)d.y, (const PT *)d.z, d.sp, d.g, d.cmat);
+    bin_count_body<PT, MT>(lb, (const PT *)d.x, (const PT *)d.y, (const PT *)d.z, d.sp, d.g, d.cmat,
+                           0, d.g.tx * d.g.ty * d.g.tz);
 }
 
 __global__ void __launch_bounds__(256) k_bin_rowscan_g(const GTable *__restrict__ T)
@@ -1062,8 +1134,8 @@ __global__ void __launch_bounds__(256) k_bin_scan_g(const GTable *__restrict__ T
 {
     SC_GROUP_PROLOGUE(kGScan)
     (void)lb;
-    bin_scan_body(cnt, cnt + kMaxTiles, cnt + 2 * kMaxTiles, d.chunks, cnt + 4 * kMaxTiles,
-                  cnt + 3 * kMaxTiles, d.g.tx * d.g.ty * d.g.tz, d.chunk_log2);
+    bin_scan_body(cnt, cnt + kRange, cnt + 2 * kRange, d.chunks, cnt + 4 * kRange,
+                  cnt + 3 * kRange, d.g.tx * d.g.ty * d.g.tz, d.chunk_log2);
 }
 
 template <typename PT, typename MT>
@@ -1071,10 +1143,10 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_scatter_g(const GTable *__r
 {
     SC_GROUP_PROLOGUE(kGScatter)
     bin_scatter_body<PT, MT, 1>(lb, (const PT *)d.x, (const PT *)d.y, (const PT *)d.z,
-                                (const PT *)d.q, d.sp, d.g, d.cmat, cnt + kMaxTiles,
-                                static_cast<Rec<PT> *>(d.sorted), cnt + 3 * kMaxTiles,
-                                reinterpret_cast<unsigned long long *>(cnt + 4 * kMaxTiles + 4),
-                                nullptr, nullptr, nullptr, nullptr);
+                                (const PT *)d.q, d.sp, d.g, d.cmat, cnt + kRange,
+                                static_cast<Rec<PT> *>(d.sorted), cnt + 3 * kRange,
+                                reinterpret_cast<unsigned long long *>(cnt + 4 * kRange + 4),
+                                nullptr, nullptr, nullptr, nullptr, 0, d.g.tx * d.g.ty * d.g.tz);
 }
 
 template <typename PT, typename MT>
@@ -1082,9 +1154,9 @@ __global__ void __launch_bounds__(kDepThreads) k_deposit_tiles_g(const GTable *_
 {
     SC_GROUP_PROLOGUE(kGDeposit)
     deposit_tiles_body<PT, MT>(lb, 0, static_cast<const Rec<PT> *>(d.sorted), nullptr, nullptr,
-                               d.chunks, cnt + 4 * kMaxTiles, d.g, d.partials, 0,
-                               cnt + 3 * kMaxTiles,
-                               reinterpret_cast<const unsigned long long *>(cnt + 4 * kMaxTiles + 4),
+                               d.chunks, cnt + 4 * kRange, d.g, d.partials, 0,
+                               cnt + 3 * kRange,
+                               reinterpret_cast<const unsigned long long *>(cnt + 4 * kRange + 4),
                                d.sp.G, d.chunk_log2);
 }
 
@@ -1092,15 +1164,14 @@ __global__ void __launch_bounds__(256) k_tile_reduce_g(const GTable *__restrict_
 {
     SC_GROUP_PROLOGUE(kGReduce)
     tile_reduce_body(lb % kReduceBlocks, (int)(lb / kReduceBlocks), 0, d.partials, 0, cnt,
-                     cnt + 2 * kMaxTiles, d.chunk_log2);
+                     cnt + 2 * kRange, d.chunk_log2, 0);
 }
 
 template <typename MT>
 __global__ void __launch_bounds__(256) k_gather_tiles_g(const GTable *__restrict__ T)
 {
     SC_GROUP_PROLOGUE(kGGather)
-    (void)cnt;
-    gather_tiles_body<MT>(lb, d.partials, d.g, static_cast<MT *>(d.rho), T->accumulate);
+    gather_tiles_body<MT>(lb, d.partials, cnt, d.g, static_cast<MT *>(d.rho), T->accumulate, 0);
 }
 
 #undef SC_GROUP_PROLOGUE
@@ -1154,7 +1225,7 @@ int ensure_group_table(Ctx *ctx, size_t bytes)
 size_t group_table_bytes();
 
 // Qualifies: at most kGroupMax bunches, one mesh type, every bunch with
-// particles (n in [1, 2^31)) and at most kMaxTiles tiles.
+// particles (n in [1, 2^31)) and at most one binning range (kRange tiles).
 int launch_deposit_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const void *const *x,
                            const void *const *y, const void *const *z, const void *const *q,
                            const int64_t *n, const double *lo, const double *dl, int clear)
@@ -1172,7 +1243,7 @@ int launch_deposit_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const voi
         g.tx = (g.nx - 1 + kTile - 1) / kTile;
         g.ty = (g.ny - 1 + kTile - 1) / kTile;
         g.tz = (g.nz - 1 + kTile - 1) / kTile;
-        if ((int64_t)g.tx * g.ty * g.tz > kMaxTiles) return SC_ERR_UNSUPPORTED;
+        if ((int64_t)g.tx * g.ty * g.tz > kRange) return SC_ERR_UNSUPPORTED;
         g.dev = nullptr;
         for (int a = 0; a < 3; ++a) {
             g.lo[a] = lo[3 * b + a];
@@ -1212,7 +1283,7 @@ int launch_deposit_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const voi
             d.y = y[b0 + k];
             d.z = z[b0 + k];
             d.q = q[b0 + k];
-            d.sp = split_of(n[b0 + k], m->slice_log2);
+            d.sp = split_of(n[b0 + k], m->slice_log2, ntiles);
             d.g = g;
             d.counts = B.counts;
             d.cmat = B.cmat;

@@ -242,10 +242,13 @@ def test_cathode_matches_oracle(grid, gamma, T):
     assert max(errs) <= 1e-5
 
 
-@pytest.mark.parametrize("grid", [(20, 33, 17), (1025, 1025, 18)])
+@pytest.mark.parametrize("grid", [(20, 33, 17), (1025, 1025, 18), (300, 300, 300),
+                                  (4097, 4097, 3)])
 def test_deposit_clamped_and_fallback(grid):
-    """Out-of-mesh particles follow the clamp policy; a grid with more than
-    4096 tiles of 16^3 cells takes the one-pass atomic kernel."""
+    """Out-of-mesh particles follow the clamp policy.  (1025, 1025, 18) and
+    300^3 have 8192 and 6859 tiles of 16^3 cells, more than one LDS
+    histogram holds: the binning runs in two ranges.  (4097, 4097, 3) has
+    65536 tiles, above the binned path's limit: the one-pass atomic kernel."""
     sc = _sc()
     rng = np.random.default_rng(11)
     n = 50_000
@@ -315,3 +318,68 @@ def test_deposit_dense_core():
         err = np.abs(got - ref).max() / np.abs(ref).max()
         print(f"dense core {T}: max rel err vs Float64 {err:.3e}")
         assert err <= (1e-12 if T == torch.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("pt", [torch.float64, torch.float32])
+def test_deposit_512_cubed_float32_binned_and_reproducible(pt):
+    """512^3 Float32 mesh (32768 tiles, eight binning ranges), 2M particles.
+    Two deposits are bit for bit equal (the Float32 mesh accumulates in fixed
+    point, so the result does not depend on atomic order).  Float64
+    particles: the cell arithmetic runs in Float64, and the result is within
+    1e-5 of the Float64 oracle.  Float32 particles: the reference's cell
+    arithmetic is Float32, whose normalised coordinate has an ulp of 6e-5 of
+    a cell at 512 cells, so the yardstick is the oracle in that arithmetic
+    (the Float64 distance is printed)."""
+    sc = _sc()
+    grid = (512, 512, 512)
+    rng = np.random.default_rng(21)
+    n = 2_000_000
+    npt = np.float64 if pt == torch.float64 else np.float32
+    x = rng.normal(0.0, 1e-3, n).astype(npt)
+    y = rng.normal(0.0, 1.5e-3, n).astype(npt)
+    z = rng.normal(0.0, 3e-3, n).astype(npt)
+    q = np.full(n, 1e-9 / n, dtype=npt)
+    xt, yt, zt, qt = _tensors((x, y, z, q), pt)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=torch.float32)
+    with sc.KernelTimer(torch.device(DEV)) as kt:
+        sc.deposit(mesh, xt, yt, zt, qt)
+        torch.cuda.synchronize()
+    used = kt.results()
+    assert "deposit_tiles" in used and "deposit" not in used, used
+    first = mesh.rho.clone()
+    sc.deposit(mesh, xt, yt, zt, qt)
+    torch.cuda.synchronize()
+    assert torch.equal(mesh.rho, first)
+    got = mesh_to_np(mesh._rho, grid).astype(np.float64)
+    xs, ys, zs, qs = (a.astype(np.float64) for a in (x, y, z, q))
+    ref64 = orc.deposit(xs, ys, zs, qs, grid, mesh.min_bounds, mesh.delta)
+    e64 = np.abs(got - ref64).max() / np.abs(ref64).max()
+    if pt == torch.float64:
+        print(f"512^3 Float32 mesh, Float64 particles: max rel err vs Float64 oracle {e64:.3e}")
+        assert e64 <= 1e-5
+    else:
+        ref32 = orc.deposit(x, y, z, q, grid, mesh.min_bounds, mesh.delta, np.float32)
+        e32 = np.abs(got - ref32).max() / np.abs(ref32).max()
+        print(f"512^3 Float32 mesh, Float32 particles: max rel err vs Float32 oracle {e32:.3e}, "
+              f"vs Float64 oracle {e64:.3e}")
+        assert e32 <= 1e-5
+    assert abs(got.sum() - qs.sum()) <= 1e-6 * qs.sum()
+
+
+def test_sort_permutation_over_binning_ranges():
+    """The tile order of a 300^3 mesh (two binning ranges): a permutation,
+    and the particles it orders have non-decreasing tiles."""
+    sc = _sc()
+    grid = (300, 300, 300)
+    x, y, z = gaussian(400_000, (1e-3, 1e-3, 2e-3), 22)
+    xt, yt, zt = _tensors((x, y, z), torch.float64)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=torch.float64)
+    perm = sc.sort_permutation(mesh, xt, yt, zt).cpu().numpy()
+    assert np.array_equal(np.sort(perm), np.arange(x.size))
+    lo, dl = mesh.min_bounds, mesh.delta
+    cells = [np.clip(np.floor((a - lo[k]) / dl[k]), 0, grid[k] - 2).astype(np.int64)
+             for k, a in enumerate((x, y, z))]
+    nt = [(g - 1 + 15) // 16 for g in grid]
+    tile = cells[0] // 16 + nt[0] * (cells[1] // 16 + nt[1] * (cells[2] // 16))
+    assert np.all(np.diff(tile[perm]) >= 0)
+    assert tile.max() >= 4096  # the particles reach the second range
