@@ -173,7 +173,9 @@ __device__ __forceinline__ void bin_count_body(unsigned w, const PT *__restrict_
         cmat[(size_t)(r0 + t) * sp.G + w] = hist[t];
 }
 
-// Grid (slices, tile ranges).
+// Grid (slices, tile ranges); the slices go to workgroups in XCD order
+// (xcd_task), so that the workgroups of one XCD own neighbouring columns of
+// the count matrix and share its lines in their L2.
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
@@ -181,7 +183,7 @@ k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__re
 {
     const int ntiles = g.tx * g.ty * g.tz;
     const int r0 = (int)blockIdx.y * kRange;
-    bin_count_body<PT, MT>(blockIdx.x, px, py, pz, sp, g, cmat, r0,
+    bin_count_body<PT, MT>(xcd_task((int)blockIdx.x, (int)gridDim.x), px, py, pz, sp, g, cmat, r0,
                            ntiles - r0 < kRange ? ntiles - r0 : kRange);
 }
 
@@ -446,10 +448,11 @@ k_bin_scatter(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__
               const PT *__restrict__ q1, const PT *__restrict__ q2, PT *__restrict__ q1s,
               PT *__restrict__ q2s)
 {
-    // grid (slices, tile ranges)
+    // grid (slices, tile ranges), slices in XCD order as in k_bin_count
     const int ntiles = g.tx * g.ty * g.tz;
     const int r0 = (int)blockIdx.y * kRange;
-    bin_scatter_body<PT, MT, NCH>(blockIdx.x, px, py, pz, pq, sp, g, cmat, start, sorted,
+    bin_scatter_body<PT, MT, NCH>(xcd_task((int)blockIdx.x, (int)gridDim.x), px, py, pz, pq, sp,
+                                  g, cmat, start, sorted,
                                   ext_flags, slice_qmax, q1, q2, q1s, q2s, r0,
                                   ntiles - r0 < kRange ? ntiles - r0 : kRange);
 }

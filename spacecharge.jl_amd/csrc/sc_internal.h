@@ -301,6 +301,19 @@ int ensure_workspace(Mesh *m);
 int ensure_side_stream(Mesh *m);
 void free_workspace(Mesh *m);
 
+// Task of workgroup `id` in a one-dimensional launch of `total` workgroups.
+// The dispatcher hands consecutive workgroups to the eight XCDs in turn, and
+// each XCD has an L2 of its own.  This order gives XCD x one contiguous range
+// of tasks, so the workgroups resident together on one XCD take neighbouring
+// tasks and find the lines they share in that XCD's L2.
+__device__ __forceinline__ int xcd_task(int id, int total)
+{
+    constexpr int kXcd = 8;
+    const int q = total / kXcd, r = total % kXcd;
+    const int x = id % kXcd, local = id / kXcd;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + local;
+}
+
 }  // namespace sc
 
 struct sc_ctx {

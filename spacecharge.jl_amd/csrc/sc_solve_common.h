@@ -138,6 +138,18 @@ __device__ __forceinline__ int group_block(const SBunch *tab, int per, const SBu
     return (int)blockIdx.x - k * per;
 }
 
+// Pencil (x block, ky) of z-convolution task `task`, with nbx x blocks per ky
+// row.  The rows go in the order 0, 1, Ly-1, 2, Ly-2, ..., ny, because rows
+// ky and Ly - ky read the same Green row (the spectrum is even in ky), and
+// neighbouring x blocks share the 128-byte lines of that row.
+__device__ __forceinline__ void zconv_pencils(int task, int nbx, int Ly, int &bx, int &ky)
+{
+    bx = task % nbx;
+    const int u = task / nbx;
+    const int kyp = (u + 1) >> 1;
+    ky = u == 0 ? 0 : ((u & 1) ? kyp : Ly - kyp);
+}
+
 // Builds the transformed Green functions for green_mode `mode` into ws.Gf
 // (and ws.Gi for kFieldCathode): three field components, or the
 // potential's one (stored in the first component's slot).

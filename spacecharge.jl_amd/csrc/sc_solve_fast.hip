@@ -125,8 +125,16 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
           const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d,
           const SBunch *__restrict__ tab, int per)
 {
-    const SBunch *sb;
-    const int bx = group_block(tab, per, sb);
+    // grouped launches: grid (per * bunches, Ly); one bunch: a 1-D grid of
+    // per * Ly workgroups in XCD order, per x blocks per row
+    const SBunch *sb = nullptr;
+    int bx, ky;
+    if (tab) {
+        bx = group_block(tab, per, sb);
+        ky = (int)blockIdx.y;
+    } else {
+        zconv_pencils(xcd_task((int)blockIdx.x, (int)gridDim.x), per, d.Ly, bx, ky);
+    }
     if (sb) {
         A = static_cast<const cpx<T> *>(sb->fwd);
         B = static_cast<cpx<T> *>(sb->inv);
@@ -142,7 +150,7 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
     const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
     for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    const int kx = bx * NB + b, ky = blockIdx.y;
+    const int kx = bx * NB + b;
     const bool live = kx < d.Kx;
     const size_t zs = (size_t)Ly * Kp;
     const cpx<T> *Ap = A + (size_t)ky * Kp + kx;
@@ -215,7 +223,10 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
     const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
     for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    const int kx = blockIdx.x * NB + b, ky = blockIdx.y;
+    // a 1-D grid of div_up(Kx, NB) * Ly workgroups in XCD order
+    int bx, ky;
+    zconv_pencils(xcd_task((int)blockIdx.x, (int)gridDim.x), (d.Kx + NB - 1) / NB, Ly, bx, ky);
+    const int kx = bx * NB + b;
     const bool live = kx < d.Kx;
     const size_t zs = (size_t)Ly * Kp;
     const bool kyneg = ky > ny;
@@ -380,18 +391,19 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
         constexpr int ZT = zconv_threads<T, L>();
         using SZ = Shape<T, L, ZT>;
         launch_begin(ctx, K_ZCONV);
-        const dim3 zgrid(div_up(d.Kx, SZ::NB), d.Ly);
+        const int nbx = div_up(d.Kx, SZ::NB);
+        const dim3 zgrid(nbx * d.Ly);
         const cpx<T> *Fc = F;
         const T *Gf = static_cast<const T *>(ws.Gf);
         const cpx<T> *Gi = static_cast<const cpx<T> *>(ws.Gi);
         switch (io.mode) {
         case kFieldCathode:
             hipLaunchKernelGGL((k_zconv_r<T, L, kFieldCathode, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf,
-                               Gi, tw, d, (const SBunch *)nullptr, 0);
+                               Gi, tw, d, (const SBunch *)nullptr, nbx);
             break;
         case kPotential:
             hipLaunchKernelGGL((k_zconv_r<T, L, kPotential, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
-                               tw, d, (const SBunch *)nullptr, 0);
+                               tw, d, (const SBunch *)nullptr, nbx);
             break;
         case kAdjointFree:
             hipLaunchKernelGGL((k_zconv_adj_r<T, L, false, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
@@ -403,7 +415,7 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
             break;
         default:
             hipLaunchKernelGGL((k_zconv_r<T, L, kFieldFree, ZT>), zgrid, dim3(ZT), 0, st, Fc, B, Gf, Gi,
-                               tw, d, (const SBunch *)nullptr, 0);
+                               tw, d, (const SBunch *)nullptr, nbx);
             break;
         }
         launch_end(ctx, K_ZCONV);
