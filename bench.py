@@ -594,6 +594,27 @@ def run_leg(args, sc, torch, dev, T):
     elapsed = time.perf_counter() - t0
     ms = elapsed * 1e3 / args.steps
 
+    # the same step captured once into a HIP graph and replayed: the same
+    # launches and work (the Green build still forks onto the library's
+    # second stream), without the host's launch path between the kernels
+    E_eager = step()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        E_graph = step()
+    graph.replay()
+    torch.cuda.synchronize()
+    # bitwise equal on Float32 meshes; Float64 deposits add in LDS atomics
+    # whose order varies, so there the last bits may differ
+    graph_diff = max(float((a.double() - b.double()).abs().max() / b.double().abs().max())
+                     for a, b in zip(E_graph, E_eager))
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    ms_graph = (time.perf_counter() - t2) * 1e3 / args.steps
+    del graph, E_graph, E_eager
+
     # instrumented pass: per-kernel HIP events on the launching stream
     with sc.KernelTimer(dev) as kt:
         t1 = time.perf_counter()
@@ -636,7 +657,8 @@ def run_leg(args, sc, torch, dev, T):
     }
     return {
         "n": n, "grid": grid, "dtype": dtype_name, "mesh": mesh, "parts": (x, y, z, q),
-        "E_last": E_last, "ms": ms, "ms_instr": ms_instr, "roofline": roofline,
+        "E_last": E_last, "ms": ms, "ms_instr": ms_instr, "ms_graph": ms_graph,
+        "graph_diff": graph_diff, "roofline": roofline,
         "step_roofline": step_roofline, "breakdown": breakdown,
         "share": {k: round(v[0] / tot, 3) for k, v in per_kernel.items()}, "tuning": tuning,
     }
@@ -738,6 +760,8 @@ def main():
             "cpu_baseline": o_cpu,
             "kernel_us_per_launch": other["breakdown"],
             "ms_per_step_instrumented": other["ms_instr"],
+            "graph_ms_per_step": other["ms_graph"],
+            "graph_max_rel_diff_vs_eager": other["graph_diff"],
         }
 
     out = {
@@ -776,6 +800,9 @@ def main():
         "kernel_us_per_launch": head["breakdown"],
         "kernel_share": head["share"],
         "ms_per_step_instrumented": head["ms_instr"],
+        "graph_ms_per_step": head["ms_graph"],
+        "graph_particles_per_s": n / (head["ms_graph"] * 1e-3),
+        "graph_max_rel_diff_vs_eager": head["graph_diff"],
     }
     print(json.dumps(out))
 

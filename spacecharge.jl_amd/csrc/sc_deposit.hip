@@ -1258,13 +1258,8 @@ int launch_deposit_grouped(Ctx *ctx, Mesh *const *ms, int nb, int pdt, const voi
         if (int rc = ensure_bins(ms[b], n[b], pdt, geos[b].tx * geos[b].ty * geos[b].tz, 1)) return rc;
     // where each mesh's deposit starts on the stream (a later solve's Green
     // build on the side stream waits for it, as after launch_deposit)
-    for (int b = 0; b < nb; ++b) {
-        Workspace &ws = ms[b]->ws;
-        if (ws.deposit_start) {
-            SC_HIP(ctx, hipEventRecord(ws.deposit_start, ctx->stream));
-            ws.deposit_start_recorded = true;
-        }
-    }
+    for (int b = 0; b < nb; ++b)
+        if (int rc = mark_deposit_start(ms[b])) return rc;
     GTable *T = static_cast<GTable *>(ctx->gtable);
     // descriptors and block starts, stored kGroupStore at a time
     unsigned total[kGPhases] = {};

@@ -139,14 +139,19 @@ struct Workspace {
     FftPlan plan[3];
     // ordering of the Green build on the side stream: green_done after the
     // build (the z convolution waits for it), gf_free after the z
-    // convolution (the next build waits for it)
+    // convolution (the next eager build waits for it; recorded by eager
+    // calls only)
     hipEvent_t green_done = nullptr, gf_free = nullptr;
     bool gf_free_recorded = false;
-    // recorded where each deposit into this mesh starts: the next build
-    // waits for it too, so that it overlaps the deposit kernels (latency
-    // bound) rather than the previous step's interpolation
+    // recorded where each deposit into this mesh starts (mark_deposit_start):
+    // the next build forks from it, so that it overlaps the deposit kernels
+    // (latency bound) rather than the previous step's interpolation.
+    // fork_ready: recorded by an eager deposit that no build has forked from
+    // yet; fork_capture: the id of the stream capture whose deposit recorded
+    // it (0: none).  A build with neither forks from the end of the stream.
     hipEvent_t deposit_start = nullptr;
-    bool deposit_start_recorded = false;
+    bool fork_ready = false;
+    unsigned long long fork_capture = 0;
     // cached Green key for the opt-in reuse
     bool g_valid = false;
     double g_key[6] = {};
@@ -299,6 +304,11 @@ int solve_adjoint_mesh(Mesh *m, const double lo[3], const double hi[3], const do
                        double gamma, int at_cathode, const void *g_efield, void *g_rho);
 int ensure_workspace(Mesh *m);
 int ensure_side_stream(Mesh *m);
+// true while a stream capture is active on st (its id in *id, 0 otherwise)
+bool stream_capturing(hipStream_t st, unsigned long long *id);
+// Records where a deposit into m starts on the context stream, for the next
+// solve's Green build on the side stream (no-op before the first solve).
+int mark_deposit_start(Mesh *m);
 void free_workspace(Mesh *m);
 
 // Task of workgroup `id` in a one-dimensional launch of `total` workgroups.

@@ -585,10 +585,7 @@ int launch_deposit(Mesh *m, int pdt, const void *x, const void *y, const void *z
 {
     // marks where this deposit starts on the stream; a following solve's
     // Green build (side stream) waits for it, see run_solve
-    if (m->ws.deposit_start) {
-        SC_HIP(m->ctx, hipEventRecord(m->ws.deposit_start, m->ctx->stream));
-        m->ws.deposit_start_recorded = true;
-    }
+    if (int rc = mark_deposit_start(m)) return rc;
     return launch_deposit_to(m, pdt, x, y, z, q, n, lo, dl, clear, m->rho);
 }
 

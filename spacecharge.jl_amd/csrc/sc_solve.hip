@@ -377,22 +377,30 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
                        std::memcmp(key, ws.g_key, sizeof(key)) == 0;
     // The Green spectra depend on the spacing, gamma and the cathode offset
     // only, not on rho: built on the context's side stream they overlap the
-    // deposit kernels still running on `st`.  Under stream capture the
-    // build stays on `st`.
+    // deposit kernels still running on `st`.  Under stream capture the side
+    // stream joins the capture through the fork event and rejoins `st`
+    // before the z convolution, so the graph keeps the overlap; it must
+    // exist already (no stream is created while capturing).
+    unsigned long long cid = 0;
+    const bool cap = stream_capturing(st, &cid);
     bool overlap = false;
     if (!reuse) {
-        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-        (void)hipStreamIsCapturing(st, &cs);
-        overlap = cs == hipStreamCaptureStatusNone && !ctx->in_batch && ensure_side_stream(m) == SC_OK;
+        overlap = !ctx->in_batch &&
+                  (cap ? ctx->side && ws.green_done && ws.deposit_start : ensure_side_stream(m) == SC_OK);
         if (overlap) {
-            // the build starts where this step's deposit started; the first
-            // time (no deposit marked yet) it follows everything on `st`,
-            // which may still read this mesh's Green spectra
-            if (!ws.deposit_start_recorded) {
-                SC_HIP(ctx, hipEventRecord(ws.deposit_start, st));
-                ws.deposit_start_recorded = true;
-            }
-            if (ws.gf_free_recorded) SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.gf_free, 0));
+            // the build forks where this step's deposit started (recorded by
+            // a deposit of this capture, or by an eager deposit not yet
+            // forked from), else at the current end of `st`, which may still
+            // read this mesh's Green spectra
+            const bool from_deposit = cap ? ws.fork_capture == cid : ws.fork_ready;
+            if (!from_deposit) SC_HIP(ctx, hipEventRecord(ws.deposit_start, st));
+            ws.fork_ready = false;
+            ws.fork_capture = 0;
+            // an eager build also waits for the last eager z convolution,
+            // which may have run on another stream (a graph is ordered by its
+            // launch stream instead)
+            if (!cap && ws.gf_free_recorded)
+                SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.gf_free, 0));
             SC_HIP(ctx, hipStreamWaitEvent(ctx->side, ws.deposit_start, 0));
             ctx->stream = ctx->side;  // the build launches (and times) on ctx->stream
             const int rc = build_green<T>(m, dl, gamma, offz, gmode);
@@ -449,7 +457,7 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
                                static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, mode);
         launch_end(ctx, K_ZCONV);
     }
-    if (ws.green_done) {  // the side stream exists: the next build waits for this point
+    if (ws.green_done && !cap) {  // the side stream exists: the next build waits for this point
         SC_HIP(ctx, hipEventRecord(ws.gf_free, st));
         ws.gf_free_recorded = true;
     }
@@ -507,6 +515,29 @@ bool make_plan(int n, FftPlan *p)
         }
     }
     return true;
+}
+
+bool stream_capturing(hipStream_t st, unsigned long long *id)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    unsigned long long cid = 0;
+    if (hipStreamGetCaptureInfo(st, &cs, &cid) != hipSuccess) cs = hipStreamCaptureStatusNone;
+    const bool active = cs == hipStreamCaptureStatusActive;
+    *id = active ? cid : 0ull;
+    return active;
+}
+
+int mark_deposit_start(Mesh *m)
+{
+    Workspace &ws = m->ws;
+    if (!ws.deposit_start) return SC_OK;
+    hipStream_t st = m->ctx->stream;
+    unsigned long long cid = 0;
+    const bool cap = stream_capturing(st, &cid);
+    SC_HIP(m->ctx, hipEventRecord(ws.deposit_start, st));
+    ws.fork_ready = !cap;
+    ws.fork_capture = cid;
+    return SC_OK;
 }
 
 int ensure_side_stream(Mesh *m)
@@ -653,6 +684,7 @@ int solve_class_t(Ctx *ctx, Mesh *const *ms, int k, const double *const *lo, con
                   const double *const *dl, const double *gamma, int cath, SBunch *tab)
 {
     hipStream_t st = ctx->stream;
+    unsigned long long cid = 0;
     const int mode = cath ? kFieldCathode : kFieldFree;
     for (int i = 0; i < k; ++i)
         if (int rc = ensure_workspace(ms[i])) return rc;
@@ -695,7 +727,7 @@ int solve_class_t(Ctx *ctx, Mesh *const *ms, int k, const double *const *lo, con
         launch_begin(ctx, kid[which]);
         launch_fast_grouped<T>(ms[0], k, tab, which, mode, 1, 3, st);
         launch_end(ctx, kid[which]);
-        if (which == 2) {
+        if (which == 2 && !stream_capturing(st, &cid)) {
             // a later single solve's side-stream Green build waits for this
             for (int i = 0; i < k; ++i) {
                 Workspace &ws = ms[i]->ws;
