@@ -125,12 +125,17 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
           const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, Dims d,
           const SBunch *__restrict__ tab, int per)
 {
-    // grouped launches: grid (per * bunches, Ly); one bunch: a 1-D grid of
-    // per * Ly workgroups in XCD order, per x blocks per row
+    // grouped launches and the cathode mode: grid (x blocks [* bunches], Ly);
+    // one bunch in free space or for the potential: a 1-D grid of per * Ly
+    // workgroups in XCD order, per x blocks per row (the cathode mode, at two
+    // waves per SIMD, measured 2 us slower in that order in Float64)
     const SBunch *sb = nullptr;
     int bx, ky;
     if (tab) {
         bx = group_block(tab, per, sb);
+        ky = (int)blockIdx.y;
+    } else if (MODE == kFieldCathode) {
+        bx = (int)blockIdx.x;
         ky = (int)blockIdx.y;
     } else {
         zconv_pencils(xcd_task((int)blockIdx.x, (int)gridDim.x), per, d.Ly, bx, ky);
@@ -223,9 +228,15 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
     const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
     for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
-    // a 1-D grid of div_up(Kx, NB) * Ly workgroups in XCD order
+    // free space: a 1-D grid of div_up(Kx, NB) * Ly workgroups in XCD order;
+    // with the cathode image a grid (div_up(Kx, NB), Ly), as in k_zconv_r
     int bx, ky;
-    zconv_pencils(xcd_task((int)blockIdx.x, (int)gridDim.x), (d.Kx + NB - 1) / NB, Ly, bx, ky);
+    if (CATH) {
+        bx = (int)blockIdx.x;
+        ky = (int)blockIdx.y;
+    } else {
+        zconv_pencils(xcd_task((int)blockIdx.x, (int)gridDim.x), (d.Kx + NB - 1) / NB, Ly, bx, ky);
+    }
     const int kx = bx * NB + b;
     const bool live = kx < d.Kx;
     const size_t zs = (size_t)Ly * Kp;
@@ -392,7 +403,8 @@ void launch_all(Mesh *m, const Dims &d, int axis_kernel, const SolveIO &io)
         using SZ = Shape<T, L, ZT>;
         launch_begin(ctx, K_ZCONV);
         const int nbx = div_up(d.Kx, SZ::NB);
-        const dim3 zgrid(nbx * d.Ly);
+        const bool cath_mode = io.mode == kFieldCathode || io.mode == kAdjointCathode;
+        const dim3 zgrid = cath_mode ? dim3(nbx, d.Ly) : dim3(nbx * d.Ly);
         const cpx<T> *Fc = F;
         const T *Gf = static_cast<const T *>(ws.Gf);
         const cpx<T> *Gi = static_cast<const cpx<T> *>(ws.Gi);
