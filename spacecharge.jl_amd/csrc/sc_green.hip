@@ -551,14 +551,26 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     constexpr int E = S::E, TPP = S::TPP, NB = S::NB;
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * reg::padded<L, E>()];
-    const SBunch *sb;
-    const int bx = group_block(tab, per, sb);
+    // one bunch: workgroups in XCD order over the whole grid (pencils of
+    // plane k + 1 read the F rows of plane k's pencils a few workgroups
+    // earlier, which then sit on the same XCD and its L2)
+    const SBunch *sb = nullptr;
+    int bx, q;
+    if (tab) {
+        bx = group_block(tab, per, sb);
+        q = (int)blockIdx.y;
+    } else {
+        const int task = xcd_task((int)(blockIdx.x + blockIdx.y * gridDim.x),
+                                  (int)(gridDim.x * gridDim.y));
+        bx = task % (int)gridDim.x;
+        q = task / (int)gridDim.x;
+    }
     if (sb) {
         F = sb->F;
         R = static_cast<T *>(sb->R);
     }
     for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
-    const int q = blockIdx.y, c = comp_of(a, q), image = q / 3;
+    const int c = comp_of(a, q), image = q / 3;
     const int nx = a.d.nx, ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int bpairs = (ny + 1) / 2;
     const int npencils = z_cells(a, image) * bpairs;
