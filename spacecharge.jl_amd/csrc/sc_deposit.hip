@@ -175,22 +175,44 @@ __device__ __forceinline__ void bin_count_body(unsigned w, const PT *__restrict_
 
 // Grid (slices, tile ranges); the slices go to workgroups in XCD order
 // (xcd_task), so that the workgroups of one XCD own neighbouring columns of
-// the count matrix and share its lines in their L2.
+// the count matrix and share its lines in their L2.  A non-null `resv` (the
+// row scan's reservation counter, k_bin_rowscan) is zeroed here, one launch
+// ahead of its first use.
 template <typename PT, typename MT>
 __global__ void __launch_bounds__(kBinThreads)
 k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__restrict__ pz,
-            Split sp, Geo g, unsigned *__restrict__ cmat)
+            Split sp, Geo g, unsigned *__restrict__ cmat, unsigned long long *__restrict__ resv)
 {
     const int ntiles = g.tx * g.ty * g.tz;
     const int r0 = (int)blockIdx.y * kRange;
+    if (resv && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *resv = 0ull;
     bin_count_body<PT, MT>(xcd_task((int)blockIdx.x, (int)gridDim.x), px, py, pz, sp, g, cmat, r0,
                            ntiles - r0 < kRange ? ntiles - r0 : kRange);
 }
 
+// What the row scan also does for a mesh of one binning range (then
+// k_bin_scan is not launched): each tile's wave reserves the tile's run of
+// records and its chunk table entries with one 64-bit atomic on `resv`
+// (records in the low half, chunks in the high half; zeroed by k_bin_count)
+// and writes the tile's start (`cursor`), first extra sub-grid slot, chunk
+// entries and cleared extrapolation flag.  The runs then lie in the order
+// the tiles reserved them; each chunk is the same records (the tile's run
+// from k 2^chunk_log2) and the reduction adds a tile's chunks in the same
+// order, so the sums are those of the tile-ordered layout.  Sub-grid slots:
+// a tile's first chunk uses slot t, its further chunks ntiles + c + k for
+// chunk entries c .. c + nk - 1 (slots below 2 ntiles + chunks).
+struct RowscanChunks {
+    unsigned long long *resv;  // null: k_bin_scan does this
+    unsigned *cursor, *extra_base, *ext_flags;
+    int4 *chunks;
+    int chunk_log2;
+};
+
 // One wave per tile: exclusive prefix of the tile's row over the slices (in
 // place), and the tile's total.
 __device__ __forceinline__ void bin_rowscan_body(unsigned bx, unsigned *__restrict__ cmat, int G,
-                                                 int ntiles, unsigned *__restrict__ tot)
+                                                 int ntiles, unsigned *__restrict__ tot,
+                                                 RowscanChunks rc = RowscanChunks{})
 {
     constexpr int kPer = kBinMaxSlices / 64;
     const int lane = threadIdx.x & 63;
@@ -220,13 +242,34 @@ __device__ __forceinline__ void bin_rowscan_body(unsigned bx, unsigned *__restri
         run += v[i];
     }
     if (lane == 63) tot[t] = incl;
+    if (!rc.resv) return;
+    // the tile's total is lane 63's inclusive prefix
+    const unsigned total = __shfl(incl, 63, 64);
+    const unsigned chunk = 1u << rc.chunk_log2;
+    const unsigned nk = (total + chunk - 1) >> rc.chunk_log2;
+    unsigned long long base = 0ull;
+    if (lane == 0 && total)
+        base = atomicAdd(rc.resv, ((unsigned long long)nk << 32) | (unsigned long long)total);
+    base = __shfl(base, 0, 64);
+    const unsigned r = (unsigned)base, c = (unsigned)(base >> 32);
+    if (lane == 0) {
+        rc.cursor[t] = r;
+        rc.extra_base[t] = (unsigned)ntiles + c + 1u;
+        rc.ext_flags[t] = 0u;
+    }
+    for (unsigned k = (unsigned)lane; k < nk; k += 64u) {
+        const unsigned len = total - k * chunk < chunk ? total - k * chunk : chunk;
+        const unsigned slot = k == 0 ? (unsigned)t : (unsigned)ntiles + c + k;
+        rc.chunks[c + k] = make_int4(t, (int)(r + k * chunk), (int)len, (int)slot);
+    }
 }
 
 __global__ void __launch_bounds__(256)
-k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restrict__ tot)
+k_bin_rowscan(unsigned *__restrict__ cmat, int G, int ntiles, unsigned *__restrict__ tot,
+              RowscanChunks rc)
 {
     __builtin_amdgcn_s_setprio(kChainPrio);
-    bin_rowscan_body(blockIdx.x, cmat, G, ntiles, tot);
+    bin_rowscan_body(blockIdx.x, cmat, G, ntiles, tot, rc);
 }
 
 // One workgroup.  Per-tile totals -> tile start (exclusive offsets), each
@@ -796,7 +839,9 @@ int ensure_bins(Mesh *m, int64_t n, int pdt, int ntiles, int nchan)
         if (hipMalloc(&B.sorted, need) != hipSuccess) return fail();
         B.sorted_bytes = need;
     }
-    const int64_t maxchunks = ntiles + n / chunk + 1;
+    // chunks: at most ntiles + n / chunk; sub-grid slots: below 2 ntiles + chunks
+    // (slots of the row scan's reservation order, k_bin_rowscan)
+    const int64_t maxchunks = 2 * (int64_t)ntiles + n / chunk + 1;
     if (B.chunk_cap < maxchunks || B.partial_sets < nchan) {
         const int64_t cap = B.chunk_cap > maxchunks ? B.chunk_cap : maxchunks;
         const int sets = B.partial_sets > nchan ? B.partial_sets : nchan;
@@ -848,18 +893,31 @@ int binned(Mesh *m, const void *x, const void *y, const void *z, const void *con
     auto *slice_qmax = reinterpret_cast<unsigned long long *>(B.counts + 4 * cs + 4);
     int4 *chunks = static_cast<int4 *>(B.chunks);
     const Split sp = split_of(n, m->slice_log2, ntiles);
+    // one binning range: the row scan reserves the tiles' runs and chunks
+    // (records and chunks counted in the two halves of one 64-bit word after
+    // nchunks; the chunk count is its high half) and the tile scan is not
+    // launched
+    const bool by_rowscan = ranges == 1 && n < ((int64_t)1 << 31);
+    auto *resv = reinterpret_cast<unsigned long long *>(B.counts + 4 * cs + 2);
+    if (by_rowscan) nch = B.counts + 4 * cs + 3;
     launch_begin(ctx, K_BIN_COUNT);
     hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G, ranges), dim3(kBinThreads), 0, st,
-                       (const PT *)x, (const PT *)y, (const PT *)z, sp, g, B.cmat);
+                       (const PT *)x, (const PT *)y, (const PT *)z, sp, g, B.cmat,
+                       by_rowscan ? resv : nullptr);
     launch_end(ctx, K_BIN_COUNT);
     launch_begin(ctx, K_BIN_ROWSCAN);
+    const RowscanChunks rc = by_rowscan
+                                 ? RowscanChunks{resv, cursor, cbase, ext, chunks, m->chunk_log2}
+                                 : RowscanChunks{};
     hipLaunchKernelGGL(k_bin_rowscan, dim3((unsigned)div_up4(ntiles)), dim3(256), 0, st, B.cmat,
-                       sp.G, ntiles, counts);
+                       sp.G, ntiles, counts, rc);
     launch_end(ctx, K_BIN_ROWSCAN);
-    launch_begin(ctx, K_BIN_SCAN);
-    hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
-                       cbase, chunks, nch, ext, ntiles, m->chunk_log2);
-    launch_end(ctx, K_BIN_SCAN);
+    if (!by_rowscan) {
+        launch_begin(ctx, K_BIN_SCAN);
+        hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
+                           cbase, chunks, nch, ext, ntiles, m->chunk_log2);
+        launch_end(ctx, K_BIN_SCAN);
+    }
     Rec<PT> *sorted = static_cast<Rec<PT> *>(B.sorted);
     PT *q1s = static_cast<PT *>(B.qs), *q2s = q1s ? q1s + n : nullptr;
     launch_begin(ctx, K_BIN_SCATTER);
@@ -937,11 +995,12 @@ int perm_binned(Mesh *m, const void *x, const void *y, const void *z, int64_t n,
     const Split sp = split_of(n, m->slice_log2, ntiles);
     launch_begin(ctx, K_BIN_COUNT);
     hipLaunchKernelGGL((k_bin_count<PT, MT>), dim3(sp.G, ranges), dim3(kBinThreads), 0, st,
-                       (const PT *)x, (const PT *)y, (const PT *)z, sp, g, B.cmat);
+                       (const PT *)x, (const PT *)y, (const PT *)z, sp, g, B.cmat,
+                       (unsigned long long *)nullptr);
     launch_end(ctx, K_BIN_COUNT);
     launch_begin(ctx, K_BIN_ROWSCAN);
     hipLaunchKernelGGL(k_bin_rowscan, dim3((unsigned)div_up4(ntiles)), dim3(256), 0, st, B.cmat,
-                       sp.G, ntiles, counts);
+                       sp.G, ntiles, counts, RowscanChunks{});
     launch_end(ctx, K_BIN_ROWSCAN);
     launch_begin(ctx, K_BIN_SCAN);
     hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(256), 0, st, (const unsigned *)counts, cursor,
