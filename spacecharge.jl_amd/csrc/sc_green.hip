@@ -267,7 +267,7 @@ k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, i
     F[f_off(a, q) + idx] = field_green_comp<FAST>(c, u, v, w);
 }
 
-// Float32 meshes: the three field components of a node from shared terms.
+// The three field components of a node from shared terms.
 // With r = |(u, v, w)| and F_x = f(u, v, w), F_y = f(v, w, u),
 // F_z = f(w, u, v) (field_green), each component's quotient log follows
 // from the logs of r + u, r + v, r + w and of sums of two squares, since
@@ -280,10 +280,12 @@ k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, i
 // node (x, y) of kF3Planes consecutive z planes, so log(u^2 + v^2) is formed
 // once per lane.  Per node this is 1 sqrt, 2 atan and 5.25 logs against
 // 3 sqrt, 3 atan, 6 logs and 3 quotients for the three components taken one
-// by one, with the few-ulp functions of k_green_f<true>.
+// by one.  FAST: the few-ulp log and atan (Float32 meshes); otherwise the
+// library's (Float64 meshes).
 // Grid (x-y plane blocks, z plane groups, image sets).
 constexpr int kF3Planes = 4;
 
+template <bool FAST>
 __global__ void __launch_bounds__(256)
 k_green_f3(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, int per)
 {
@@ -304,7 +306,7 @@ k_green_f3(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, 
     const double u = ((double)in - 0.5) * dx;
     const double v = ((double)jn - 0.5) * dy;
     const double uu = u * u, vv = v * v;
-    const double Quv = fast_log(uu + vv);
+    const double Quv = log_<FAST>(uu + vv);
     const int64_t plane = (int64_t)nx1 * ny1;
     double *Fx = F + f_off(a, 3 * image);
     double *Fy = F + f_off(a, 3 * image + 1);
@@ -318,10 +320,10 @@ k_green_f3(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, 
         const double w = image ? ((double)(kn - a.d.nz) + 0.5) * dz + offzg : ((double)kn - 0.5) * dz;
         const double ww = w * w;
         const double r = sqrt((uu + vv) + ww);
-        const double Lu = fast_log(r + u), Lv = fast_log(r + v), Lw = fast_log(r + w);
-        const double Qvw = fast_log(vv + ww), Qwu = fast_log(ww + uu);
-        const double ax = fast_atan_q(v * w, r * u);
-        const double ay = fast_atan_q(w * u, r * v);
+        const double Lu = log_<FAST>(r + u), Lv = log_<FAST>(r + v), Lw = log_<FAST>(r + w);
+        const double Qvw = log_<FAST>(vv + ww), Qwu = log_<FAST>(ww + uu);
+        const double ax = atan_q<FAST>(v * w, r * u);
+        const double ay = atan_q<FAST>(w * u, r * v);
         const double hp = w == 0.0 ? 0.0
                                    : ((signbit(u) != signbit(v)) != signbit(w) ? -1.5707963267948966
                                                                                : 1.5707963267948966);
@@ -871,9 +873,9 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
         const int plane = (d.nx + 1) * (d.ny + 1);
         const int znodes = cath ? d.Lz : d.nz + 1;
         launch_begin(ctx, K_GREEN_F);
-        if (sizeof(T) == 4 && !a.pot)
-            hipLaunchKernelGGL(k_green_f3, dim3((unsigned)div_up(plane, 256), div_up(znodes, kF3Planes),
-                                                cath ? 2 : 1),
+        if (!a.pot)
+            hipLaunchKernelGGL(k_green_f3<sizeof(T) == 4>,
+                               dim3((unsigned)div_up(plane, 256), div_up(znodes, kF3Planes), cath ? 2 : 1),
                                dim3(256), 0, st, ws.F, a, (const SBunch *)nullptr, 0);
         else
             hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>,
@@ -941,12 +943,9 @@ void build_green_grouped(const Mesh *m0, int nb, const SBunch *tab, double gamma
     {
         const int per = div_up((d.nx + 1) * (d.ny + 1), 256);
         const int znodes = cath ? d.Lz : d.nz + 1;
-        if (sizeof(T) == 4)
-            hipLaunchKernelGGL(k_green_f3, dim3((unsigned)(per * nb), div_up(znodes, kF3Planes), cath ? 2 : 1),
-                               dim3(256), 0, st, (double *)nullptr, a, tab, per);
-        else
-            hipLaunchKernelGGL(k_green_f<sizeof(T) == 4>, dim3((unsigned)(per * nb), znodes, ncomp),
-                               dim3(256), 0, st, (double *)nullptr, a, tab, per);
+        hipLaunchKernelGGL(k_green_f3<sizeof(T) == 4>,
+                           dim3((unsigned)(per * nb), div_up(znodes, kF3Planes), cath ? 2 : 1),
+                           dim3(256), 0, st, (double *)nullptr, a, tab, per);
     }
     const int zc = cath ? d.Lz - 1 : d.nz;
     const int kp = (d.Kx + 1) / 2;
