@@ -196,9 +196,10 @@ k_bin_count(const PT *__restrict__ px, const PT *__restrict__ py, const PT *__re
 // (records in the low half, chunks in the high half; zeroed by k_bin_count)
 // and writes the tile's start (`cursor`), first extra sub-grid slot, chunk
 // entries and cleared extrapolation flag.  The runs then lie in the order
-// the tiles reserved them; each chunk is the same records (the tile's run
-// from k 2^chunk_log2) and the reduction adds a tile's chunks in the same
-// order, so the sums are those of the tile-ordered layout.  Sub-grid slots:
+// the tiles reserved them, but chunk k of a tile still holds the records at
+// positions k 2^chunk_log2 .. of the tile's run, and the reduction adds a
+// tile's chunks in the same order, so the sums are those of the tile-ordered
+// layout of k_bin_scan.  Sub-grid slots:
 // a tile's first chunk uses slot t, its further chunks ntiles + c + k for
 // chunk entries c .. c + nk - 1 (slots below 2 ntiles + chunks).
 struct RowscanChunks {
@@ -209,7 +210,7 @@ struct RowscanChunks {
 };
 
 // One wave per tile: exclusive prefix of the tile's row over the slices (in
-// place), and the tile's total.
+// place), the tile's total, and with rc.resv the reservation above.
 __device__ __forceinline__ void bin_rowscan_body(unsigned bx, unsigned *__restrict__ cmat, int G,
                                                  int ntiles, unsigned *__restrict__ tot,
                                                  RowscanChunks rc = RowscanChunks{})
