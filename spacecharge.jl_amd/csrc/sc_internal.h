@@ -147,10 +147,17 @@ struct Workspace {
     // the next build forks from it, so that it overlaps the deposit kernels
     // (latency bound) rather than the previous step's interpolation.
     // fork_ready: recorded by an eager deposit that no build has forked from
-    // yet; fork_capture: the id of the stream capture whose deposit recorded
-    // it (0: none).  A build with neither forks from the end of the stream.
+    // yet; start_recorded: an eager record exists (an older one is still a
+    // safe fork point for an eager build, which also waits for gf_free after
+    // the last eager z convolution, as long as no capture has used this
+    // mesh: a graph's z convolutions record no gf_free); captured: a stream
+    // capture has used this mesh's events; fork_capture: the id of the
+    // capture whose deposit recorded the event (0: none).  A build with no
+    // usable record forks from the end of the stream.
     hipEvent_t deposit_start = nullptr;
     bool fork_ready = false;
+    bool start_recorded = false;
+    bool captured = false;
     unsigned long long fork_capture = 0;
     // cached Green key for the opt-in reuse
     bool g_valid = false;

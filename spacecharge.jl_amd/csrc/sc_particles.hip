@@ -660,7 +660,10 @@ int launch_interp_vjp(Mesh *m, int pdt, const void *x, const void *y, const void
     const int64_t M = m->n[0] * m->n[1] * m->n[2];
     if (g_efield) {
         // the transpose of the gather is the deposit of the cotangents: the
-        // three channels share one binning pass where the binned path applies
+        // three channels share one binning pass where the binned path applies.
+        // Its start is a deposit start: a following (adjoint) solve's Green
+        // build forks there and runs beside it
+        if (int rc = mark_deposit_start(m)) return rc;
         const void *gE[3] = {gEx, gEy, gEz};
         void *outs[3];
         for (int c = 0; c < 3; ++c) outs[c] = static_cast<char *>(g_efield) + (size_t)c * M * m->dtype;

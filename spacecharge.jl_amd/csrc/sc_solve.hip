@@ -389,11 +389,17 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
                   (cap ? ctx->side && ws.green_done && ws.deposit_start : ensure_side_stream(m) == SC_OK);
         if (overlap) {
             // the build forks where this step's deposit started (recorded by
-            // a deposit of this capture, or by an eager deposit not yet
-            // forked from), else at the current end of `st`, which may still
-            // read this mesh's Green spectra
-            const bool from_deposit = cap ? ws.fork_capture == cid : ws.fork_ready;
-            if (!from_deposit) SC_HIP(ctx, hipEventRecord(ws.deposit_start, st));
+            // a deposit of this capture, or by an eager deposit; an older
+            // eager record only if no capture has used this mesh), else at
+            // the current end of `st`, which may still read this mesh's
+            // Green spectra
+            const bool from_deposit =
+                cap ? ws.fork_capture == cid : (ws.fork_ready || (ws.start_recorded && !ws.captured));
+            if (!from_deposit) {
+                SC_HIP(ctx, hipEventRecord(ws.deposit_start, st));
+                if (!cap) ws.start_recorded = true;
+            }
+            if (cap) ws.captured = true;
             ws.fork_ready = false;
             ws.fork_capture = 0;
             // an eager build also waits for the last eager z convolution,
@@ -537,6 +543,8 @@ int mark_deposit_start(Mesh *m)
     SC_HIP(m->ctx, hipEventRecord(ws.deposit_start, st));
     ws.fork_ready = !cap;
     ws.fork_capture = cid;
+    if (cap) ws.captured = true;
+    else ws.start_recorded = true;
     return SC_OK;
 }
 
