@@ -14,9 +14,13 @@
 //               slice of the particles in LDS and stores it as one column of
 //               a [tile][slice] count matrix (no global atomics)
 //   bin_rowscan one wave per tile: prefix of its row over the slices, and
-//               the tile's total
+//               the tile's total; on meshes of one binning range also the
+//               tile's run and chunk table entries (one 64-bit atomic per
+//               tile reserves them), and bin_scan is not launched
 //   bin_scan    one workgroup: tile offsets and the (tile, start, length)
 //               chunk table, chunks of at most 2^chunk_log2 particles
+//               (meshes of several binning ranges, the sort permutation,
+//               grouped launches)
 //   bin_scatter each slice writes (x, y, z, q) records into its reserved run
 //               of each tile, slots from an LDS cursor per tile
 //   deposit_tiles  one workgroup per chunk: Float64 LDS sub-grid, CIC with
