@@ -26,6 +26,9 @@ torch.autograd (segments of 10 steps checkpointed) whose loss, the final rms
 emittance, is differentiated with respect to the initial positions and
 charges.
 
+--workload fftlib times the field solve of the bunch's mesh against its
+FFT-library form (rocFFT through torch.fft on the 2n-padded grid).
+
 Prints one JSON line.  `value` is particles/s of the whole job (one GPU).
 The `roofline` object is for the kernel with the largest share of the step
 time, measured with the library's HIP-event timers on the launching stream
@@ -242,6 +245,62 @@ def bench_batched(args, sc, torch, dev, T):
         "deposit_tilings": tilings,
         "grouped_launches": "particle phases and same-shape small-mesh solves (default)",
         "other_grouping": by_mode,
+    }
+
+
+def bench_fftlib(args, sc, torch, dev, T):
+    """The field solve of the headline mesh against its FFT-library form on
+    the same GPU: torch.fft (rocFFT on ROCm) on the 2n-padded grid, one
+    rfftn of rho, three spectral products and three irfftn (plus three rfftn
+    when the Green spectra are transformed too), against sc.solve with the
+    Green spectra rebuilt (as solve! does) and reused.  Both legs in the
+    requested precision; the FFT-library legs use random data of the right
+    shape (their cost does not depend on the values)."""
+    n = args.grid
+    L = 2 * n
+    x, y, z, q = make_particles(args.particles, T, dev)
+    mesh = sc.Mesh3D((n, n, n), x, y, z, T=T, gamma=args.gamma, total_charge=1e-9)
+    sc.deposit(mesh, x, y, z, q)
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    ms_rebuild = timed(lambda: sc.solve(mesh))
+    mesh.set_green_reuse(True)
+    ms_reuse = timed(lambda: sc.solve(mesh))
+    mesh.set_green_reuse(False)
+    rho = torch.zeros((L, L, L), dtype=T, device=dev)
+    rho[:n, :n, :n] = torch.rand((n, n, n), dtype=T, device=dev)
+    G = [torch.rand((L, L, L), dtype=T, device=dev) for _ in range(3)]
+    Gh = [torch.fft.rfftn(g) for g in G]
+
+    def lib_solve(green):
+        spectra = [torch.fft.rfftn(g) for g in G] if green else Gh
+        R = torch.fft.rfftn(rho)
+        return [torch.fft.irfftn(R * g, s=(L, L, L))[:n, :n, :n] for g in spectra]
+
+    ms_fft = timed(lambda: lib_solve(False))
+    ms_fft_green = timed(lambda: lib_solve(True))
+    return {
+        "metric": "ms per field solve, library against an FFT-library (rocFFT via torch.fft) formulation",
+        "value": ms_rebuild, "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_rebuild, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.dtype, "data": "synthetic (seeded bunch on the GPU; random arrays for the FFT legs)",
+        "config": {"workload": f"field solve of a {n}^3 mesh (transforms on {L}^3)", "grid": [n, n, n],
+                   "parallelism": "single GPU"},
+        "library_solve_ms_green_rebuilt": ms_rebuild,
+        "library_solve_ms_green_reused": ms_reuse,
+        "fftlib_solve_ms": ms_fft,
+        "fftlib_solve_ms_with_green_transforms": ms_fft_green,
+        "fftlib_rfftn_ms": timed(lambda: torch.fft.rfftn(rho)),
+        "fftlib_irfftn_ms": timed(lambda: torch.fft.irfftn(Gh[0], s=(L, L, L))),
     }
 
 
@@ -683,14 +742,15 @@ def main():
     ap.add_argument("--particles", type=int, default=4_000_000)
     ap.add_argument("--gamma", type=float, default=1.0)
     ap.add_argument("--cathode", action="store_true")
-    ap.add_argument("--workload", choices=["bunch", "cathode", "batched", "adjoint"],
+    ap.add_argument("--workload", choices=["bunch", "cathode", "batched", "adjoint", "fftlib"],
                     default="bunch",
                     help="bunch: Gaussian + uniform ellipsoid (defaults 4M particles, 128^3); "
                          "cathode: uniform disk with exponential z emitted from a cathode, "
                          "2M particles, 64x64x128, image charge on; batched: 32 bunches of "
                          "uneven size and mesh through the batched calls; adjoint: forward "
                          "step, its VJP and a differentiated 100-step tracking run, 1M "
-                         "particles, 64^3")
+                         "particles, 64^3; fftlib: the field solve of the bunch's mesh against "
+                         "its rocFFT (torch.fft) formulation")
     ap.add_argument("--sorted", action="store_true",
                     help="put the particles in the deposit's tile order once before the warmup "
                          "(sort_permutation), as a tracking code does every few steps")
@@ -725,6 +785,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps(bench_adjoint(args, sc, torch, dev, T)))
+        return
+    if args.workload == "fftlib":
+        print(json.dumps(bench_fftlib(args, sc, torch, dev, T)))
         return
     if args.workload == "cathode":
         args.cathode = True
