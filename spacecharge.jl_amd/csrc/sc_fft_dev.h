@@ -1,7 +1,9 @@
 // sc_fft_dev.h -- workgroup-cooperative mixed-radix FFT in LDS (gfx950).
 //
 // A workgroup of kFftThreads lanes transforms a batch of nb pencils of one
-// length n held in LDS.  Element e of pencil b lives at buf[e*es + b*ps]:
+// length n held in LDS, each lane holding up to E complex values of a stage
+// (nb*n <= E*kFftThreads; E = fft_elems<T>() by default, twice that for the
+// long pencils of axes beyond fft_cap<T>()).  Element e of pencil b lives at buf[e*es + b*ps]:
 //   es = nb, ps = 1  ("element-major") for pencils that are strided in HBM
 //                    and batched over the contiguous axis, so a global
 //                    load of one element row is one coalesced segment;
@@ -111,11 +113,11 @@ __device__ __forceinline__ void dft_prime_store(const cpx<T> *v, const cpx<T> *t
     }
 }
 
-template <typename T, int R>
+template <typename T, int R, int E>
 __device__ __forceinline__ void fft_stage(cpx<T> *buf, const cpx<T> *tw, int n, int Ns, int nb,
                                           int es, int ps, bool bfast, int sign)
 {
-    constexpr int MAXB = (fft_elems<T>() + R - 1) / R;
+    constexpr int MAXB = (E + R - 1) / R;
     const int nR = n / R;
     const int TB = nb * nR;
     const int tstep = n / (Ns * R);
@@ -164,9 +166,9 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, const cpx<T> *tw, int n, 
 }
 
 // Transform nb pencils of length pl.n in LDS.  All lanes of the workgroup
-// must call it (it contains barriers).  Requires nb*n <= fft_cap<T>(),
+// must call it (it contains barriers).  Requires nb*n <= E*kFftThreads,
 // which the host guarantees.
-template <typename T>
+template <typename T, int E = fft_elems<T>()>
 __device__ void lds_fft(cpx<T> *buf, const cpx<T> *tw, const FftPlan &pl, int nb, int es, int ps,
                         bool bfast, int sign)
 {
@@ -175,12 +177,12 @@ __device__ void lds_fft(cpx<T> *buf, const cpx<T> *tw, const FftPlan &pl, int nb
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.rad[st];
         switch (R) {
-        case 2: fft_stage<T, 2>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 3: fft_stage<T, 3>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 4: fft_stage<T, 4>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 5: fft_stage<T, 5>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 7: fft_stage<T, 7>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 8: fft_stage<T, 8>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 2: fft_stage<T, 2, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 3: fft_stage<T, 3, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 4: fft_stage<T, 4, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 5: fft_stage<T, 5, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 7: fft_stage<T, 7, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 8: fft_stage<T, 8, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
         default: break;  // unreachable: make_plan rejects other radices
         }
         Ns *= R;

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "spacecharge_mi355.h"
@@ -70,6 +71,28 @@ template <typename T>
 constexpr int fft_cap()
 {
     return fft_elems<T>() * kFftThreads;
+}
+// Pencils longer than fft_cap (axes of more than fft_cap / 2 nodes) take
+// twice the values per lane, one pencil per workgroup, and LDS buffers of
+// up to 4 pencils (the z convolution with the cathode image), which the
+// device's 160 KiB per workgroup holds up to fft_cap_long.
+template <typename T>
+constexpr int fft_elems_long()
+{
+    return 2 * fft_elems<T>();
+}
+template <typename T>
+constexpr int fft_cap_long()
+{
+    return fft_elems_long<T>() * kFftThreads;
+}
+// Calls f(std::integral_constant<int, E>{}) with the values per lane for
+// pencils of length L.
+template <typename T, typename F>
+void with_fft_elems(int L, F &&f)
+{
+    if (L <= fft_cap<T>()) f(std::integral_constant<int, fft_elems<T>()>{});
+    else f(std::integral_constant<int, fft_elems_long<T>()>{});
 }
 constexpr int kFftMaxRadix = 7;
 

@@ -46,7 +46,7 @@ namespace {
 // ---------------------------------------------------------------------------
 // x pass of the real input: rows (y, z) of component blockIdx.y, zero-padded
 // to 2nx, kx in [0, nx] to that component's spectrum.
-template <typename T>
+template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
 k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__restrict__ twg,
             FftPlan pl, Dims d, int nb)
@@ -67,7 +67,7 @@ k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__
         buf[idx] = cpx<T>{v, (T)0};
     }
     __syncthreads();
-    lds_fft(buf, tw, pl, nb, 1, Lx, false, -1);
+    lds_fft<T, E>(buf, tw, pl, nb, 1, Lx, false, -1);
     for (int idx = threadIdx.x; idx < nb * d.Kx; idx += blockDim.x) {
         const int b = idx / d.Kx, kx = idx - b * d.Kx;
         const int r = r0 + b;
@@ -80,7 +80,7 @@ k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__
 
 // y pass: pencils (kx, z < nz) of component blockIdx.z along y, ny inputs
 // zero-padded to 2ny.
-template <typename T>
+template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
 k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb)
 {
@@ -99,7 +99,7 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
         buf[idx] = v;
     }
     __syncthreads();
-    lds_fft(buf, tw, pl, nb, nb, 1, true, -1);
+    lds_fft<T, E>(buf, tw, pl, nb, nb, 1, true, -1);
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
@@ -111,7 +111,7 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
 // image charge -rho[.., nz-1-z]), products with the three Green spectra,
 // backward z transforms, z < nz of each field spectrum to B.  The potential
 // (mode kPotential) has one Green spectrum, real and even in every axis.
-template <typename T>
+template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
 k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
         const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb,
@@ -144,8 +144,8 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
         if (cathode) P1[idx] = vi;
     }
     __syncthreads();
-    lds_fft(P0, tw, pl, nb, nb, 1, true, -1);
-    if (cathode) lds_fft(P1, tw, pl, nb, nb, 1, true, -1);
+    lds_fft<T, E>(P0, tw, pl, nb, nb, 1, true, -1);
+    if (cathode) lds_fft<T, E>(P1, tw, pl, nb, nb, 1, true, -1);
     const bool kyneg = ky > ny;
     const int kyp = kyneg ? Ly - ky : ky;
     const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;  // one free-space Green spectrum
@@ -176,7 +176,7 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
             W[idx] = out;
         }
         __syncthreads();
-        lds_fft(W, tw, pl, nb, nb, 1, true, +1);
+        lds_fft<T, E>(W, tw, pl, nb, nb, 1, true, +1);
         cpx<T> *Bc = B + (size_t)c * mz * zs;
         for (int idx = threadIdx.x; idx < nb * mz; idx += blockDim.x) {
             const int z = idx / nb, b = idx - z * nb;
@@ -191,7 +191,7 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
 // the three g_E spectra in Gs, product with conj(G_c^) summed over c (and,
 // with the cathode, with conj(Gi_c^) into a second sum), backward z
 // transforms, and for z < nz: out[z] = W[z] - Wi[nz-1-z].
-template <typename T>
+template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
 k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__restrict__ Gf,
             const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d,
@@ -226,7 +226,7 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
             Q[idx] = v;
         }
         __syncthreads();
-        lds_fft(Q, tw, pl, nb, nb, 1, true, -1);
+        lds_fft<T, E>(Q, tw, pl, nb, nb, 1, true, -1);
         const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
             const int kz = idx / nb, b = idx - kz * nb;
@@ -250,8 +250,8 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
         }
         __syncthreads();
     }
-    lds_fft(W, tw, pl, nb, nb, 1, true, +1);
-    if (cathode) lds_fft(W1, tw, pl, nb, nb, 1, true, +1);
+    lds_fft<T, E>(W, tw, pl, nb, nb, 1, true, +1);
+    if (cathode) lds_fft<T, E>(W1, tw, pl, nb, nb, 1, true, +1);
     __syncthreads();
     for (int idx = threadIdx.x; idx < nb * mz; idx += blockDim.x) {
         const int z = idx / nb, b = idx - z * nb;
@@ -265,7 +265,7 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
 }
 
 // Backward y pass: pencils (kx, z < nz) of each field spectrum, y < ny kept.
-template <typename T>
+template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
 k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb)
 {
@@ -284,7 +284,7 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
         buf[idx] = v;
     }
     __syncthreads();
-    lds_fft(buf, tw, pl, nb, nb, 1, true, +1);
+    lds_fft<T, E>(buf, tw, pl, nb, nb, 1, true, +1);
     for (int idx = threadIdx.x; idx < nb * d.my; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
@@ -294,9 +294,9 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
 
 // Backward x pass (complex to real): rows (y < ny, z < nz) of each field
 // spectrum, Hermitian completion, x < nx of the real part to efield[c].
-template <typename T>
+template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
-k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ E, const cpx<T> *__restrict__ twg, FftPlan pl,
+k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ efield, const cpx<T> *__restrict__ twg, FftPlan pl,
         Dims d, int nb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -324,8 +324,8 @@ k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ E, const cpx<T> *__restric
         buf[b * Lx + Lx - k] = cpx<T>{v.x, -v.y};
     }
     __syncthreads();
-    lds_fft(buf, tw, pl, nb, 1, Lx, false, +1);
-    T *Ec = E + (size_t)c * nx * nrows;
+    lds_fft<T, E>(buf, tw, pl, nb, 1, Lx, false, +1);
+    T *Ec = efield + (size_t)c * nx * nrows;
     for (int idx = threadIdx.x; idx < nb * nx; idx += blockDim.x) {
         const int b = idx / nx, x = idx - b * nx;
         const int r = r0 + b;
@@ -430,38 +430,50 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     cpx<T> *F = static_cast<cpx<T> *>(io.fwd);
     cpx<T> *B = static_cast<cpx<T> *>(io.inv);
     if (!fast(0)) {
-        const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
-        const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
-        launch_begin(ctx, K_RHO_FWD_X);
-        hipLaunchKernelGGL(k_rho_fwd_x<T>, dim3(div_up(d.my * d.mz, nb), io.nin), dim3(kFftThreads),
-                           sh, st, static_cast<const T *>(io.in), F, twx, ws.plan[0], d, nb);
-        launch_end(ctx, K_RHO_FWD_X);
+        with_fft_elems<T>(d.Lx, [&](auto e) {
+            constexpr int E = decltype(e)::value;
+            const int nb = pick_nb(d.Lx, E * kFftThreads, 16);
+            const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
+            launch_begin(ctx, K_RHO_FWD_X);
+            hipLaunchKernelGGL((k_rho_fwd_x<T, E>), dim3(div_up(d.my * d.mz, nb), io.nin),
+                               dim3(kFftThreads), sh, st, static_cast<const T *>(io.in), F, twx,
+                               ws.plan[0], d, nb);
+            launch_end(ctx, K_RHO_FWD_X);
+        });
     }
     if (!fast(1)) {
-        const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
-        const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
-        launch_begin(ctx, K_RHO_FWD_Y);
-        hipLaunchKernelGGL(k_rho_fwd_y<T>, dim3(div_up(d.Kx, nb), d.mz, io.nin), dim3(kFftThreads),
-                           sh, st, F, twy, ws.plan[1], d, nb);
-        launch_end(ctx, K_RHO_FWD_Y);
+        with_fft_elems<T>(d.Ly, [&](auto e) {
+            constexpr int E = decltype(e)::value;
+            const int nb = pick_nb(d.Ly, E * kFftThreads, 16);
+            const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
+            launch_begin(ctx, K_RHO_FWD_Y);
+            hipLaunchKernelGGL((k_rho_fwd_y<T, E>), dim3(div_up(d.Kx, nb), d.mz, io.nin),
+                               dim3(kFftThreads), sh, st, F, twy, ws.plan[1], d, nb);
+            launch_end(ctx, K_RHO_FWD_Y);
+        });
     }
     // ---- fused z convolution ----
     if (overlap) SC_HIP(ctx, hipStreamWaitEvent(st, ws.green_done, 0));
     if (!fast(2)) {
-        const int nbuf = cath ? 3 : 2;
-        int nb = pick_nb(d.Lz, fft_cap<T>(), 16);
-        while (nb > 1 && sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz) > 65536) nb /= 2;
-        const size_t sh = sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz);
-        launch_begin(ctx, K_ZCONV);
-        if (mode_adjoint(mode))
-            hipLaunchKernelGGL(k_zconv_adj<T>, dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh, st,
-                               (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
-                               static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, cath);
-        else
-            hipLaunchKernelGGL(k_zconv<T>, dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh, st,
-                               (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
-                               static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, mode);
-        launch_end(ctx, K_ZCONV);
+        with_fft_elems<T>(d.Lz, [&](auto e) {
+            constexpr int E = decltype(e)::value;
+            const int nbuf = cath ? 3 : 2;
+            // at most 64 KiB of LDS where more pencils fit (occupancy); one
+            // long pencil may take more (up to the device's 160 KiB)
+            int nb = pick_nb(d.Lz, E * kFftThreads, 16);
+            while (nb > 1 && sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz) > 65536) nb /= 2;
+            const size_t sh = sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz);
+            launch_begin(ctx, K_ZCONV);
+            if (mode_adjoint(mode))
+                hipLaunchKernelGGL((k_zconv_adj<T, E>), dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads),
+                                   sh, st, (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
+                                   static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, cath);
+            else
+                hipLaunchKernelGGL((k_zconv<T, E>), dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh,
+                                   st, (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
+                                   static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, mode);
+            launch_end(ctx, K_ZCONV);
+        });
     }
     if (ws.green_done && !cap) {  // the side stream exists: the next build waits for this point
         SC_HIP(ctx, hipEventRecord(ws.gf_free, st));
@@ -469,20 +481,27 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     }
     // ---- backward y, x ----
     if (!fast(3)) {
-        const int nb = pick_nb(d.Ly, fft_cap<T>(), 16);
-        const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
-        launch_begin(ctx, K_INV_Y);
-        hipLaunchKernelGGL(k_inv_y<T>, dim3(div_up(d.Kx, nb), d.mz, io.nout), dim3(kFftThreads), sh,
-                           st, B, twy, ws.plan[1], d, nb);
-        launch_end(ctx, K_INV_Y);
+        with_fft_elems<T>(d.Ly, [&](auto e) {
+            constexpr int E = decltype(e)::value;
+            const int nb = pick_nb(d.Ly, E * kFftThreads, 16);
+            const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
+            launch_begin(ctx, K_INV_Y);
+            hipLaunchKernelGGL((k_inv_y<T, E>), dim3(div_up(d.Kx, nb), d.mz, io.nout), dim3(kFftThreads),
+                               sh, st, B, twy, ws.plan[1], d, nb);
+            launch_end(ctx, K_INV_Y);
+        });
     }
     if (!fast(4)) {
-        const int nb = pick_nb(d.Lx, fft_cap<T>(), 16);
-        const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
-        launch_begin(ctx, K_INV_X);
-        hipLaunchKernelGGL(k_inv_x<T>, dim3(div_up(d.my * d.mz, nb), io.nout), dim3(kFftThreads), sh,
-                           st, (const cpx<T> *)B, static_cast<T *>(io.out), twx, ws.plan[0], d, nb);
-        launch_end(ctx, K_INV_X);
+        with_fft_elems<T>(d.Lx, [&](auto e) {
+            constexpr int E = decltype(e)::value;
+            const int nb = pick_nb(d.Lx, E * kFftThreads, 16);
+            const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
+            launch_begin(ctx, K_INV_X);
+            hipLaunchKernelGGL((k_inv_x<T, E>), dim3(div_up(d.my * d.mz, nb), io.nout),
+                               dim3(kFftThreads), sh, st, (const cpx<T> *)B, static_cast<T *>(io.out),
+                               twx, ws.plan[0], d, nb);
+            launch_end(ctx, K_INV_X);
+        });
     }
     SC_HIP(ctx, hipGetLastError());
     return SC_OK;
@@ -596,13 +615,15 @@ int ensure_workspace(Mesh *m)
     const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
     const int L[3] = {d.Lx, d.Ly, d.Lz};
     const int64_t mn[3] = {m->n[0], m->n[1], m->n[2]};
-    const int cap = (m->dtype == SC_F32) ? fft_cap<float>() : fft_cap<double>();
+    const int cap = (m->dtype == SC_F32) ? fft_cap_long<float>() : fft_cap_long<double>();
     for (int a = 0; a < 3; ++a) {
-        // one pencil must fit a workgroup's register budget
+        // one pencil must fit one workgroup (registers and LDS)
         if (L[a] > cap)
             return set_err(ctx, SC_ERR_UNSUPPORTED,
-                           "grid size " + std::to_string(mn[a]) + " exceeds the maximum of " +
-                               std::to_string(cap / 2) + " per axis for this mesh type");
+                           "grid size " + std::to_string(mn[a]) + " needs a transform of length " +
+                               std::to_string(L[a]) + ", beyond the maximum of " +
+                               std::to_string(cap) + " for this mesh type (at most " +
+                               std::to_string(cap / 2) + " nodes per axis)");
         if (!make_plan(L[a], &ws.plan[a]))  // unreachable: transform_half gives 7-smooth lengths
             return set_err(ctx, SC_ERR_UNSUPPORTED,
                            "no transform plan for length " + std::to_string(L[a]));

@@ -25,7 +25,7 @@ LLVM = "/opt/rocm/llvm/bin"
 MAGIC = b"__CLANG_OFFLOAD_BUNDLE__"
 TARGET = "hipv4-amdgcn-amd-amdhsa--gfx950"
 # generic-engine Float64 kernels (sc_solve.hip) allowed a small private segment
-GENERIC_F64 = re.compile(r"^_ZN2sc12_GLOBAL__N_1\d+k_(rho_fwd_y|zconv|zconv_adj|inv_y|inv_x)IdE")
+GENERIC_F64 = re.compile(r"^_ZN2sc12_GLOBAL__N_1\d+k_(rho_fwd_y|zconv|zconv_adj|inv_y|inv_x)Id(E|Li)")
 
 
 def _tool(name):

@@ -94,6 +94,36 @@ def test_solve_adjoint_matches_oracle(grid, gamma, cath, generic):
                 assert e_gpu <= (1e-5 if kind == "field" else 1e-4)
 
 
+@pytest.mark.parametrize("grid,cath,T", [((6, 6, 1500), False, torch.float32),
+                                         ((5, 6, 2048), True, torch.float32),
+                                         ((1000, 6, 5), False, torch.float64),
+                                         ((6, 5, 1024), True, torch.float64)])
+def test_solve_adjoint_long_axis(grid, cath, T):
+    """Transforms beyond 2048 Float32 / 1024 Float64 points, where the engine
+    holds twice the values per lane; the bunch's own field as cotangent."""
+    sc = _sc()
+    x, y, z, q = _bunch(20_000, 33, zshift=4e-3 if cath else 0.0)
+    mesh = sc.Mesh3D(grid, x, y, z, T=T)
+    sc.deposit(mesh, x, y, z, q)
+    rho = np.asfortranarray(mesh_to_np(mesh._rho, grid).astype(np.float64))
+    field = orc.solve(rho, mesh.min_bounds, mesh.max_bounds, mesh.delta, mesh.gamma, cath)
+    g = np.asfortranarray(field / np.abs(field).max())
+    got = sc.solve_adjoint(mesh, torch.as_tensor(g, device=DEV), at_cathode=cath)
+    torch.cuda.synchronize()
+    assert got.shape == tuple(grid) and got.dtype == T
+    args = (g, mesh.min_bounds, mesh.max_bounds, mesh.delta, mesh.gamma, cath)
+    truth = orc.solve_transpose_extended(*args)
+    e_gpu = rel_l2(got.cpu().numpy().astype(np.float64), truth)
+    if T == torch.float64:
+        e_orc = rel_l2(orc.solve_transpose(*args), truth)
+        print(f"solve adjoint long axis {T} {grid} cathode={cath}: vs extended GPU {e_gpu:.2e} "
+              f"oracle(f64) {e_orc:.2e}")
+        assert e_gpu <= 1e-12 and e_gpu <= e_orc + 1e-14
+    else:
+        print(f"solve adjoint long axis {T} {grid} cathode={cath}: vs extended {e_gpu:.2e}")
+        assert e_gpu <= 1e-5
+
+
 @pytest.mark.parametrize("cath", [False, True])
 def test_solve_adjoint_dot_product_identity(cath):
     """<S rho, g> = <rho, S^T g> with the GPU's forward and transposed solves

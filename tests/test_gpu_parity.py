@@ -242,6 +242,51 @@ def test_cathode_matches_oracle(grid, gamma, T):
     assert max(errs) <= 1e-5
 
 
+# Axes whose transform exceeds 2048 Float32 / 1024 Float64 points: the
+# engine then holds twice the values per lane, one pencil per workgroup, and
+# the z convolution takes more than 64 KiB of LDS.  Each case reaches the
+# per-axis maximum (4096 / 2048 points) or pads to a 7-smooth length.
+LONG_AXIS_CASES = [
+    ((8, 8, 1500), 2.0, False, torch.float32),    # z: 3000 points
+    ((1100, 6, 6), 1.0, False, torch.float32),    # x: 2200 padded to 2240
+    ((6, 2048, 4), 1.0, False, torch.float32),    # y: 4096
+    ((6, 6, 1800), 1.0, True, torch.float32),     # cathode, z: 3600
+    ((5, 6, 2048), 2.0, True, torch.float32),     # cathode, z: 4096
+    ((8, 8, 700), 1.0, False, torch.float64),     # z: 1400
+    ((1000, 5, 5), 3.0, False, torch.float64),    # x: 2000
+    ((6, 1024, 6), 1.0, False, torch.float64),    # y: 2048
+    ((6, 5, 1024), 1.5, True, torch.float64),     # cathode, z: 2048
+]
+
+
+@pytest.mark.parametrize("grid,gamma,cath,T", LONG_AXIS_CASES)
+def test_long_axis_matches_oracle(grid, gamma, cath, T):
+    res = _solve_pair(grid, T, gamma, at_cathode=cath, zshift=4e-3 if cath else 0.0)
+    if T == torch.float64:
+        _check_fp64(f"long axis f64 {grid} gamma={gamma} cathode={cath}", *res)
+        return
+    got, ref = res
+    errs = [rel_l2(got[..., c], ref[..., c]) for c in range(3)]
+    print(f"long axis f32 {grid} gamma={gamma} cathode={cath}: rel L2 "
+          f"{['%.2e' % e for e in errs]}")
+    assert max(errs) <= 1e-5
+
+
+@pytest.mark.parametrize("grid,T", [((4, 4, 2049), torch.float32), ((2049, 4, 4), torch.float32),
+                                    ((4, 1025, 4), torch.float64)])
+def test_axis_beyond_the_maximum_is_refused(grid, T):
+    """2049 Float32 / 1025 Float64 nodes need more than 4096 / 2048 points
+    (padded to 4116 / 2058); every axis up to 2048 / 1024 nodes fits, since
+    4096 and 2048 are themselves 7-smooth."""
+    sc = _sc()
+    x, y, z = gaussian(1000, (1e-3, 1e-3, 2e-3), 5)
+    xt, yt, zt, qt = _tensors((x, y, z, np.full(1000, 1e-12)), torch.float64)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=T)
+    sc.deposit(mesh, xt, yt, zt, qt)
+    with pytest.raises(sc.SpaceChargeError, match="beyond the maximum"):
+        sc.solve(mesh)
+
+
 @pytest.mark.parametrize("grid", [(20, 33, 17), (1025, 1025, 18), (300, 300, 300),
                                   (4097, 4097, 3)])
 def test_deposit_clamped_and_fallback(grid):

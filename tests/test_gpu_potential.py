@@ -91,6 +91,23 @@ def test_potential_fp32_matches_extended(grid, gamma, generic):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("grid,T", [((6, 6, 1500), torch.float32), ((6, 2048, 5), torch.float32),
+                                    ((6, 700, 6), torch.float64), ((1024, 5, 6), torch.float64)])
+def test_potential_long_axis(grid, T):
+    """Transforms beyond 2048 Float32 / 1024 Float64 points (twice the
+    engine's values per lane), against the extended-precision potential."""
+    sc = _sc()
+    mesh = _mesh(grid, T, 1.0)
+    phi = sc.solve_potential(mesh)
+    torch.cuda.synchronize()
+    got = phi.cpu().numpy().astype(np.float64)
+    rho = np.asfortranarray(mesh_to_np(mesh._rho, grid).astype(np.float64))
+    truth = orc.solve_potential_extended(rho, mesh.delta, mesh.gamma)
+    err = rel_l2(got, truth)
+    print(f"potential long axis {T} {grid}: vs extended {err:.2e}")
+    assert err <= (1e-12 if T == torch.float64 else 1e-5)
+
+
 @pytest.mark.parametrize("gamma", [1.0, 2.5])
 def test_potential_gradient_is_field(gamma):
     """-dphi/dx, -dphi/dy = Ex, Ey and -dphi/dz = gamma^2 Ez (lab-frame
