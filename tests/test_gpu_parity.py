@@ -227,6 +227,52 @@ def test_solve_any_grid_size_matches_oracle(grid, gamma, T):
     assert max(errs) <= 1e-5
 
 
+# Seeded random mesh shapes (2 to 40 nodes per axis): register-engine,
+# padded and tiny lengths in every combination, both mesh types, free space
+# and cathode; the whole step (bounds, deposit, solve, interpolation).
+_rs = np.random.default_rng(2024)
+RANDOM_STEPS = [(tuple(int(v) for v in _rs.integers(2, 41, 3)), k) for k in range(12)]
+
+
+@pytest.mark.parametrize("grid,k", RANDOM_STEPS)
+def test_random_grid_shapes_full_step(grid, k):
+    sc = _sc()
+    T = torch.float64 if k % 2 == 0 else torch.float32
+    mnp = np.float64 if T == torch.float64 else np.float32
+    cath = k % 3 == 0
+    gamma = (1.0, 2.0, 5.0)[k % 3]
+    x, y, z = gaussian(5_000, (1e-3, 1.5e-3, 2e-3), 40 + k)
+    z = z + (4e-3 if cath else 0.0)
+    q = np.full(5_000, 1e-10 / 5_000)
+    xt, yt, zt, qt = _tensors((x, y, z, q), torch.float64)
+    mesh = sc.Mesh3D(grid, xt, yt, zt, T=T, gamma=gamma)
+    lo, hi, dl = orc.bounds_auto(x, y, z, grid, mnp)
+    assert mesh.min_bounds == lo and mesh.max_bounds == hi and mesh.delta == dl
+    sc.deposit(mesh, xt, yt, zt, qt)
+    sc.solve(mesh, at_cathode=cath)
+    Ex, Ey, Ez = sc.interpolate_field(mesh, xt, yt, zt)
+    torch.cuda.synchronize()
+    rho = mesh_to_np(mesh._rho, grid)
+    rref = orc.deposit(x, y, z, q, grid, lo, dl, mnp)
+    derr = np.abs(rho.astype(np.float64) - rref).max() / np.abs(rref).max()
+    assert derr <= (1e-13 if T == torch.float64 else 1e-5), derr
+    rho64 = np.asfortranarray(rho.astype(np.float64))
+    ef = mesh_to_np(mesh._ef, grid)
+    ref = orc.solve(rho64, lo, hi, dl, mesh.gamma, cath)
+    label = f"random step {grid} {T} gamma={gamma} cathode={cath}"
+    if T == torch.float64:
+        truth = orc.solve_extended(rho64, lo, hi, dl, mesh.gamma, cath)
+        _check_fp64(label, ef, ref, truth)
+    else:
+        errs = [rel_l2(ef[..., c], ref[..., c]) for c in range(3)]
+        print(f"{label}: rel L2 {['%.2e' % e for e in errs]}")
+        assert max(errs) <= 1e-5
+    # the gather, bit for bit, from the GPU's own field
+    rE = orc.interpolate(np.asfortranarray(ef), x, y, z, lo, dl)
+    for g, r in zip((Ex, Ey, Ez), rE):
+        assert np.array_equal(g.cpu().numpy(), r)
+
+
 @pytest.mark.parametrize("grid,gamma", [((8, 8, 8), 1.0), ((16, 16, 32), 2.0), ((12, 10, 6), 1.0),
                                         ((32, 64, 32), 1.5), ((11, 13, 17), 1.0),
                                         ((9, 10, 65), 2.0)])
