@@ -28,6 +28,40 @@ struct cpx {
     T x, y;
 };
 
+// Loads and stores through the global address space.  Kernels that may take
+// their buffers from a descriptor table in device memory (the grouped
+// launches) cannot show the compiler that those pointers are global, and
+// plain accesses become flat instructions, which also count against the LDS
+// wait counter: every LDS exchange of a transform then waits for the memory
+// operations still in flight.  Complex values go as their two components
+// (the compiler merges them into one 8- or 16-byte access).
+template <typename T>
+using gbl = __attribute__((address_space(1))) T;
+
+template <typename T>
+__device__ __forceinline__ T gld(const T *p)
+{
+    return *(const gbl<T> *)p;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> gld(const cpx<T> *p)
+{
+    const gbl<T> *q = (const gbl<T> *)p;
+    return cpx<T>{q[0], q[1]};
+}
+template <typename T>
+__device__ __forceinline__ void gst(T *p, T v)
+{
+    *(gbl<T> *)p = v;
+}
+template <typename T>
+__device__ __forceinline__ void gst(cpx<T> *p, cpx<T> v)
+{
+    gbl<T> *q = (gbl<T> *)p;
+    q[0] = v.x;
+    q[1] = v.y;
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> c_add(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>

@@ -264,7 +264,7 @@ k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, i
     const double u = ((double)in - 0.5) * dx;
     const double v = ((double)jn - 0.5) * dy;
     const double w = image ? ((double)(kn - a.d.nz) + 0.5) * dz + offzg : ((double)kn - 0.5) * dz;
-    F[f_off(a, q) + idx] = field_green_comp<FAST>(c, u, v, w);
+    gst(F + f_off(a, q) + idx, field_green_comp<FAST>(c, u, v, w));
 }
 
 // The three field components of a node from shared terms.
@@ -308,9 +308,9 @@ k_green_f3(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, 
     const double uu = u * u, vv = v * v;
     const double Quv = log_<FAST>(uu + vv);
     const int64_t plane = (int64_t)nx1 * ny1;
-    double *Fx = F + f_off(a, 3 * image);
-    double *Fy = F + f_off(a, 3 * image + 1);
-    double *Fz = F + f_off(a, 3 * image + 2);
+    gbl<double> *Fx = (gbl<double> *)(F + f_off(a, 3 * image));
+    gbl<double> *Fy = (gbl<double> *)(F + f_off(a, 3 * image + 1));
+    gbl<double> *Fz = (gbl<double> *)(F + f_off(a, 3 * image + 2));
     const int zn = z_nodes(a, image);
     const int k0 = blockIdx.y * kF3Planes;
 #pragma unroll
@@ -571,7 +571,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
         F = sb->F;
         R = static_cast<T *>(sb->R);
     }
-    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = gld(twg + i);
     const int c = comp_of(a, q), image = q / 3;
     const int nx = a.d.nx, ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int bpairs = (ny + 1) / 2;
@@ -581,7 +581,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     const bool live = P < npencils;
     const int k = live ? P / bpairs : 0, ya = live ? 2 * (P - k * bpairs) : 0;
     const bool hasb = ya + 1 < ny;
-    const double *Fq = F + f_off(a, q);
+    const gbl<double> *Fq = (const gbl<double> *)(F + f_off(a, q));
     const T sgn = (c == 0) ? (T)-1 : (T)1;
     // The extension's upper half (e > nx) mirrors cells nx - 1 .. 1, so each
     // cell row is differenced once (e < nx, the first E/2 registers when
@@ -605,7 +605,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
             for (int yy = 0; yy < 3; ++yy)
 #pragma unroll
                 for (int zz = 0; zz < 2; ++zz) {
-                    const double *p = Fq + e + sy * yr[yy] + sz * (k + zz);
+                    const gbl<double> *p = Fq + e + sy * yr[yy] + sz * (k + zz);
                     f[u][yy][zz][0] = p[0];
                     f[u][yy][zz][1] = p[1];
                 }
@@ -631,7 +631,7 @@ k_green_1r(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__rest
     }
     reg::fft<T, L, E, -1>(v, j, row, 1, tw);
     if (live) {
-        T *Rq = R + r_off(a, q);
+        gbl<T> *Rq = (gbl<T> *)(R + r_off(a, q));
 #pragma unroll
         for (int r = 0; r <= E / 2; ++r) {
             const int kx = j + r * TPP;
@@ -660,11 +660,11 @@ k_green_2r(T *__restrict__ R, const cpx<T> *__restrict__ twg, GreenArgs a,
     const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int k = blockIdx.y;
     if (k >= z_cells(a, image)) return;  // uniform over the workgroup
-    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = gld(twg + i);
     const int ny = a.d.ny, Kx = a.d.Kx, Kp = a.d.Kp;
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kxa = 2 * (bx * NB + b), kxb = kxa + 1;
-    T *Rrow = R + r_off(a, q) + (size_t)k * (ny + 1) * Kp;
+    gbl<T> *Rrow = (gbl<T> *)(R + r_off(a, q) + (size_t)k * (ny + 1) * Kp);
     const T sgn = (c == 1) ? (T)-1 : (T)1;
     // every load is in range (clamped row and column) and issued without a
     // branch, so a lane's loads are all in flight together; the selects
@@ -716,13 +716,13 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
         Gi = static_cast<cpx<T> *>(sb->Gi);
         green_from_table(a, sb);
     }
-    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += kRThreads) tw[i] = gld(twg + i);
     const int q = blockIdx.z, c = comp_of(a, q), image = q / 3;
     const int ky = blockIdx.y;
     const int nz = a.d.nz, Kx = a.d.Kx, Kp = a.d.Kp, ny1 = a.d.ny + 1;
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kxa = 2 * (bx * NB + b), kxb = kxa + 1;
-    const T *Rq = R + r_off(a, q);
+    const gbl<T> *Rq = (const gbl<T> *)(R + r_off(a, q));
     const T sgn = (c == 2) ? (T)-1 : (T)1;
     // loads clamped in range and issued without branches (see k_green_2r)
     const int kxa_c = kxa < Kx ? kxa : Kx - 1, kxb_c = kxb < Kx ? kxb : Kx - 1;
@@ -743,7 +743,7 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
             // slot e holds displacement e (e < nz) or e - L (e > nz); row = displacement + nz - 1
             cell = e < nz ? e + nz - 1 : e - nz - 1;
         }
-        const T *row = Rq + ((size_t)cell * ny1 + ky) * Kp;
+        const gbl<T> *row = Rq + ((size_t)cell * ny1 + ky) * Kp;
         ra[r] = row[kxa_c];
         rb[r] = row[kxb_c];
         sc[r] = s;
@@ -757,14 +757,14 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
     const T scale = (T)green_scale(a, c);
     if (!image) {
-        T *G = Gf + (size_t)(q % 3) * (nz + 1) * ny1 * Kp;
+        gbl<T> *G = (gbl<T> *)(Gf + (size_t)(q % 3) * (nz + 1) * ny1 * Kp);
 #pragma unroll
         for (int r = 0; r <= E / 2; ++r) {
             const int kz = j + r * TPP;
             if (r < E / 2 || kz == nz) {
                 T ra, rb;
                 unpack_pair(v[r], c == 2, ra, rb);
-                T *row = G + ((size_t)kz * ny1 + ky) * Kp;
+                gbl<T> *row = G + ((size_t)kz * ny1 + ky) * Kp;
                 if (kxa < Kx) row[kxa] = ra * scale;
                 if (kxb < Kx) row[kxb] = rb * scale;
             }
@@ -783,8 +783,8 @@ k_green_3r(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
             const cpx<T> X = v[r];
             const cpx<T> Y = lds[b + ((L - kz) & (L - 1)) * NB];
             cpx<T> *row = G + ((size_t)kz * ny1 + ky) * Kp;
-            if (kxa < Kx) row[kxa] = cpx<T>{(X.x + Y.x) * h, (X.y - Y.y) * h};
-            if (kxb < Kx) row[kxb] = cpx<T>{(X.y + Y.y) * h, (Y.x - X.x) * h};
+            if (kxa < Kx) gst(row + kxa, cpx<T>{(X.x + Y.x) * h, (X.y - Y.y) * h});
+            if (kxb < Kx) gst(row + kxb, cpx<T>{(X.y + Y.y) * h, (Y.x - X.x) * h});
         }
     }
 }

@@ -53,7 +53,7 @@ k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> 
         in = static_cast<const T *>(sb->in);
         spec = static_cast<cpx<T> *>(sb->fwd);
     }
-    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = gld(twg + i);
     const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
     const int r = bx * NB + b;
     const int nrows = d.ny * d.nz, nx = d.nx;
@@ -64,7 +64,7 @@ k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> 
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         v[q] = cpx<T>{(T)0, (T)0};
-        if (q < E / 2 && live) v[q].x = rho[(size_t)r * nx + j + q * TPP];
+        if (q < E / 2 && live) v[q].x = gld(rho + (size_t)r * nx + j + q * TPP);
     }
     reg::fft<T, L, E, -1>(v, j, lds + b * reg::padded<L, E>(), 1, tw);
     if (live) {
@@ -73,7 +73,7 @@ k_rho_fwd_x_r(const T *__restrict__ in, cpx<T> *__restrict__ spec, const cpx<T> 
 #pragma unroll
         for (int q = 0; q <= E / 2; ++q) {
             const int k = j + q * TPP;
-            if (q < E / 2 || k == nx) out[k] = v[q];
+            if (q < E / 2 || k == nx) gst(out + k, v[q]);
         }
     }
 }
@@ -92,7 +92,7 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d,
     const SBunch *sb;
     const int bx = group_block(tab, per, sb);
     if (sb) A = static_cast<cpx<T> *>(sb->fwd);
-    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = gld(twg + i);
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kx = bx * NB + b, z = blockIdx.y;
     const int Kx = d.Kx, Kp = d.Kp;
@@ -102,12 +102,12 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d,
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         v[q] = cpx<T>{(T)0, (T)0};
-        if (q < E / 2 && live) v[q] = Ap[(size_t)(j + q * TPP) * Kp];
+        if (q < E / 2 && live) v[q] = gld(Ap + (size_t)(j + q * TPP) * Kp);
     }
     reg::fft<T, L, E, -1>(v, j, lds + b, NB, tw);
     if (live) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) Ap[(size_t)(j + q * TPP) * Kp] = v[q];
+        for (int q = 0; q < E; ++q) gst(Ap + (size_t)(j + q * TPP) * Kp, v[q]);
     }
 }
 
@@ -153,7 +153,7 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
     const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
-    for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = gld(twg + i);
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kx = bx * NB + b;
     const bool live = kx < d.Kx;
@@ -166,9 +166,9 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
         pi[q] = cpx<T>{(T)0, (T)0};
         if (q < E / 2 && live) {
             const int z = j + q * TPP;
-            p[q] = Ap[(size_t)z * zs];
+            p[q] = gld(Ap + (size_t)z * zs);
             if constexpr (CATH) {
-                const cpx<T> t = Ap[(size_t)(nz - 1 - z) * zs];
+                const cpx<T> t = gld(Ap + (size_t)(nz - 1 - z) * zs);
                 pi[q] = cpx<T>{-t.x, -t.y};
             }
         }
@@ -190,13 +190,13 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
             const bool kzneg = kz > nz;
             const int kzp = kzneg ? L - kz : kz;
             const T sz = (!POT && c == 2 && kzneg) ? (T)-1 : (T)1;
-            const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kxc];
+            const T g = sy * sz * gld(Gf + c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kxc);
             if constexpr (POT)
                 w[q] = cpx<T>{p[q].x * g, p[q].y * g};  // rho_hat * g
             else
                 w[q] = cpx<T>{-p[q].y * g, p[q].x * g};  // rho_hat * (i g)
             if constexpr (CATH) {
-                cpx<T> gi = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kxc];
+                cpx<T> gi = gld(Gi + c * MI + ((size_t)kz * ny1 + kyp) * Kp + kxc);
                 gi.x *= sy;
                 gi.y *= sy;
                 if (c < 2) gi = cpx<T>{-gi.y, gi.x};
@@ -207,7 +207,7 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
         if (live) {
             cpx<T> *Bc = B + (size_t)c * nz * zs + (size_t)ky * Kp + kx;
 #pragma unroll
-            for (int q = 0; q < E / 2; ++q) Bc[(size_t)(j + q * TPP) * zs] = w[q];
+            for (int q = 0; q < E / 2; ++q) gst(Bc + (size_t)(j + q * TPP) * zs, w[q]);
         }
     }
 }
@@ -226,7 +226,7 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
     __shared__ cpx<T> tw[L];
     __shared__ cpx<T> lds[NB * L];
     const int Kp = d.Kp, Ly = d.Ly, ny = d.ny, nz = d.nz, ny1 = ny + 1;
-    for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += THREADS) tw[i] = gld(twg + i);
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     // free space: a 1-D grid of div_up(Kx, NB) * Ly workgroups in XCD order;
     // with the cathode image a grid (div_up(Kx, NB), Ly), as in k_zconv_r
@@ -258,7 +258,7 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
 #pragma unroll
         for (int q = 0; q < E; ++q) {
             p[q] = cpx<T>{(T)0, (T)0};
-            if (q < E / 2 && live) p[q] = Gp[(size_t)(j + q * TPP) * zs];
+            if (q < E / 2 && live) p[q] = gld(Gp + (size_t)(j + q * TPP) * zs);
         }
         reg::fft<T, L, E, -1>(p, j, lds + b, NB, tw);
         const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
@@ -268,10 +268,10 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
             const bool kzneg = kz > nz;
             const int kzp = kzneg ? L - kz : kz;
             const T sz = (c == 2 && kzneg) ? (T)-1 : (T)1;
-            const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kxc];
+            const T g = sy * sz * gld(Gf + c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kxc);
             w[q] = c_add(w[q], cpx<T>{p[q].y * g, -p[q].x * g});  // p * conj(i g)
             if constexpr (CATH) {
-                cpx<T> gi = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kxc];
+                cpx<T> gi = gld(Gi + c * MI + ((size_t)kz * ny1 + kyp) * Kp + kxc);
                 gi.x *= sy;
                 gi.y *= sy;
                 if (c < 2) gi = cpx<T>{-gi.y, gi.x};
@@ -293,7 +293,7 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
     if (live) {
         cpx<T> *o = out + (size_t)ky * Kp + kx;
 #pragma unroll
-        for (int q = 0; q < E / 2; ++q) o[(size_t)(j + q * TPP) * zs] = w[q];
+        for (int q = 0; q < E / 2; ++q) gst(o + (size_t)(j + q * TPP) * zs, w[q]);
     }
 }
 
@@ -310,7 +310,7 @@ k_inv_y_r(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, Dims d,
     const SBunch *sb;
     const int bx = group_block(tab, per, sb);
     if (sb) B = static_cast<cpx<T> *>(sb->inv);
-    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = gld(twg + i);
     const int b = threadIdx.x % NB, j = threadIdx.x / NB;
     const int kx = bx * NB + b, z = blockIdx.y, c = blockIdx.z;
     const int Kx = d.Kx, Kp = d.Kp;
@@ -318,11 +318,11 @@ k_inv_y_r(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, Dims d,
     cpx<T> *Bp = B + ((size_t)c * d.nz + z) * d.Ly * Kp + (live ? kx : 0);
     cpx<T> v[E];
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = Bp[(size_t)(j + q * TPP) * Kp];
+    for (int q = 0; q < E; ++q) v[q] = gld(Bp + (size_t)(j + q * TPP) * Kp);
     reg::fft<T, L, E, +1>(v, j, lds + b, NB, tw);
     if (live) {
 #pragma unroll
-        for (int q = 0; q < E / 2; ++q) Bp[(size_t)(j + q * TPP) * Kp] = v[q];
+        for (int q = 0; q < E / 2; ++q) gst(Bp + (size_t)(j + q * TPP) * Kp, v[q]);
     }
 }
 
@@ -342,7 +342,7 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
         B = static_cast<const cpx<T> *>(sb->inv);
         E_ = static_cast<T *>(sb->out);
     }
-    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = twg[i];
+    for (int i = threadIdx.x; i < L; i += kThreads) tw[i] = gld(twg + i);
     const int b = threadIdx.x / TPP, j = threadIdx.x % TPP;
     const int r = bx * NB + b, c = blockIdx.y;
     const int nrows = d.ny * d.nz, nx = d.nx;
@@ -357,7 +357,7 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
     for (int q = 0; q < E; ++q) {
         const int k = j + q * TPP;
         const bool direct = q < E / 2 || k == nx;
-        v[q] = row[direct ? k : L - k];
+        v[q] = gld(row + (direct ? k : L - k));
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -368,7 +368,7 @@ k_inv_x_r(const cpx<T> *__restrict__ B, T *__restrict__ E_, const cpx<T> *__rest
     if (live) {
         T *out = E_ + (size_t)c * nx * nrows + (size_t)r * nx;
 #pragma unroll
-        for (int q = 0; q < E / 2; ++q) out[j + q * TPP] = v[q].x;
+        for (int q = 0; q < E / 2; ++q) gst(out + j + q * TPP, v[q].x);
     }
 }
 
