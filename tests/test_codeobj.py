@@ -1,4 +1,4 @@
-"""CPU: register and scratch use of the library's gfx950 kernels.
+"""CPU: register, scratch and memory-instruction use of the library's gfx950 kernels.
 
 Reads the AMDGPU metadata of the code objects built into
 libspacecharge_mi355.so (one offload bundle per translation unit in its
@@ -31,6 +31,28 @@ GENERIC_F64 = re.compile(r"^_ZN2sc12_GLOBAL__N_1\d+k_(rho_fwd_y|zconv|zconv_adj|
 def _tool(name):
     path = os.path.join(LLVM, name)
     return path if os.path.exists(path) else None
+
+
+def _code_objects(tmp_path):
+    """The gfx950 code objects of the library's offload bundles (paths)."""
+    objcopy, bundler = (_tool(n) for n in ("llvm-objcopy", "clang-offload-bundler"))
+    if not (objcopy and bundler) or not os.path.exists(LIB):
+        pytest.skip("ROCm LLVM tools or the built library not available")
+    fat = tmp_path / "fat.bin"
+    subprocess.run([objcopy, "--dump-section", f".hip_fatbin={fat}", LIB], check=True)
+    data = fat.read_bytes()
+    starts = [m.start() for m in re.finditer(re.escape(MAGIC), data)]
+    assert starts, "no offload bundle in .hip_fatbin"
+    out = []
+    for i, s in enumerate(starts):
+        e = starts[i + 1] if i + 1 < len(starts) else len(data)
+        part = tmp_path / f"c{i}.bin"
+        part.write_bytes(data[s:e])
+        co = tmp_path / f"c{i}.co"
+        subprocess.run([bundler, "--unbundle", "--type=o", f"--input={part}", f"--targets={TARGET}",
+                        f"--output={co}"], check=True)
+        out.append(co)
+    return out
 
 
 def _kernels(tmp_path):
@@ -78,4 +100,32 @@ def test_no_spills_and_no_scratch(tmp_path):
         elif f["private_segment_fixed_size"]:
             if not (GENERIC_F64.match(name) and f["private_segment_fixed_size"] <= 32):
                 bad.append((name, f))
+    assert not bad, bad
+
+
+# Kernels that may take their buffers from a descriptor table (grouped
+# launches) but run the single-mesh solve too: their loads and stores must be
+# global, not flat (flat ones also count against the LDS wait counter, so the
+# transforms' LDS exchanges would wait for memory operations in flight).
+GLOBAL_ONLY = re.compile(r"k_(rho_fwd_x_r|rho_fwd_y_r|zconv_r|inv_y_r|inv_x_r|green_1r|green_2r|"
+                         r"green_3r|green_f3|green_f)I")
+
+
+def test_transform_and_green_kernels_use_global_memory_instructions(tmp_path):
+    objdump = _tool("llvm-objdump")
+    if not objdump:
+        pytest.skip("llvm-objdump not available")
+    seen, bad = 0, []
+    for co in _code_objects(tmp_path):
+        asm = subprocess.run([objdump, "-d", "--no-show-raw-insn", str(co)], check=True,
+                             capture_output=True, text=True).stdout
+        for block in re.split(r"\n(?=[0-9a-f]+ <)", asm):
+            head = re.match(r"[0-9a-f]+ <(_Z\S+)>:", block)
+            if not head or not GLOBAL_ONLY.search(head.group(1)):
+                continue
+            seen += 1
+            n = len(re.findall(r"\bflat_(load|store)", block))
+            if n:
+                bad.append((head.group(1), n))
+    assert seen >= 40, seen
     assert not bad, bad
