@@ -3,7 +3,8 @@
 // A workgroup of kFftThreads lanes transforms a batch of nb pencils of one
 // length n held in LDS, each lane holding up to E complex values of a stage
 // (nb*n <= E*kFftThreads; E = fft_elems<T>() by default, twice that for the
-// long pencils of axes beyond fft_cap<T>()).  Element e of pencil b lives at buf[e*es + b*ps]:
+// long pencils of axes beyond fft_cap<T>()).  Element e of pencil b lives
+// at buf[e*es + b*ps]:
 //   es = nb, ps = 1  ("element-major") for pencils that are strided in HBM
 //                    and batched over the contiguous axis, so a global
 //                    load of one element row is one coalesced segment;
