@@ -115,9 +115,10 @@ k_rho_fwd_y_r(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, Dims d,
 // a SolveMode.  Free space: three waves per SIMD; the forward pencil stays
 // in registers across the three inverse transforms, and unconstrained the
 // kernel takes 169 VGPRs (two waves); 168 fit without spilling.  With the
-// cathode image the z-reversed pencil's transform is held as well, which
-// does not fit in 168 VGPRs (held to it, the compiler spilled 25-62 VGPRs
-// to scratch), so that mode runs two waves per SIMD.
+// cathode image the z-reversed pencil's spectrum is held as well (formed
+// from the forward spectrum, not transformed), which does not fit in 168
+// VGPRs (held to it, the compiler spilled 25-62 VGPRs to scratch), so that
+// mode runs two waves per SIMD.
 template <typename T, int L, int MODE, int THREADS>
 __global__ void __launch_bounds__(THREADS)
 __attribute__((amdgpu_waves_per_eu(MODE == kFieldCathode ? 2 : 3)))
@@ -163,18 +164,25 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
 #pragma unroll
     for (int q = 0; q < E; ++q) {
         p[q] = cpx<T>{(T)0, (T)0};
-        pi[q] = cpx<T>{(T)0, (T)0};
-        if (q < E / 2 && live) {
-            const int z = j + q * TPP;
-            p[q] = gld(Ap + (size_t)z * zs);
-            if constexpr (CATH) {
-                const cpx<T> t = gld(Ap + (size_t)(nz - 1 - z) * zs);
-                pi[q] = cpx<T>{-t.x, -t.y};
-            }
-        }
+        if (q < E / 2 && live) p[q] = gld(Ap + (size_t)(j + q * TPP) * zs);
     }
     reg::fft<T, L, E, -1>(p, j, lds + b, NB, tw);
-    if constexpr (CATH) reg::fft<T, L, E, -1>(pi, j, lds + b, NB, tw);
+    if constexpr (CATH) {
+        // The image -rho(nz - 1 - z) (z < nz, zero-padded) has the spectrum
+        // -w_L^((nz-1) k) P[-k]; P[-k] is held by another lane, so the
+        // spectrum goes through LDS once instead of a second transform.
+        __syncthreads();  // every lane past the transform's last exchange
+#pragma unroll
+        for (int q = 0; q < E; ++q) lds[b + (j + q * TPP) * NB] = p[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+            const int k = j + q * TPP;
+            const cpx<T> m = lds[b + ((L - k) & (L - 1)) * NB];
+            const cpx<T> r = c_mul(tw[((nz - 1) * k) & (L - 1)], m);
+            pi[q] = cpx<T>{-r.x, -r.y};
+        }
+    }
     const bool kyneg = ky > ny;
     const int kyp = kyneg ? Ly - ky : ky;
     const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;
