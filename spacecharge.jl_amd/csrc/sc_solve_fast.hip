@@ -222,8 +222,9 @@ k_zconv_r(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restr
 
 // Transposed z convolution (see k_zconv_adj in sc_solve.hip): the three g_E
 // spectra in Gs are transformed one at a time and their products with
-// conj(G_c^) summed in registers; the image sum is reversed in z through
-// LDS before it is subtracted.
+// conj(G_c^) summed in registers; the image sum, reversed in z, is
+// subtracted in the spectrum (a phase factor and the mirror bin, through
+// LDS), so one backward transform serves both.
 template <typename T, int L, bool CATH, int THREADS>
 __global__ void __launch_bounds__(THREADS)
 k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__restrict__ Gf,
@@ -287,17 +288,22 @@ k_zconv_adj_r(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *
             }
         }
     }
-    reg::fft<T, L, E, +1>(w, j, lds + b, NB, tw);
     if constexpr (CATH) {
-        reg::fft<T, L, E, +1>(wi, j, lds + b, NB, tw);
-        // out[z] = w[z] - wi[nz-1-z]: the reversed image sum lives in other lanes
-        __syncthreads();
+        // out[z] = IFFT(W)[z] - IFFT(WI)[nz-1-z].  The reversed transform is
+        // IFFT(U) with U[k] = WI[-k] w_L^((nz-1) k), so one backward
+        // transform of W - U gives both; WI[-k] is held by another lane.
+        __syncthreads();  // every lane past the last forward transform's exchange
 #pragma unroll
         for (int q = 0; q < E; ++q) lds[b + (j + q * TPP) * NB] = wi[q];
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < E / 2; ++q) w[q] = c_sub(w[q], lds[b + (nz - 1 - (j + q * TPP)) * NB]);
+        for (int q = 0; q < E; ++q) {
+            const int k = j + q * TPP;
+            const cpx<T> m = lds[b + ((L - k) & (L - 1)) * NB];
+            w[q] = c_sub(w[q], c_mul(tw[((nz - 1) * k) & (L - 1)], m));
+        }
     }
+    reg::fft<T, L, E, +1>(w, j, lds + b, NB, tw);
     if (live) {
         cpx<T> *o = out + (size_t)ky * Kp + kx;
 #pragma unroll
