@@ -107,10 +107,11 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
     }
 }
 
-// Fused z convolution.  Per (kx, ky): forward z transform of rho (and of the
-// image charge -rho[.., nz-1-z]), products with the three Green spectra,
-// backward z transforms, z < nz of each field spectrum to B.  The potential
-// (mode kPotential) has one Green spectrum, real and even in every axis.
+// Fused z convolution.  Per (kx, ky): forward z transform of rho,
+// products with the three Green spectra (with the cathode also the image
+// charge -rho[.., mz-1-z], whose spectrum follows from rho's), backward z
+// transforms, z < nz of each field spectrum to B.  The potential (mode
+// kPotential) has one Green spectrum, real and even in every axis.
 template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
 k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
@@ -122,7 +123,6 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *P0 = tw + d.Lz;
     cpx<T> *W = P0 + nb * d.Lz;
-    cpx<T> *P1 = W + nb * d.Lz;  // only with the cathode
     // nz, ny: transform half-lengths (spectral octant); mz: mesh z nodes
     const int Lz = d.Lz, nz = d.nz, mz = d.mz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
     const int ny1 = ny + 1;
@@ -132,20 +132,12 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
     for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
-        cpx<T> v{(T)0, (T)0}, vi{(T)0, (T)0};
-        if (kx < Kx && e < mz) {
-            v = A[(size_t)e * zs + (size_t)ky * Kp + kx];
-            if (cathode) {
-                const cpx<T> t = A[(size_t)(mz - 1 - e) * zs + (size_t)ky * Kp + kx];
-                vi = cpx<T>{-t.x, -t.y};
-            }
-        }
+        cpx<T> v{(T)0, (T)0};
+        if (kx < Kx && e < mz) v = A[(size_t)e * zs + (size_t)ky * Kp + kx];
         P0[idx] = v;
-        if (cathode) P1[idx] = vi;
     }
     __syncthreads();
     lds_fft<T, E>(P0, tw, pl, nb, nb, 1, true, -1);
-    if (cathode) lds_fft<T, E>(P1, tw, pl, nb, nb, 1, true, -1);
     const bool kyneg = ky > ny;
     const int kyp = kyneg ? Ly - ky : ky;
     const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;  // one free-space Green spectrum
@@ -170,7 +162,11 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
                     q.x *= sy;
                     q.y *= sy;
                     if (c < 2) q = cpx<T>{-q.y, q.x};  // times i (odd in x or y)
-                    out = c_add(out, c_mul(P1[idx], q));
+                    // the image -rho(mz - 1 - z) (z < mz, zero-padded) has the
+                    // spectrum -w_Lz^((mz-1) kz) P[-kz]: no transform of its own
+                    const int km = kz == 0 ? 0 : Lz - kz;
+                    const cpx<T> r = c_mul(tw[(int)(((int64_t)(mz - 1) * kz) % Lz)], P0[km * nb + b]);
+                    out = c_add(out, c_mul(cpx<T>{-r.x, -r.y}, q));
                 }
             }
             W[idx] = out;
@@ -457,7 +453,9 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     if (!fast(2)) {
         with_fft_elems<T>(d.Lz, [&](auto e) {
             constexpr int E = decltype(e)::value;
-            const int nbuf = cath ? 3 : 2;
+            // LDS buffers of nb pencils: the spectrum and the product (and
+            // the transposed cathode solve's image sum)
+            const int nbuf = (cath && mode_adjoint(mode)) ? 3 : 2;
             // at most 64 KiB of LDS where more pencils fit (occupancy); one
             // long pencil may take more (up to the device's 160 KiB)
             int nb = pick_nb(d.Lz, E * kFftThreads, 16);
