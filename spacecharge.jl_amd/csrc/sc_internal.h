@@ -74,8 +74,8 @@ constexpr int fft_cap()
 }
 // Pencils longer than fft_cap (axes of more than fft_cap / 2 nodes) take
 // twice the values per lane, one pencil per workgroup, and LDS buffers of
-// up to 4 pencils (the z convolution with the cathode image), which the
-// device's 160 KiB per workgroup holds up to fft_cap_long.
+// up to 3 pencils (the z convolution: twiddles, spectrum, product), which
+// the device's 160 KiB per workgroup holds up to fft_cap_long.
 template <typename T>
 constexpr int fft_elems_long()
 {

@@ -184,9 +184,10 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
 }
 
 // Transposed z convolution.  Per (kx, ky): forward z transform of each of
-// the three g_E spectra in Gs, product with conj(G_c^) summed over c (and,
-// with the cathode, with conj(Gi_c^) into a second sum), backward z
-// transforms, and for z < nz: out[z] = W[z] - Wi[nz-1-z].
+// the three g_E spectra in Gs, product with conj(G_c^) summed over c, and
+// with the cathode minus the image sum Wi = sum_c p conj(Gi_c^) reversed in
+// z (out[z] = W[z] - Wi[mz-1-z] after the backward transform, formed in the
+// spectrum), one backward z transform, z < mz to out.
 template <typename T, int E = fft_elems<T>()>
 __global__ void __launch_bounds__(kFftThreads)
 k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__restrict__ Gf,
@@ -197,7 +198,6 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
     cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
     cpx<T> *Q = tw + d.Lz;
     cpx<T> *W = Q + nb * d.Lz;
-    cpx<T> *W1 = W + nb * d.Lz;  // only with the cathode
     // nz, ny: transform half-lengths (spectral octant); mz: mesh z nodes
     const int Lz = d.Lz, nz = d.nz, mz = d.mz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
     const int ny1 = ny + 1;
@@ -208,10 +208,7 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
     const int kyp = kyneg ? Ly - ky : ky;
     const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;
     const size_t MI = (size_t)Lz * ny1 * Kp;
-    for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
-        W[idx] = cpx<T>{(T)0, (T)0};
-        if (cathode) W1[idx] = cpx<T>{(T)0, (T)0};
-    }
+    for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) W[idx] = cpx<T>{(T)0, (T)0};
     for (int c = 0; c < 3; ++c) {
         const cpx<T> *Gc = Gs + (size_t)c * mz * zs;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
@@ -234,29 +231,31 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
                 const T g = sy * sz * Gf[c * M1 + ((size_t)kzp * ny1 + kyp) * Kp + kx];
                 const cpx<T> p = Q[idx];
                 // p * conj(i g) = p * (-i g)
-                W[idx] = c_add(W[idx], cpx<T>{p.y * g, -p.x * g});
+                cpx<T> acc = c_add(W[idx], cpx<T>{p.y * g, -p.x * g});
                 if (cathode) {
-                    cpx<T> q = Gi[c * MI + ((size_t)kz * ny1 + kyp) * Kp + kx];
+                    // the image sum W1 = sum_c p conj(q), reversed in z after its
+                    // backward transform, is the backward transform of
+                    // U[k] = W1[-k] w_Lz^((mz-1) k): its terms, taken at the
+                    // mirror bin, go into W (one backward transform for both)
+                    const int km = kz == 0 ? 0 : Lz - kz;
+                    cpx<T> q = Gi[c * MI + ((size_t)km * ny1 + kyp) * Kp + kx];
                     q.x *= sy;
                     q.y *= sy;
                     if (c < 2) q = cpx<T>{-q.y, q.x};
-                    W1[idx] = c_add(W1[idx], c_mul(p, cpx<T>{q.x, -q.y}));
+                    const cpx<T> t = c_mul(Q[km * nb + b], cpx<T>{q.x, -q.y});
+                    acc = c_sub(acc, c_mul(tw[(int)(((int64_t)(mz - 1) * kz) % Lz)], t));
                 }
+                W[idx] = acc;
             }
         }
         __syncthreads();
     }
     lds_fft<T, E>(W, tw, pl, nb, nb, 1, true, +1);
-    if (cathode) lds_fft<T, E>(W1, tw, pl, nb, nb, 1, true, +1);
     __syncthreads();
     for (int idx = threadIdx.x; idx < nb * mz; idx += blockDim.x) {
         const int z = idx / nb, b = idx - z * nb;
         const int kx = kx0 + b;
-        if (kx < Kx) {
-            cpx<T> v = W[idx];
-            if (cathode) v = c_sub(v, W1[(mz - 1 - z) * nb + b]);
-            out[(size_t)z * zs + (size_t)ky * Kp + kx] = v;
-        }
+        if (kx < Kx) out[(size_t)z * zs + (size_t)ky * Kp + kx] = W[idx];
     }
 }
 
@@ -453,9 +452,8 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     if (!fast(2)) {
         with_fft_elems<T>(d.Lz, [&](auto e) {
             constexpr int E = decltype(e)::value;
-            // LDS buffers of nb pencils: the spectrum and the product (and
-            // the transposed cathode solve's image sum)
-            const int nbuf = (cath && mode_adjoint(mode)) ? 3 : 2;
+            // LDS buffers of nb pencils: the spectrum and the product
+            const int nbuf = 2;
             // at most 64 KiB of LDS where more pencils fit (occupancy); one
             // long pencil may take more (up to the device's 160 KiB)
             int nb = pick_nb(d.Lz, E * kFftThreads, 16);
