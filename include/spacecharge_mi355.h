@@ -190,8 +190,8 @@ int sc_deposit(sc_mesh *mesh, int pdtype, const void *x, const void *y, const vo
  * at_cathode != 0 the image charge -rho flipped in z, at z-offset
  * min_bounds[2] + max_bounds[2] (:34-45), is added.  The Green function is
  * evaluated and differenced in Float64 for both mesh types; the transforms
- * and products run in the mesh type.  Any axis of 2 to 2048 nodes on a
- * Float32 mesh (2 to 1024 on a Float64 mesh) is solved; beyond that the
+ * and products run in the mesh type.  Any axis of 2 to 4096 nodes on a
+ * Float32 mesh (2 to 2048 on a Float64 mesh) is solved; beyond that the
  * call returns SC_ERR_UNSUPPORTED (one transform pencil per workgroup). */
 int sc_solve(sc_mesh *mesh, const double min_bounds[3], const double max_bounds[3],
              const double delta[3], double gamma, int at_cathode);

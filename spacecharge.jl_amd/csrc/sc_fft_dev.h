@@ -1,9 +1,10 @@
 // sc_fft_dev.h -- workgroup-cooperative mixed-radix FFT in LDS (gfx950).
 //
-// A workgroup of kFftThreads lanes transforms a batch of nb pencils of one
-// length n held in LDS, each lane holding up to E complex values of a stage
-// (nb*n <= E*kFftThreads; E = fft_elems<T>() by default, twice that for the
-// long pencils of axes beyond fft_cap<T>()).  Element e of pencil b lives
+// A workgroup of NT lanes (kFftThreads, or kFftThreadsXL for the longest
+// pencils) transforms a batch of nb pencils of one length n held in LDS,
+// each lane holding up to E complex values of a stage (nb*n <= E*NT; E =
+// fft_elems<T>() by default, twice that for the long pencils of axes beyond
+// fft_cap<T>()).  Element e of pencil b lives
 // at buf[e*es + b*ps]:
 //   es = nb, ps = 1  ("element-major") for pencils that are strided in HBM
 //                    and batched over the contiguous axis, so a global
@@ -148,7 +149,7 @@ __device__ __forceinline__ void dft_prime_store(const cpx<T> *v, const cpx<T> *t
     }
 }
 
-template <typename T, int R, int E>
+template <typename T, int R, int E, int NT>
 __device__ __forceinline__ void fft_stage(cpx<T> *buf, const cpx<T> *tw, int n, int Ns, int nb,
                                           int es, int ps, bool bfast, int sign)
 {
@@ -159,7 +160,7 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, const cpx<T> *tw, int n, 
     cpx<T> v[MAXB][R];
 #pragma unroll
     for (int s = 0; s < MAXB; ++s) {
-        const int bf = (int)threadIdx.x + s * kFftThreads;
+        const int bf = (int)threadIdx.x + s * NT;
         if (bf < TB) {
             int b, j;
             if (bfast) { b = bf % nb; j = bf / nb; }
@@ -171,7 +172,7 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, const cpx<T> *tw, int n, 
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < MAXB; ++s) {
-        const int bf = (int)threadIdx.x + s * kFftThreads;
+        const int bf = (int)threadIdx.x + s * NT;
         if (bf < TB) {
             int b, j;
             if (bfast) { b = bf % nb; j = bf / nb; }
@@ -200,10 +201,10 @@ __device__ __forceinline__ void fft_stage(cpx<T> *buf, const cpx<T> *tw, int n, 
     __syncthreads();
 }
 
-// Transform nb pencils of length pl.n in LDS.  All lanes of the workgroup
-// must call it (it contains barriers).  Requires nb*n <= E*kFftThreads,
-// which the host guarantees.
-template <typename T, int E = fft_elems<T>()>
+// Transform nb pencils of length pl.n in LDS.  All NT lanes of the
+// workgroup must call it (it contains barriers).  Requires nb*n <= E*NT,
+// which the host guarantees.  tw may be in LDS or in global memory.
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
 __device__ void lds_fft(cpx<T> *buf, const cpx<T> *tw, const FftPlan &pl, int nb, int es, int ps,
                         bool bfast, int sign)
 {
@@ -212,12 +213,12 @@ __device__ void lds_fft(cpx<T> *buf, const cpx<T> *tw, const FftPlan &pl, int nb
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.rad[st];
         switch (R) {
-        case 2: fft_stage<T, 2, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 3: fft_stage<T, 3, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 4: fft_stage<T, 4, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 5: fft_stage<T, 5, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 7: fft_stage<T, 7, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
-        case 8: fft_stage<T, 8, E>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 2: fft_stage<T, 2, E, NT>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 3: fft_stage<T, 3, E, NT>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 4: fft_stage<T, 4, E, NT>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 5: fft_stage<T, 5, E, NT>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 7: fft_stage<T, 7, E, NT>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
+        case 8: fft_stage<T, 8, E, NT>(buf, tw, n, Ns, nb, es, ps, bfast, sign); break;
         default: break;  // unreachable: make_plan rejects other radices
         }
         Ns *= R;

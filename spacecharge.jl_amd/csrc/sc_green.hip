@@ -361,8 +361,8 @@ __device__ __forceinline__ double diff8(const double *__restrict__ F, int i, int
     return diff8v(p[0], p[1], p[sz], p[1 + sz], p[sy], p[1 + sy], p[sy + sz], p[1 + sy + sz]);
 }
 
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restrict__ twg,
           FftPlan pl, GreenArgs a, int nb)
 {
@@ -400,7 +400,7 @@ k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restr
         buf[p * Lx + e] = v;
     }
     __syncthreads();
-    lds_fft<T, E>(buf, tw, pl, nb, 1, Lx, false, -1);
+    lds_fft<T, E, NT>(buf, tw, pl, nb, 1, Lx, false, -1);
     for (int idx = threadIdx.x; idx < nb * Kx; idx += blockDim.x) {
         const int p = idx / Kx, kx = idx - p * Kx;
         const int P = p0 + p;
@@ -414,8 +414,8 @@ k_green_1(const double *__restrict__ F, T *__restrict__ R, const cpx<T> *__restr
     }
 }
 
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_green_2(T *__restrict__ R, const cpx<T> *__restrict__ twg, FftPlan pl, GreenArgs a, int nb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -447,7 +447,7 @@ k_green_2(T *__restrict__ R, const cpx<T> *__restrict__ twg, FftPlan pl, GreenAr
         buf[e * nb + p] = v;
     }
     __syncthreads();
-    lds_fft<T, E>(buf, tw, pl, nb, nb, 1, true, -1);
+    lds_fft<T, E, NT>(buf, tw, pl, nb, nb, 1, true, -1);
     for (int idx = threadIdx.x; idx < nb * (ny + 1); idx += blockDim.x) {
         const int ky = idx / nb, p = idx - ky * nb;
         const int kxa = 2 * (kp0 + p), kxb = kxa + 1;
@@ -458,8 +458,8 @@ k_green_2(T *__restrict__ R, const cpx<T> *__restrict__ twg, FftPlan pl, GreenAr
     }
 }
 
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
           const cpx<T> *__restrict__ twg, FftPlan pl, GreenArgs a, int nb)
 {
@@ -504,7 +504,7 @@ k_green_3(const T *__restrict__ R, T *__restrict__ Gf, cpx<T> *__restrict__ Gi,
         }
         __syncthreads();
     }
-    lds_fft<T, E>(buf, tw, pl, nb, nb, 1, true, -1);
+    lds_fft<T, E, NT>(buf, tw, pl, nb, nb, 1, true, -1);
     const T scale = (T)green_scale(a, c);
     if (!image) {
         T *G = Gf + (size_t)(q % 3) * (nz + 1) * ny1 * Kp;
@@ -886,13 +886,13 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
     const bool fast_ok = !m->generic_transforms;
     launch_begin(ctx, K_GREEN_X);
     if (!(fast_ok && green_pass_fast<T>(1, d.Lx, ncomp, d, cath, st, ws, a, twx))) {
-        with_fft_elems<T>(d.Lx, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            const int nb = pick_nb(d.Lx, E * kFftThreads, 16);
+        with_fft_elems<T>(d.Lx, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            const int nb = pick_nb(d.Lx, E * NT, 16);
             const int zc = cath ? d.Lz - 1 : d.nz;
             const int np = zc * ((d.ny + 1) / 2);
             const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
-            hipLaunchKernelGGL((k_green_1<T, E>), dim3(div_up(np, nb), ncomp), dim3(kFftThreads), sh,
+            hipLaunchKernelGGL((k_green_1<T, E, NT>), dim3(div_up(np, nb), ncomp), dim3(NT), sh,
                                st, (const double *)ws.F, R, twx, ws.plan[0], a, nb);
         });
     }
@@ -900,24 +900,24 @@ int build_green(Mesh *m, const double dl[3], double gamma, double offz, int mode
     const int kp = (d.Kx + 1) / 2;
     launch_begin(ctx, K_GREEN_Y);
     if (!(fast_ok && green_pass_fast<T>(2, d.Ly, ncomp, d, cath, st, ws, a, twy))) {
-        with_fft_elems<T>(d.Ly, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            const int nb = pick_nb(d.Ly, E * kFftThreads, 16);
+        with_fft_elems<T>(d.Ly, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            const int nb = pick_nb(d.Ly, E * NT, 16);
             const int zc = cath ? d.Lz - 1 : d.nz;
             const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
-            hipLaunchKernelGGL((k_green_2<T, E>), dim3(div_up(kp, nb), zc, ncomp), dim3(kFftThreads),
+            hipLaunchKernelGGL((k_green_2<T, E, NT>), dim3(div_up(kp, nb), zc, ncomp), dim3(NT),
                                sh, st, R, twy, ws.plan[1], a, nb);
         });
     }
     launch_end(ctx, K_GREEN_Y);
     launch_begin(ctx, K_GREEN_Z);
     if (!(fast_ok && green_pass_fast<T>(3, d.Lz, ncomp, d, cath, st, ws, a, twz))) {
-        with_fft_elems<T>(d.Lz, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            const int nb = pick_nb(d.Lz, E * kFftThreads, 16);
+        with_fft_elems<T>(d.Lz, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            const int nb = pick_nb(d.Lz, E * NT, 16);
             const size_t sh = sizeof(cpx<T>) * (d.Lz + (size_t)nb * d.Lz);
-            hipLaunchKernelGGL((k_green_3<T, E>), dim3(div_up(kp, nb), d.ny + 1, ncomp),
-                               dim3(kFftThreads), sh, st, (const T *)R, static_cast<T *>(ws.Gf),
+            hipLaunchKernelGGL((k_green_3<T, E, NT>), dim3(div_up(kp, nb), d.ny + 1, ncomp),
+                               dim3(NT), sh, st, (const T *)R, static_cast<T *>(ws.Gf),
                                static_cast<cpx<T> *>(ws.Gi), twz, ws.plan[2], a, nb);
         });
     }

@@ -86,13 +86,28 @@ constexpr int fft_cap_long()
 {
     return fft_elems_long<T>() * kFftThreads;
 }
-// Calls f(std::integral_constant<int, E>{}) with the values per lane for
-// pencils of length L.
+// Pencils longer than fft_cap_long run the same values per lane on
+// workgroups of kFftThreadsXL lanes, up to fft_cap_xl (8192 Float32 or 4096
+// Float64 points, 64 KiB per pencil buffer).  The z convolution then reads
+// its twiddles from global memory: its two pencil buffers take 128 KiB.
+constexpr int kFftThreadsXL = 512;
+template <typename T>
+constexpr int fft_cap_xl()
+{
+    return fft_elems_long<T>() * kFftThreadsXL;
+}
+// Calls f(std::integral_constant<int, E>{}, std::integral_constant<int, NT>{})
+// with the values per lane and the workgroup size for pencils of length L.
 template <typename T, typename F>
 void with_fft_elems(int L, F &&f)
 {
-    if (L <= fft_cap<T>()) f(std::integral_constant<int, fft_elems<T>()>{});
-    else f(std::integral_constant<int, fft_elems_long<T>()>{});
+    using std::integral_constant;
+    if (L <= fft_cap<T>())
+        f(integral_constant<int, fft_elems<T>()>{}, integral_constant<int, kFftThreads>{});
+    else if (L <= fft_cap_long<T>())
+        f(integral_constant<int, fft_elems_long<T>()>{}, integral_constant<int, kFftThreads>{});
+    else
+        f(integral_constant<int, fft_elems_long<T>()>{}, integral_constant<int, kFftThreadsXL>{});
 }
 constexpr int kFftMaxRadix = 7;
 

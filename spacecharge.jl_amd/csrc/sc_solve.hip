@@ -46,8 +46,8 @@ namespace {
 // ---------------------------------------------------------------------------
 // x pass of the real input: rows (y, z) of component blockIdx.y, zero-padded
 // to 2nx, kx in [0, nx] to that component's spectrum.
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__restrict__ twg,
             FftPlan pl, Dims d, int nb)
 {
@@ -67,7 +67,7 @@ k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__
         buf[idx] = cpx<T>{v, (T)0};
     }
     __syncthreads();
-    lds_fft<T, E>(buf, tw, pl, nb, 1, Lx, false, -1);
+    lds_fft<T, E, NT>(buf, tw, pl, nb, 1, Lx, false, -1);
     for (int idx = threadIdx.x; idx < nb * d.Kx; idx += blockDim.x) {
         const int b = idx / d.Kx, kx = idx - b * d.Kx;
         const int r = r0 + b;
@@ -80,8 +80,8 @@ k_rho_fwd_x(const T *__restrict__ in, cpx<T> *__restrict__ out, const cpx<T> *__
 
 // y pass: pencils (kx, z < nz) of component blockIdx.z along y, ny inputs
 // zero-padded to 2ny.
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -99,7 +99,7 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
         buf[idx] = v;
     }
     __syncthreads();
-    lds_fft<T, E>(buf, tw, pl, nb, nb, 1, true, -1);
+    lds_fft<T, E, NT>(buf, tw, pl, nb, nb, 1, true, -1);
     for (int idx = threadIdx.x; idx < nb * Ly; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
@@ -112,21 +112,26 @@ k_rho_fwd_y(cpx<T> *__restrict__ A, const cpx<T> *__restrict__ twg, FftPlan pl, 
 // charge -rho[.., mz-1-z], whose spectrum follows from rho's), backward z
 // transforms, z < nz of each field spectrum to B.  The potential (mode
 // kPotential) has one Green spectrum, real and even in every axis.
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restrict__ Gf,
         const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb,
         int mode)
 {
     const bool cathode = mode == kFieldCathode, pot = mode == kPotential;
     extern __shared__ __align__(16) unsigned char smem[];
-    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
-    cpx<T> *P0 = tw + d.Lz;
+    // twiddles in LDS, except on the widest workgroups (the longest
+    // pencils), whose two pencil buffers fill the LDS: they read them from
+    // global memory
+    constexpr bool kLdsTw = NT == kFftThreads;
+    cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem);
+    const cpx<T> *tw = kLdsTw ? lds : twg;
+    cpx<T> *P0 = kLdsTw ? lds + d.Lz : lds;
     cpx<T> *W = P0 + nb * d.Lz;
     // nz, ny: transform half-lengths (spectral octant); mz: mesh z nodes
     const int Lz = d.Lz, nz = d.nz, mz = d.mz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
     const int ny1 = ny + 1;
-    load_twiddles(tw, twg, Lz);
+    if constexpr (kLdsTw) load_twiddles(lds, twg, Lz);
     const int kx0 = blockIdx.x * nb, ky = blockIdx.y;
     const size_t zs = (size_t)Ly * Kp;  // stride between z planes of A and B
     for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
@@ -137,7 +142,7 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
         P0[idx] = v;
     }
     __syncthreads();
-    lds_fft<T, E>(P0, tw, pl, nb, nb, 1, true, -1);
+    lds_fft<T, E, NT>(P0, tw, pl, nb, nb, 1, true, -1);
     const bool kyneg = ky > ny;
     const int kyp = kyneg ? Ly - ky : ky;
     const size_t M1 = (size_t)(nz + 1) * ny1 * Kp;  // one free-space Green spectrum
@@ -172,7 +177,7 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
             W[idx] = out;
         }
         __syncthreads();
-        lds_fft<T, E>(W, tw, pl, nb, nb, 1, true, +1);
+        lds_fft<T, E, NT>(W, tw, pl, nb, nb, 1, true, +1);
         cpx<T> *Bc = B + (size_t)c * mz * zs;
         for (int idx = threadIdx.x; idx < nb * mz; idx += blockDim.x) {
             const int z = idx / nb, b = idx - z * nb;
@@ -188,20 +193,23 @@ k_zconv(const cpx<T> *__restrict__ A, cpx<T> *__restrict__ B, const T *__restric
 // with the cathode minus the image sum Wi = sum_c p conj(Gi_c^) reversed in
 // z (out[z] = W[z] - Wi[mz-1-z] after the backward transform, formed in the
 // spectrum), one backward z transform, z < mz to out.
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__restrict__ Gf,
             const cpx<T> *__restrict__ Gi, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d,
             int nb, int cathode)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    cpx<T> *tw = reinterpret_cast<cpx<T> *>(smem);
-    cpx<T> *Q = tw + d.Lz;
+    // twiddles as in k_zconv (global memory on the widest workgroups)
+    constexpr bool kLdsTw = NT == kFftThreads;
+    cpx<T> *lds = reinterpret_cast<cpx<T> *>(smem);
+    const cpx<T> *tw = kLdsTw ? lds : twg;
+    cpx<T> *Q = kLdsTw ? lds + d.Lz : lds;
     cpx<T> *W = Q + nb * d.Lz;
     // nz, ny: transform half-lengths (spectral octant); mz: mesh z nodes
     const int Lz = d.Lz, nz = d.nz, mz = d.mz, Kx = d.Kx, Kp = d.Kp, Ly = d.Ly, ny = d.ny;
     const int ny1 = ny + 1;
-    load_twiddles(tw, twg, Lz);
+    if constexpr (kLdsTw) load_twiddles(lds, twg, Lz);
     const int kx0 = blockIdx.x * nb, ky = blockIdx.y;
     const size_t zs = (size_t)Ly * Kp;
     const bool kyneg = ky > ny;
@@ -219,7 +227,7 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
             Q[idx] = v;
         }
         __syncthreads();
-        lds_fft<T, E>(Q, tw, pl, nb, nb, 1, true, -1);
+        lds_fft<T, E, NT>(Q, tw, pl, nb, nb, 1, true, -1);
         const T sy = (c == 1 && kyneg) ? (T)-1 : (T)1;
         for (int idx = threadIdx.x; idx < nb * Lz; idx += blockDim.x) {
             const int kz = idx / nb, b = idx - kz * nb;
@@ -250,7 +258,7 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
         }
         __syncthreads();
     }
-    lds_fft<T, E>(W, tw, pl, nb, nb, 1, true, +1);
+    lds_fft<T, E, NT>(W, tw, pl, nb, nb, 1, true, +1);
     __syncthreads();
     for (int idx = threadIdx.x; idx < nb * mz; idx += blockDim.x) {
         const int z = idx / nb, b = idx - z * nb;
@@ -260,8 +268,8 @@ k_zconv_adj(const cpx<T> *__restrict__ Gs, cpx<T> *__restrict__ out, const T *__
 }
 
 // Backward y pass: pencils (kx, z < nz) of each field spectrum, y < ny kept.
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims d, int nb)
 {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -279,7 +287,7 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
         buf[idx] = v;
     }
     __syncthreads();
-    lds_fft<T, E>(buf, tw, pl, nb, nb, 1, true, +1);
+    lds_fft<T, E, NT>(buf, tw, pl, nb, nb, 1, true, +1);
     for (int idx = threadIdx.x; idx < nb * d.my; idx += blockDim.x) {
         const int e = idx / nb, b = idx - e * nb;
         const int kx = kx0 + b;
@@ -289,8 +297,8 @@ k_inv_y(cpx<T> *__restrict__ B, const cpx<T> *__restrict__ twg, FftPlan pl, Dims
 
 // Backward x pass (complex to real): rows (y < ny, z < nz) of each field
 // spectrum, Hermitian completion, x < nx of the real part to efield[c].
-template <typename T, int E = fft_elems<T>()>
-__global__ void __launch_bounds__(kFftThreads)
+template <typename T, int E = fft_elems<T>(), int NT = kFftThreads>
+__global__ void __launch_bounds__(NT)
 k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ efield, const cpx<T> *__restrict__ twg, FftPlan pl,
         Dims d, int nb)
 {
@@ -319,7 +327,7 @@ k_inv_x(const cpx<T> *__restrict__ B, T *__restrict__ efield, const cpx<T> *__re
         buf[b * Lx + Lx - k] = cpx<T>{v.x, -v.y};
     }
     __syncthreads();
-    lds_fft<T, E>(buf, tw, pl, nb, 1, Lx, false, +1);
+    lds_fft<T, E, NT>(buf, tw, pl, nb, 1, Lx, false, +1);
     T *Ec = efield + (size_t)c * nx * nrows;
     for (int idx = threadIdx.x; idx < nb * nx; idx += blockDim.x) {
         const int b = idx / nx, x = idx - b * nx;
@@ -425,47 +433,49 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     cpx<T> *F = static_cast<cpx<T> *>(io.fwd);
     cpx<T> *B = static_cast<cpx<T> *>(io.inv);
     if (!fast(0)) {
-        with_fft_elems<T>(d.Lx, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            const int nb = pick_nb(d.Lx, E * kFftThreads, 16);
+        with_fft_elems<T>(d.Lx, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            const int nb = pick_nb(d.Lx, E * NT, 16);
             const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
             launch_begin(ctx, K_RHO_FWD_X);
-            hipLaunchKernelGGL((k_rho_fwd_x<T, E>), dim3(div_up(d.my * d.mz, nb), io.nin),
-                               dim3(kFftThreads), sh, st, static_cast<const T *>(io.in), F, twx,
+            hipLaunchKernelGGL((k_rho_fwd_x<T, E, NT>), dim3(div_up(d.my * d.mz, nb), io.nin),
+                               dim3(NT), sh, st, static_cast<const T *>(io.in), F, twx,
                                ws.plan[0], d, nb);
             launch_end(ctx, K_RHO_FWD_X);
         });
     }
     if (!fast(1)) {
-        with_fft_elems<T>(d.Ly, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            const int nb = pick_nb(d.Ly, E * kFftThreads, 16);
+        with_fft_elems<T>(d.Ly, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            const int nb = pick_nb(d.Ly, E * NT, 16);
             const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
             launch_begin(ctx, K_RHO_FWD_Y);
-            hipLaunchKernelGGL((k_rho_fwd_y<T, E>), dim3(div_up(d.Kx, nb), d.mz, io.nin),
-                               dim3(kFftThreads), sh, st, F, twy, ws.plan[1], d, nb);
+            hipLaunchKernelGGL((k_rho_fwd_y<T, E, NT>), dim3(div_up(d.Kx, nb), d.mz, io.nin),
+                               dim3(NT), sh, st, F, twy, ws.plan[1], d, nb);
             launch_end(ctx, K_RHO_FWD_Y);
         });
     }
     // ---- fused z convolution ----
     if (overlap) SC_HIP(ctx, hipStreamWaitEvent(st, ws.green_done, 0));
     if (!fast(2)) {
-        with_fft_elems<T>(d.Lz, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            // LDS buffers of nb pencils: the spectrum and the product
+        with_fft_elems<T>(d.Lz, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            // LDS buffers of nb pencils: the spectrum and the product, after
+            // the twiddles (read from global memory on the widest workgroups)
             const int nbuf = 2;
+            const size_t ntw = NT == kFftThreads ? (size_t)d.Lz : 0;
             // at most 64 KiB of LDS where more pencils fit (occupancy); one
             // long pencil may take more (up to the device's 160 KiB)
-            int nb = pick_nb(d.Lz, E * kFftThreads, 16);
-            while (nb > 1 && sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz) > 65536) nb /= 2;
-            const size_t sh = sizeof(cpx<T>) * (d.Lz + (size_t)nbuf * nb * d.Lz);
+            int nb = pick_nb(d.Lz, E * NT, 16);
+            while (nb > 1 && sizeof(cpx<T>) * (ntw + (size_t)nbuf * nb * d.Lz) > 65536) nb /= 2;
+            const size_t sh = sizeof(cpx<T>) * (ntw + (size_t)nbuf * nb * d.Lz);
             launch_begin(ctx, K_ZCONV);
             if (mode_adjoint(mode))
-                hipLaunchKernelGGL((k_zconv_adj<T, E>), dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads),
+                hipLaunchKernelGGL((k_zconv_adj<T, E, NT>), dim3(div_up(d.Kx, nb), d.Ly), dim3(NT),
                                    sh, st, (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
                                    static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, cath);
             else
-                hipLaunchKernelGGL((k_zconv<T, E>), dim3(div_up(d.Kx, nb), d.Ly), dim3(kFftThreads), sh,
+                hipLaunchKernelGGL((k_zconv<T, E, NT>), dim3(div_up(d.Kx, nb), d.Ly), dim3(NT), sh,
                                    st, (const cpx<T> *)F, B, static_cast<const T *>(ws.Gf),
                                    static_cast<const cpx<T> *>(ws.Gi), twz, ws.plan[2], d, nb, mode);
             launch_end(ctx, K_ZCONV);
@@ -477,24 +487,24 @@ int run_solve(Mesh *m, const double lo[3], const double hi[3], const double dl[3
     }
     // ---- backward y, x ----
     if (!fast(3)) {
-        with_fft_elems<T>(d.Ly, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            const int nb = pick_nb(d.Ly, E * kFftThreads, 16);
+        with_fft_elems<T>(d.Ly, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            const int nb = pick_nb(d.Ly, E * NT, 16);
             const size_t sh = sizeof(cpx<T>) * (d.Ly + (size_t)nb * d.Ly);
             launch_begin(ctx, K_INV_Y);
-            hipLaunchKernelGGL((k_inv_y<T, E>), dim3(div_up(d.Kx, nb), d.mz, io.nout), dim3(kFftThreads),
+            hipLaunchKernelGGL((k_inv_y<T, E, NT>), dim3(div_up(d.Kx, nb), d.mz, io.nout), dim3(NT),
                                sh, st, B, twy, ws.plan[1], d, nb);
             launch_end(ctx, K_INV_Y);
         });
     }
     if (!fast(4)) {
-        with_fft_elems<T>(d.Lx, [&](auto e) {
-            constexpr int E = decltype(e)::value;
-            const int nb = pick_nb(d.Lx, E * kFftThreads, 16);
+        with_fft_elems<T>(d.Lx, [&](auto e, auto t) {
+            constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+            const int nb = pick_nb(d.Lx, E * NT, 16);
             const size_t sh = sizeof(cpx<T>) * (d.Lx + (size_t)nb * d.Lx);
             launch_begin(ctx, K_INV_X);
-            hipLaunchKernelGGL((k_inv_x<T, E>), dim3(div_up(d.my * d.mz, nb), io.nout),
-                               dim3(kFftThreads), sh, st, (const cpx<T> *)B, static_cast<T *>(io.out),
+            hipLaunchKernelGGL((k_inv_x<T, E, NT>), dim3(div_up(d.my * d.mz, nb), io.nout),
+                               dim3(NT), sh, st, (const cpx<T> *)B, static_cast<T *>(io.out),
                                twx, ws.plan[0], d, nb);
             launch_end(ctx, K_INV_X);
         });
@@ -611,7 +621,7 @@ int ensure_workspace(Mesh *m)
     const int64_t nx = d.nx, ny = d.ny, nz = d.nz;
     const int L[3] = {d.Lx, d.Ly, d.Lz};
     const int64_t mn[3] = {m->n[0], m->n[1], m->n[2]};
-    const int cap = (m->dtype == SC_F32) ? fft_cap_long<float>() : fft_cap_long<double>();
+    const int cap = (m->dtype == SC_F32) ? fft_cap_xl<float>() : fft_cap_xl<double>();
     for (int a = 0; a < 3; ++a) {
         // one pencil must fit one workgroup (registers and LDS)
         if (L[a] > cap)

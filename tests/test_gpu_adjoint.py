@@ -97,10 +97,14 @@ def test_solve_adjoint_matches_oracle(grid, gamma, cath, generic):
 @pytest.mark.parametrize("grid,cath,T", [((6, 6, 1500), False, torch.float32),
                                          ((5, 6, 2048), True, torch.float32),
                                          ((1000, 6, 5), False, torch.float64),
-                                         ((6, 5, 1024), True, torch.float64)])
+                                         ((6, 5, 1024), True, torch.float64),
+                                         ((5, 6, 4096), True, torch.float32),
+                                         ((2048, 6, 5), False, torch.float64),
+                                         ((6, 5, 2048), True, torch.float64)])
 def test_solve_adjoint_long_axis(grid, cath, T):
     """Transforms beyond 2048 Float32 / 1024 Float64 points, where the engine
-    holds twice the values per lane; the bunch's own field as cotangent."""
+    holds twice the values per lane (on 512-lane workgroups beyond 4096 /
+    2048 points); the bunch's own field as cotangent."""
     sc = _sc()
     x, y, z, q = _bunch(20_000, 33, zshift=4e-3 if cath else 0.0)
     mesh = sc.Mesh3D(grid, x, y, z, T=T)

@@ -302,6 +302,15 @@ LONG_AXIS_CASES = [
     ((1000, 5, 5), 3.0, False, torch.float64),    # x: 2000
     ((6, 1024, 6), 1.0, False, torch.float64),    # y: 2048
     ((6, 5, 1024), 1.5, True, torch.float64),     # cathode, z: 2048
+    # beyond 4096 Float32 / 2048 Float64 points: 512-lane workgroups
+    ((6, 6, 3000), 1.0, False, torch.float32),    # z: 6000
+    ((4096, 5, 4), 1.0, False, torch.float32),    # x: 8192
+    ((5, 4096, 4), 1.0, False, torch.float32),    # y: 8192
+    ((5, 6, 4096), 2.0, True, torch.float32),     # cathode, z: 8192
+    ((6, 5, 1500), 1.0, False, torch.float64),    # z: 3000
+    ((2048, 5, 4), 1.0, False, torch.float64),    # x: 4096
+    ((5, 2048, 4), 2.0, False, torch.float64),    # y: 4096
+    ((5, 6, 2048), 1.5, True, torch.float64),     # cathode, z: 4096
 ]
 
 
@@ -318,12 +327,12 @@ def test_long_axis_matches_oracle(grid, gamma, cath, T):
     assert max(errs) <= 1e-5
 
 
-@pytest.mark.parametrize("grid,T", [((4, 4, 2049), torch.float32), ((2049, 4, 4), torch.float32),
-                                    ((4, 1025, 4), torch.float64)])
+@pytest.mark.parametrize("grid,T", [((4, 4, 4097), torch.float32), ((4097, 4, 4), torch.float32),
+                                    ((4, 2049, 4), torch.float64)])
 def test_axis_beyond_the_maximum_is_refused(grid, T):
-    """2049 Float32 / 1025 Float64 nodes need more than 4096 / 2048 points
-    (padded to 4116 / 2058); every axis up to 2048 / 1024 nodes fits, since
-    4096 and 2048 are themselves 7-smooth."""
+    """4097 Float32 / 2049 Float64 nodes need more than 8192 / 4096 points
+    (padded to 8232 / 4116); every axis up to 4096 / 2048 nodes fits, since
+    8192 and 4096 are themselves 7-smooth."""
     sc = _sc()
     x, y, z = gaussian(1000, (1e-3, 1e-3, 2e-3), 5)
     xt, yt, zt, qt = _tensors((x, y, z, np.full(1000, 1e-12)), torch.float64)

@@ -92,10 +92,12 @@ def test_potential_fp32_matches_extended(grid, gamma, generic):
 
 
 @pytest.mark.parametrize("grid,T", [((6, 6, 1500), torch.float32), ((6, 2048, 5), torch.float32),
-                                    ((6, 700, 6), torch.float64), ((1024, 5, 6), torch.float64)])
+                                    ((6, 700, 6), torch.float64), ((1024, 5, 6), torch.float64),
+                                    ((6, 6, 3000), torch.float32), ((6, 2048, 5), torch.float64)])
 def test_potential_long_axis(grid, T):
     """Transforms beyond 2048 Float32 / 1024 Float64 points (twice the
-    engine's values per lane), against the extended-precision potential."""
+    engine's values per lane; 512-lane workgroups beyond 4096 / 2048),
+    against the extended-precision potential."""
     sc = _sc()
     mesh = _mesh(grid, T, 1.0)
     phi = sc.solve_potential(mesh)
