@@ -28,6 +28,7 @@ the reference's messages where it has one.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import math
 from typing import Sequence, Tuple
@@ -76,6 +77,25 @@ _ctxs = {}
 _ctx_streams = {}  # device index -> stream handle last given to that context
 
 
+# torch's current stream as a raw handle without building a Stream object
+# (~2 us of host time per call); checked against the public API once
+_raw_stream = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+_raw_stream_checked = False
+
+
+def _current_stream(idx: int) -> int:
+    global _raw_stream, _raw_stream_checked
+    if _raw_stream is not None:
+        s = _raw_stream(idx)
+        if _raw_stream_checked:
+            return s
+        if s == torch.cuda.current_stream(idx).cuda_stream:
+            _raw_stream_checked = True
+            return s
+        _raw_stream = None
+    return torch.cuda.current_stream(idx).cuda_stream
+
+
 def _ctx(device: torch.device) -> ctypes.c_void_p:
     idx = device.index if device.index is not None else torch.cuda.current_device()
     c = _ctxs.get(idx)
@@ -87,7 +107,7 @@ def _ctx(device: torch.device) -> ctypes.c_void_p:
             raise SpaceChargeError(lib.sc_last_error(None).decode())
         c = h
         _ctxs[idx] = c
-    stream = torch.cuda.current_stream(idx).cuda_stream
+    stream = _current_stream(idx)
     if _ctx_streams.get(idx) != stream:
         _lib.load().sc_ctx_set_stream(c, ctypes.c_void_p(stream))
         _ctx_streams[idx] = stream
@@ -99,6 +119,21 @@ def _check(rc: int, ctx) -> None:
         raise SpaceChargeError(_lib.load().sc_last_error(ctx).decode())
 
 
+_NO_SWITCH = contextlib.nullcontext()
+
+
+def _on(device: torch.device):
+    """torch.cuda.device(device) around a library call, which restores the
+    caller's current device afterwards; nothing when it is already current."""
+    if device.index is not None and torch.cuda.current_device() == device.index:
+        return _NO_SWITCH
+    return torch.cuda.device(device)
+
+
+def _len(a) -> int:
+    return a.shape[0] if isinstance(a, torch.Tensor) and a.dim() == 1 else len(a)
+
+
 def _default_device() -> torch.device:
     if not torch.cuda.is_available():
         raise SpaceChargeError("no GPU visible: the space-charge step runs on MI355X only")
@@ -106,6 +141,12 @@ def _default_device() -> torch.device:
 
 
 def _as_device(a, device: torch.device, dtype=None) -> torch.Tensor:
+    # the common case, a flat contiguous device tensor of a supported type,
+    # passes as it is (the conversions below cost ~2 us of host time per
+    # array, which small meshes and tracking loops feel)
+    if (isinstance(a, torch.Tensor) and a.device == device and a.dim() == 1
+            and a.dtype in _DT and (dtype is None or a.dtype == dtype) and a.is_contiguous()):
+        return a
     t = torch.as_tensor(a, device=device)
     if dtype is not None and t.dtype != dtype:
         t = t.to(dtype)
@@ -147,7 +188,7 @@ class Mesh3D:
         g = _check_grid(grid_size)
         if len(args) == 3:
             px, py, pz = args
-            lens = [len(a) for a in (px, py, pz)]
+            lens = [_len(a) for a in (px, py, pz)]
             if min(lens) == 0:
                 raise SpaceChargeError("Particle arrays cannot be empty.")
             if not (lens[0] == lens[1] == lens[2]):
@@ -159,7 +200,7 @@ class Mesh3D:
             z = _as_device(pz, dev, x.dtype)
             lo, hi, dl = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
             ctx = _ctx(dev)
-            with torch.cuda.device(dev):
+            with _on(dev):
                 rc = _lib.load().sc_bounds_from_particles(
                     ctx, _DT[x.dtype], x.data_ptr(), y.data_ptr(), z.data_ptr(), x.numel(),
                     _lib.i3(g), _DT[T], lo, hi, dl)
@@ -199,7 +240,7 @@ class Mesh3D:
         self.efield = self._ef.as_strided((nx, ny, nz, 3), (1, nx, nx * ny, M))
         ctx = _ctx(dev)
         h = ctypes.c_void_p()
-        with torch.cuda.device(dev):
+        with _on(dev):
             rc = _lib.load().sc_mesh_create_with_buffers(
                 ctx, _DT[T], nx, ny, nz, self._rho.data_ptr(), self._ef.data_ptr(), ctypes.byref(h))
         _check(rc, ctx)
@@ -258,7 +299,7 @@ class Mesh3D:
         only; picked by the library for Float64 fields of 64 MiB and more),
         0 = by size."""
         ctx = _ctx(self.device)
-        with torch.cuda.device(self.device):
+        with _on(self.device):
             rc = _lib.load().sc_mesh_set_field_layout(self._h, int(layout))
         _check(rc, ctx)
 
@@ -268,14 +309,14 @@ class Mesh3D:
         host round trip (no reference counterpart).  From then on the step
         functions read the device-resident bounds; ``min_bounds``,
         ``max_bounds`` and ``delta`` are refreshed only by fetch_bounds()."""
-        lens = [len(a) for a in (particles_x, particles_y, particles_z)]
+        lens = [_len(a) for a in (particles_x, particles_y, particles_z)]
         if min(lens) == 0:
             raise SpaceChargeError("Particle arrays cannot be empty.")
         if not (lens[0] == lens[1] == lens[2]):
             raise SpaceChargeError("Particle coordinate arrays must have the same length.")
         x, y, z = _particles(self, (particles_x, particles_y, particles_z), "xyz")
         ctx = _ctx(self.device)
-        with torch.cuda.device(self.device):
+        with _on(self.device):
             rc = _lib.load().sc_mesh_auto_bounds(self._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                                  z.data_ptr(), x.numel())
         _check(rc, ctx)
@@ -287,25 +328,31 @@ class Mesh3D:
         delta (synchronises) and return them."""
         lo, hi, dl = (ctypes.c_double * 3)(), (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
         ctx = _ctx(self.device)
-        with torch.cuda.device(self.device):
+        with _on(self.device):
             rc = _lib.load().sc_mesh_read_bounds(self._h, lo, hi, dl)
         _check(rc, ctx)
         self.min_bounds, self.max_bounds, self.delta = tuple(lo), tuple(hi), tuple(dl)
         return self.min_bounds, self.max_bounds, self.delta
 
 
-def _bounds_ld(mesh: Mesh3D):
-    """(min_bounds, delta) arguments: NULL when the mesh's bounds live on the device."""
-    if mesh._device_bounds:
-        return None, None
-    return _lib.d3(mesh.min_bounds), _lib.d3(mesh.delta)
-
-
 def _bounds_lhd(mesh: Mesh3D):
-    """(min_bounds, max_bounds, delta) arguments, NULL for device-resident bounds."""
+    """(min_bounds, max_bounds, delta) arguments, NULL for device-resident
+    bounds.  The ctypes arrays are kept on the mesh while its bounds stay
+    the same (the library reads them during the call only)."""
     if mesh._device_bounds:
         return None, None, None
-    return _lib.d3(mesh.min_bounds), _lib.d3(mesh.max_bounds), _lib.d3(mesh.delta)
+    key = (mesh.min_bounds, mesh.max_bounds, mesh.delta)
+    cached = mesh.__dict__.get("_bounds_args")
+    if cached is None or cached[0] != key:
+        cached = (key, (_lib.d3(key[0]), _lib.d3(key[1]), _lib.d3(key[2])))
+        mesh._bounds_args = cached
+    return cached[1]
+
+
+def _bounds_ld(mesh: Mesh3D):
+    """(min_bounds, delta) arguments: NULL when the mesh's bounds live on the device."""
+    lo, _, dl = _bounds_lhd(mesh)
+    return lo, dl
 
 
 def _particles(mesh: Mesh3D, arrays, names) -> list:
@@ -322,19 +369,19 @@ def _particles(mesh: Mesh3D, arrays, names) -> list:
 def clear_mesh(mesh: Mesh3D) -> None:
     """clear_mesh! (src/deposition.jl:10-12): rho = 0."""
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         _check(_lib.load().sc_clear_mesh(mesh._h), ctx)
 
 
 def deposit(mesh: Mesh3D, particles_x, particles_y, particles_z, particles_q, *,
             clear: bool = True) -> None:
     """deposit! (src/deposition.jl:218-247): CIC charge deposit into mesh.rho."""
-    lens = [len(a) for a in (particles_x, particles_y, particles_z, particles_q)]
+    lens = [_len(a) for a in (particles_x, particles_y, particles_z, particles_q)]
     if not all(n == lens[0] for n in lens):
         raise SpaceChargeError("Particle coordinate and charge arrays must have the same length.")
     x, y, z, q = _particles(mesh, (particles_x, particles_y, particles_z, particles_q), "xyzq")
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_deposit(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                     z.data_ptr(), q.data_ptr(), x.numel(),
                                     *_bounds_ld(mesh), int(bool(clear)))
@@ -344,7 +391,7 @@ def deposit(mesh: Mesh3D, particles_x, particles_y, particles_z, particles_q, *,
 def solve(mesh: Mesh3D, *, at_cathode: bool = False) -> None:
     """solve! (src/solvers/free_space.jl:14-47): mesh.efield from mesh.rho."""
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_solve(mesh._h, *_bounds_lhd(mesh), float(mesh.gamma),
                                   int(bool(at_cathode)))
     _check(rc, ctx)
@@ -365,7 +412,7 @@ def solve_potential(mesh: Mesh3D, out=None):
           or not out.is_contiguous()):
         raise SpaceChargeError("out must be a contiguous tensor of the mesh's dtype, device and size")
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_solve_potential(mesh._h, _bounds_ld(mesh)[1], float(mesh.gamma),
                                             out.data_ptr())
     _check(rc, ctx)
@@ -374,13 +421,13 @@ def solve_potential(mesh: Mesh3D, out=None):
 
 def interpolate_field(mesh: Mesh3D, particles_x, particles_y, particles_z):
     """interpolate_field (src/interpolation.jl:100-128): (Ex, Ey, Ez) in the particle dtype."""
-    lens = [len(a) for a in (particles_x, particles_y, particles_z)]
+    lens = [_len(a) for a in (particles_x, particles_y, particles_z)]
     if not all(n == lens[0] for n in lens):
         raise SpaceChargeError("Particle coordinate arrays must have the same length.")
     x, y, z = _particles(mesh, (particles_x, particles_y, particles_z), "xyz")
     Ex, Ey, Ez = (torch.empty_like(x) for _ in range(3))
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_interpolate(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                         z.data_ptr(), x.numel(), *_bounds_ld(mesh),
                                         Ex.data_ptr(), Ey.data_ptr(), Ez.data_ptr())
@@ -404,7 +451,7 @@ def autotune_deposit(mesh: Mesh3D, particles_x, particles_y, particles_z, partic
         raise SpaceChargeError("Particle coordinate and charge arrays must have the same length.")
     times = (ctypes.c_float * (len(CHUNK_LOG2) * len(SLICE_LOG2)))()
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_mesh_autotune_deposit(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                                   z.data_ptr(), q.data_ptr(), x.numel(),
                                                   *_bounds_ld(mesh), int(reps), times)
@@ -429,7 +476,7 @@ def sort_permutation(mesh: Mesh3D, particles_x, particles_y, particles_z) -> tor
     x, y, z = _particles(mesh, (particles_x, particles_y, particles_z), "xyz")
     perm = torch.empty(x.numel(), dtype=torch.int32, device=mesh.device)
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_sort_permutation(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                              z.data_ptr(), x.numel(), *_bounds_ld(mesh),
                                              perm.data_ptr())
@@ -469,7 +516,7 @@ def deposit_adjoint(mesh: Mesh3D, particles_x, particles_y, particles_z, particl
     gx, gy, gz = (torch.empty_like(x) for _ in range(3)) if positions else (None, None, None)
     ptr = (lambda t: 0 if t is None else t.data_ptr())
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_deposit_adjoint(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                             z.data_ptr(), q.data_ptr(), x.numel(),
                                             *_bounds_ld(mesh),
@@ -485,7 +532,7 @@ def solve_adjoint(mesh: Mesh3D, g_efield, *, at_cathode: bool = False):
     g = _fortran_flat(mesh, g_efield, tuple(mesh.grid_size) + (3,), "g_efield")
     out = torch.empty(math.prod(mesh.grid_size), dtype=mesh.rho.dtype, device=mesh.device)
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_solve_adjoint(mesh._h, *_bounds_lhd(mesh), float(mesh.gamma),
                                           int(bool(at_cathode)), g.data_ptr(), out.data_ptr())
     _check(rc, ctx)
@@ -506,7 +553,7 @@ def interpolate_adjoint(mesh: Mesh3D, particles_x, particles_y, particles_z, g_E
     gx, gy, gz = (torch.empty_like(x) for _ in range(3)) if positions else (None, None, None)
     ptr = (lambda t: 0 if t is None else t.data_ptr())
     ctx = _ctx(mesh.device)
-    with torch.cuda.device(mesh.device):
+    with _on(mesh.device):
         rc = _lib.load().sc_interpolate_adjoint(mesh._h, _DT[x.dtype], x.data_ptr(), y.data_ptr(),
                                                 z.data_ptr(), x.numel(), *_bounds_ld(mesh),
                                                 gE[0].data_ptr(),
@@ -545,7 +592,7 @@ def _batch_particles(meshes, groups, names):
     dtype = None
     for b, m in enumerate(meshes):
         arrs = [g[b] for g in groups]
-        lens = [len(a) for a in arrs]
+        lens = [_len(a) for a in arrs]
         if not all(n == lens[0] for n in lens):
             raise SpaceChargeError("Particle coordinate and charge arrays must have the same length.")
         for k, a in enumerate(arrs):
@@ -562,7 +609,7 @@ def deposit_batched(meshes: Sequence[Mesh3D], xs, ys, zs, qs, *, clear: bool = T
     x, y, z, q = _batch_particles(meshes, (xs, ys, zs, qs), "xyzq")
     n = (ctypes.c_int64 * len(meshes))(*[t.numel() for t in x])
     ctx = _ctx(dev)
-    with torch.cuda.device(dev):
+    with _on(dev):
         rc = _lib.load().sc_deposit_batched(
             (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes]), len(meshes), _DT[x[0].dtype],
             _ptrs(x), _ptrs(y), _ptrs(z), _ptrs(q), n, _flat3([m.min_bounds for m in meshes]),
@@ -577,7 +624,7 @@ def solve_batched(meshes: Sequence[Mesh3D], *, at_cathode=False) -> None:
     if len(flags) != len(meshes):
         raise SpaceChargeError("at_cathode needs one flag per mesh")
     ctx = _ctx(dev)
-    with torch.cuda.device(dev):
+    with _on(dev):
         rc = _lib.load().sc_solve_batched(
             (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes]), len(meshes),
             _flat3([m.min_bounds for m in meshes]), _flat3([m.max_bounds for m in meshes]),
@@ -594,7 +641,7 @@ def interpolate_batched(meshes: Sequence[Mesh3D], xs, ys, zs):
     E = [[torch.empty_like(t) for t in x] for _ in range(3)]
     n = (ctypes.c_int64 * len(meshes))(*[t.numel() for t in x])
     ctx = _ctx(dev)
-    with torch.cuda.device(dev):
+    with _on(dev):
         rc = _lib.load().sc_interpolate_batched(
             (ctypes.c_void_p * len(meshes))(*[m._h for m in meshes]), len(meshes), _DT[x[0].dtype],
             _ptrs(x), _ptrs(y), _ptrs(z), n, _flat3([m.min_bounds for m in meshes]),
