@@ -278,12 +278,18 @@ k_green_f(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, i
 // (the tangent of the first two summed is rw/(uv), and their product
 // w^2/r^2 < 1), so the third is formed from the other two.  A lane takes a
 // node (x, y) of kF3Planes consecutive z planes, so log(u^2 + v^2) is formed
-// once per lane.  Per node this is 1 sqrt, 2 atan and 5.25 logs against
-// 3 sqrt, 3 atan, 6 logs and 3 quotients for the three components taken one
-// by one.  FAST: the few-ulp log and atan (Float32 meshes); otherwise the
-// library's (Float64 meshes).
+// once per lane; log(v^2 + w^2) depends on the node's y row and z plane
+// only, and log(w^2 + u^2) on its x column and z plane, so the workgroup
+// forms them once per row and column of its 256 nodes (tables in LDS; the
+// column table for planes of at most 256 columns).  Per node this is
+// 1 sqrt, 2 atan and about 3.8 logs on a 129^2 plane against 3 sqrt, 3 atan,
+// 6 logs and 3 quotients for the three components taken one by one.  FAST:
+// the few-ulp log and atan (Float32 meshes); otherwise the library's
+// (Float64 meshes).
 // Grid (x-y plane blocks, z plane groups, image sets).
 constexpr int kF3Planes = 4;
+constexpr int kF3Rows = 128;  // y rows of one workgroup's 256 nodes (rows of >= 3 nodes)
+constexpr int kF3Cols = 256;  // x columns of the column table
 
 template <bool FAST>
 __global__ void __launch_bounds__(256)
@@ -298,30 +304,61 @@ k_green_f3(double *__restrict__ F, GreenArgs a, const SBunch *__restrict__ tab, 
     const int image = blockIdx.z;
     const int nx1 = a.d.nx + 1, ny1 = a.d.ny + 1;
     const int pidx = bx * blockDim.x + threadIdx.x;
-    if (pidx >= nx1 * ny1) return;
-    const int in = pidx % nx1;
-    const int jn = pidx / nx1;
+    const int np = nx1 * ny1;
     double dx, dy, dz, offzg;
     green_geometry(a, dx, dy, dz, offzg);
-    const double u = ((double)in - 0.5) * dx;
-    const double v = ((double)jn - 0.5) * dy;
+    const int zn = z_nodes(a, image);
+    const int k0 = blockIdx.y * kF3Planes;
+    auto coord_u = [&](int i) { return ((double)i - 0.5) * dx; };
+    auto coord_v = [&](int j) { return ((double)j - 0.5) * dy; };
+    auto coord_w = [&](int k) {
+        return image ? ((double)(k - a.d.nz) + 0.5) * dz + offzg : ((double)k - 0.5) * dz;
+    };
+    // the row and column logs of this workgroup's nodes (every lane takes
+    // part, live or not: the tables end with a barrier)
+    __shared__ double qvw_t[kF3Planes][kF3Rows];
+    __shared__ double qwu_t[kF3Planes][kF3Cols];
+    const int p0 = bx * blockDim.x;
+    const int p1 = (p0 + (int)blockDim.x < np ? p0 + (int)blockDim.x : np) - 1;
+    const int jn0 = p0 / nx1;
+    const int nrows = p0 < np ? p1 / nx1 - jn0 + 1 : 0;  // <= 255 / nx1 + 2 <= kF3Rows
+    const bool coltab = nx1 <= kF3Cols;
+    for (int t = threadIdx.x; t < kF3Planes * nrows; t += blockDim.x) {
+        const int kk = t / nrows, jr = t - kk * nrows;
+        const double v = coord_v(jn0 + jr), w = coord_w(k0 + kk);
+        const double vv = v * v, ww = w * w;
+        qvw_t[kk][jr] = log_<FAST>(vv + ww);
+    }
+    if (coltab) {
+        for (int t = threadIdx.x; t < kF3Planes * nx1; t += blockDim.x) {
+            const int kk = t / nx1, i = t - kk * nx1;
+            const double u = coord_u(i), w = coord_w(k0 + kk);
+            const double uu = u * u, ww = w * w;
+            qwu_t[kk][i] = log_<FAST>(ww + uu);
+        }
+    }
+    __syncthreads();
+    if (pidx >= np) return;
+    const int in = pidx % nx1;
+    const int jn = pidx / nx1;
+    const double u = coord_u(in);
+    const double v = coord_v(jn);
     const double uu = u * u, vv = v * v;
     const double Quv = log_<FAST>(uu + vv);
     const int64_t plane = (int64_t)nx1 * ny1;
     gbl<double> *Fx = (gbl<double> *)(F + f_off(a, 3 * image));
     gbl<double> *Fy = (gbl<double> *)(F + f_off(a, 3 * image + 1));
     gbl<double> *Fz = (gbl<double> *)(F + f_off(a, 3 * image + 2));
-    const int zn = z_nodes(a, image);
-    const int k0 = blockIdx.y * kF3Planes;
 #pragma unroll
     for (int kk = 0; kk < kF3Planes; ++kk) {
         const int kn = k0 + kk;
         if (kn >= zn) break;
-        const double w = image ? ((double)(kn - a.d.nz) + 0.5) * dz + offzg : ((double)kn - 0.5) * dz;
+        const double w = coord_w(kn);
         const double ww = w * w;
         const double r = sqrt((uu + vv) + ww);
         const double Lu = log_<FAST>(r + u), Lv = log_<FAST>(r + v), Lw = log_<FAST>(r + w);
-        const double Qvw = log_<FAST>(vv + ww), Qwu = log_<FAST>(ww + uu);
+        const double Qvw = qvw_t[kk][jn - jn0];
+        const double Qwu = coltab ? qwu_t[kk][in] : log_<FAST>(ww + uu);
         const double ax = atan_q<FAST>(v * w, r * u);
         const double ay = atan_q<FAST>(w * u, r * v);
         const double hp = w == 0.0 ? 0.0
