@@ -106,8 +106,11 @@ struct V4<double> {
 // meshes keep one node per entry, and so do Float32 meshes (16-byte nodes
 // straddle a line in one pair of eight; on 4M particles, 128^3, pairs took
 // the gather from 99 to 134 us).
-// With P = 2 a lane writes one 16-byte quarter of an entry, so the stores
-// are contiguous.
+// The six values of a pair are packed (Ex, Ey, Ez of node i, then of node
+// i + 1, then 16 bytes of zeros), so a gather of both nodes is three
+// 16-byte loads (pair_nodes) instead of four: on 4M particles, 128^3, the
+// gather 215 -> 194 us, the copy 45 -> 48 us.  With P = 2 a lane writes one
+// 16-byte quarter of an entry, so the stores are contiguous.
 template <typename MT>
 __device__ __forceinline__ void interleave_body(int64_t i, const MT *__restrict__ ef,
                                                 typename V4<MT>::type *__restrict__ efi, int64_t M,
@@ -118,13 +121,15 @@ __device__ __forceinline__ void interleave_body(int64_t i, const MT *__restrict_
             if (i >= 4 * M) return;
             const int64_t e = i >> 2;
             const int part = (int)(i & 3);
-            const int64_t node = e + (part >> 1);
-            double2 v = make_double2(0.0, 0.0);
-            if (!((part >> 1) && e % nx == nx - 1)) {
-                if (part & 1) v.x = ef[2 * M + node];
-                else v = make_double2(ef[node], ef[M + node]);
-            }
-            reinterpret_cast<double2 *>(efi)[i] = v;
+            const bool hi = e % nx != nx - 1;  // node e + 1 is in the row
+            // values 2 part and 2 part + 1 of the entry's six: value s is
+            // component s % 3 of node e + s / 3 (no branch per quarter)
+            const int s0 = 2 * part, s1 = s0 + 1;
+            const bool ok0 = s0 < 3 || (s0 < 6 && hi), ok1 = s1 < 3 || (s1 < 6 && hi);
+            const int64_t i0 = ok0 ? (s0 % 3) * M + e + s0 / 3 : 0;
+            const int64_t i1 = ok1 ? (s1 % 3) * M + e + s1 / 3 : 0;
+            const double a = ef[i0], b = ef[i1];
+            reinterpret_cast<double2 *>(efi)[i] = make_double2(ok0 ? a : 0.0, ok1 ? b : 0.0);
             return;
         }
     }
@@ -145,7 +150,19 @@ k_interleave(const MT *__restrict__ ef, typename V4<MT>::type *__restrict__ efi,
     interleave_body<MT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, ef, efi, M, nx, P);
 }
 
-template <typename PT, typename MT>
+// Nodes i and i + 1 of a packed node-pair entry (Float64, P = 2): the
+// first 32 bytes and the next 16, as two one-node entries.
+__device__ __forceinline__ void pair_nodes(const double4 *__restrict__ ent, double4 &lo, double4 &hi)
+{
+    const double4 a = ent[0];
+    const double2 b = *reinterpret_cast<const double2 *>(ent + 1);
+    lo = make_double4(a.x, a.y, a.z, 0.0);
+    hi = make_double4(a.w, b.x, b.y, 0.0);
+}
+
+// PP: the entry layout when known at compile time (1 or 2), else 0 and P
+// gives it (the grouped launches, where it varies by bunch).
+template <typename PT, typename MT, int PP = 0>
 __device__ __forceinline__ void interp_body(
     int64_t p, const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
     const PT *__restrict__ py, const PT *__restrict__ pz, int64_t n, int nx, int ny, int nz,
@@ -155,6 +172,16 @@ __device__ __forceinline__ void interp_body(
 {
 #pragma clang fp contract(off)
     using C = wider_t<PT, MT>;
+    if constexpr (PP == 0) {
+        // the two layouts as separate code, so neither pays for the other's loads
+        if (P == 2)
+            interp_body<PT, MT, 2>(p, efi, px, py, pz, n, nx, ny, nz, lx_, ly_, lz_, dx_, dy_, dz_,
+                                   geom, Ex, Ey, Ez, P);
+        else
+            interp_body<PT, MT, 1>(p, efi, px, py, pz, n, nx, ny, nz, lx_, ly_, lz_, dx_, dy_, dz_,
+                                   geom, Ex, Ey, Ez, P);
+        return;
+    }
     if (p >= n) return;
     device_geometry(geom, lx_, ly_, lz_, dx_, dy_, dz_);
     const C lx = (C)(MT)lx_, ly = (C)(MT)ly_, lz = (C)(MT)lz_;
@@ -175,8 +202,21 @@ __device__ __forceinline__ void interp_body(
     const C w111 = (dx * dy) * dz;
     const int64_t sy = (int64_t)P * nx, sz = (int64_t)P * nx * ny;
     const typename V4<MT>::type *e = efi + P * (ix + (int64_t)nx * iy + (int64_t)nx * ny * iz);
-    const auto n000 = e[0], n100 = e[1], n010 = e[sy], n110 = e[sy + 1];
-    const auto n001 = e[sz], n101 = e[sz + 1], n011 = e[sz + sy], n111 = e[sz + sy + 1];
+    typename V4<MT>::type n000, n100, n010, n110, n001, n101, n011, n111;
+    if constexpr (sizeof(MT) == 8) {
+        if (PP == 2) {
+            pair_nodes(e, n000, n100);
+            pair_nodes(e + sy, n010, n110);
+            pair_nodes(e + sz, n001, n101);
+            pair_nodes(e + sz + sy, n011, n111);
+        } else {
+            n000 = e[0], n100 = e[1], n010 = e[sy], n110 = e[sy + 1];
+            n001 = e[sz], n101 = e[sz + 1], n011 = e[sz + sy], n111 = e[sz + sy + 1];
+        }
+    } else {
+        n000 = e[0], n100 = e[1], n010 = e[sy], n110 = e[sy + 1];
+        n001 = e[sz], n101 = e[sz + 1], n011 = e[sz + sy], n111 = e[sz + sy + 1];
+    }
     // the reference's sum order, per component (src/interpolation.jl:56-85)
 #define SC_ITP(f)                                              \
     ((((((((C)n000.f * w000 + (C)n100.f * w100) + (C)n010.f * w010) + (C)n110.f * w110) + \
@@ -193,7 +233,7 @@ __device__ __forceinline__ void interp_body(
 // only after the first loads had returned, a second memory round trip: on
 // 4M particles, 128^3, the gather went from 240 to 210 us (node pairs) and
 // from 303 to 264 us (one node per entry).
-template <typename PT, typename MT>
+template <typename PT, typename MT, int PP>
 __global__ void __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(1, sizeof(MT) == 8 ? 4 : 8)))
 k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
@@ -202,8 +242,8 @@ k_interp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ p
          const double *__restrict__ geom, PT *__restrict__ Ex, PT *__restrict__ Ey,
          PT *__restrict__ Ez, int P)
 {
-    interp_body<PT, MT>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, efi, px, py, pz, n, nx, ny,
-                        nz, lx_, ly_, lz_, dx_, dy_, dz_, geom, Ex, Ey, Ez, P);
+    interp_body<PT, MT, PP>((int64_t)blockIdx.x * blockDim.x + threadIdx.x, efi, px, py, pz, n, nx,
+                            ny, nz, lx_, ly_, lz_, dx_, dy_, dz_, geom, Ex, Ey, Ez, P);
 }
 
 // ---- vector-Jacobian products of the CIC weights ----
@@ -278,8 +318,9 @@ k_deposit_vjp(const MT *__restrict__ g, const PT *__restrict__ px, const PT *__r
 }
 
 // Position part of the VJP of interpolation at cotangents (gEx, gEy, gEz):
-// g_pos = sum_c gE_c * d/dpos (sum_8 w E_c), from the interleaved field.
-template <typename PT, typename MT>
+// g_pos = sum_c gE_c * d/dpos (sum_8 w E_c), from the interleaved field
+// (entry layout PP, as P).
+template <typename PT, typename MT, int PP>
 __global__ void __launch_bounds__(256)
 k_interp_vjp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict__ px,
              const PT *__restrict__ py, const PT *__restrict__ pz, const PT *__restrict__ gEx,
@@ -301,8 +342,27 @@ k_interp_vjp(const typename V4<MT>::type *__restrict__ efi, const PT *__restrict
     const C dx = nxn - (C)ix, dy = nyn - (C)iy, dz = nzn - (C)iz;
     const int64_t sy = (int64_t)P * nx, sz = (int64_t)P * nx * ny;
     const typename V4<MT>::type *e = efi + P * (ix + (int64_t)nx * iy + (int64_t)nx * ny * iz);
-    const typename V4<MT>::type nd[8] = {e[0],  e[1],      e[sy],      e[sy + 1],
-                                         e[sz], e[sz + 1], e[sz + sy], e[sz + sy + 1]};
+    typename V4<MT>::type nd[8];
+    bool packed = false;
+    if constexpr (sizeof(MT) == 8) {
+        if (PP == 2) {
+            pair_nodes(e, nd[0], nd[1]);
+            pair_nodes(e + sy, nd[2], nd[3]);
+            pair_nodes(e + sz, nd[4], nd[5]);
+            pair_nodes(e + sz + sy, nd[6], nd[7]);
+            packed = true;
+        }
+    }
+    if (!packed) {
+        nd[0] = e[0];
+        nd[1] = e[1];
+        nd[2] = e[sy];
+        nd[3] = e[sy + 1];
+        nd[4] = e[sz];
+        nd[5] = e[sz + 1];
+        nd[6] = e[sz + sy];
+        nd[7] = e[sz + sy + 1];
+    }
     C f[8], val, Gx, Gy, Gz;
     C sx = (C)0, sy_ = (C)0, sz_ = (C)0;
 #pragma unroll
@@ -496,7 +556,8 @@ void itp(Mesh *m, const void *efi, int P, const void *x, const void *y, const vo
          const double lo[3], const double dl[3], void *Ex, void *Ey, void *Ez)
 {
     const LaunchGeo h = launch_geo(m, lo, dl);
-    hipLaunchKernelGGL((k_interp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+    auto kern = P == 2 ? k_interp<PT, MT, 2> : k_interp<PT, MT, 1>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                        m->ctx->stream, static_cast<const typename V4<MT>::type *>(efi),
                        (const PT *)x, (const PT *)y, (const PT *)z, n, (int)m->n[0], (int)m->n[1],
                        (int)m->n[2], h.l[0], h.l[1], h.l[2], h.d[0], h.d[1], h.d[2], h.dev,
@@ -526,7 +587,8 @@ void itp_vjp(Mesh *m, const void *x, const void *y, const void *z, int64_t n, co
              void *gy, void *gz, int accumulate)
 {
     const LaunchGeo h = launch_geo(m, lo, dl);
-    hipLaunchKernelGGL((k_interp_vjp<PT, MT>), dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
+    auto kern = m->efi_pair == 2 ? k_interp_vjp<PT, MT, 2> : k_interp_vjp<PT, MT, 1>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0,
                        m->ctx->stream, static_cast<const typename V4<MT>::type *>(m->efi),
                        (const PT *)x, (const PT *)y, (const PT *)z, (const PT *)gEx, (const PT *)gEy,
                        (const PT *)gEz, n, (int)m->n[0], (int)m->n[1], (int)m->n[2], h.l[0], h.l[1],
