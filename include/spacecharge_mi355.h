@@ -107,8 +107,8 @@ int sc_mesh_force_generic_transforms(sc_mesh *mesh, int enable);
  * field copy that the interpolation and its adjoint gather from.  layout 1
  * holds one (Ex, Ey, Ez, 0) entry per node; layout 2 holds node pairs (nodes
  * i and i+1 along x per entry; Float64 meshes only), which the library
- * picks by itself for Float64 fields of 64 MiB and more; 0 (the default)
- * restores that choice.
+ * picks by itself for every Float64 mesh; 0 (the default) restores that
+ * choice.
  * Results are the same bits in every layout.  Changing it synchronises the
  * context's stream and frees the current copy. */
 int sc_mesh_set_field_layout(sc_mesh *mesh, int layout);

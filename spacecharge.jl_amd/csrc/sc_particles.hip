@@ -101,16 +101,17 @@ struct V4<double> {
 // i + 1 (the next node along x; zero past the row's end, which no cell
 // reads), 64 bytes, so that a cell's two x neighbours never straddle a
 // 128-byte line.  Measured on 4M particles, 128^3: the gather 307 -> 245 us,
-// the copy 26 -> 47 us, the step 0.834 -> 0.792 ms.  On 64x64x128 with a
-// compact bunch the gather did not gain and the copy cost more, so smaller
-// meshes keep one node per entry, and so do Float32 meshes (16-byte nodes
-// straddle a line in one pair of eight; on 4M particles, 128^3, pairs took
-// the gather from 99 to 134 us).
+// the copy 26 -> 47 us, the step 0.834 -> 0.792 ms.  Float32 meshes keep one
+// node per entry (16-byte nodes straddle a line in one pair of eight; on 4M
+// particles, 128^3, pairs took the gather from 99 to 134 us).
 // The six values of a pair are packed (Ex, Ey, Ez of node i, then of node
 // i + 1, then 16 bytes of zeros), so a gather of both nodes is three
 // 16-byte loads (pair_nodes) instead of four: on 4M particles, 128^3, the
-// gather 215 -> 194 us, the copy 45 -> 48 us.  With P = 2 a lane writes one
-// 16-byte quarter of an entry, so the stores are contiguous.
+// gather 215 -> 194 us, the copy 45 -> 48 us.  Packed, pairs also pay on
+// smaller Float64 meshes (1M particles, 64^3: gather 46 -> 36 us, step
+// 0.167 -> 0.161 ms; the 2M cathode bunch on 64x64x128: 103 -> 92 us), so
+// every Float64 mesh takes them.  With P = 2 a lane writes one 16-byte
+// quarter of an entry, so the stores are contiguous.
 template <typename MT>
 __device__ __forceinline__ void interleave_body(int64_t i, const MT *__restrict__ ef,
                                                 typename V4<MT>::type *__restrict__ efi, int64_t M,
@@ -600,10 +601,10 @@ int ensure_efi(Mesh *m)
 {
     if (!m->efi) {
         const int64_t M = m->n[0] * m->n[1] * m->n[2];
-        // node pairs for Float64 fields of 64 MiB and more (see interleave_body),
-        // unless the mesh asks for a layout
+        // node pairs for Float64 fields (see interleave_body), unless the
+        // mesh asks for a layout
         if (m->efi_layout) m->efi_pair = m->efi_layout;
-        else m->efi_pair = (m->dtype == SC_F64 && (size_t)M * 32 >= ((size_t)64 << 20)) ? 2 : 1;
+        else m->efi_pair = m->dtype == SC_F64 ? 2 : 1;
         if (hipMalloc(&m->efi, (size_t)M * 4 * m->dtype * m->efi_pair) != hipSuccess)
             return set_err(m->ctx, SC_ERR_NOMEM, "device allocation of the interleaved field failed");
     }

@@ -312,8 +312,8 @@ def test_batched_edge_sizes_equal_the_loop(nb):
 
 def test_grouped_interpolation_node_pair_layout_matches_oracle():
     """Float64 meshes whose interpolation copy uses node pairs along x (the
-    layout the library picks for Float64 fields of 64 MiB and more, forced
-    here on small meshes) and meshes with one entry per node, in one grouped
+    layout the library picks for Float64 meshes) and meshes forced to one
+    entry per node, in one grouped
     interpolation: every bunch's fields bit-identical to the oracle's
     interpolation of that mesh's own efield, particles on and beyond the
     upper x face included."""
