@@ -130,7 +130,14 @@ __device__ __forceinline__ void interleave_body(int64_t i, const MT *__restrict_
             const int64_t i0 = ok0 ? (s0 % 3) * M + e + s0 / 3 : 0;
             const int64_t i1 = ok1 ? (s1 % 3) * M + e + s1 / 3 : 0;
             const double a = ef[i0], b = ef[i1];
-            reinterpret_cast<double2 *>(efi)[i] = make_double2(ok0 ? a : 0.0, ok1 ? b : 0.0);
+            // streamed past the L2: a pair copy (64 bytes per node) outgrows
+            // it anyway, and the gather then finds it in the last-level cache
+            // (4M particles, 128^3: the copy 47.5 -> 34.1 us, the gather
+            // 194.4 -> 196.8 us).  The padding quarter is written too: left
+            // out, the partial lines made the copy slower (55 us).
+            double *dst = reinterpret_cast<double *>(efi) + 2 * i;
+            __builtin_nontemporal_store(ok0 ? a : 0.0, dst);
+            __builtin_nontemporal_store(ok1 ? b : 0.0, dst + 1);
             return;
         }
     }
